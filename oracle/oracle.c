@@ -1,0 +1,1784 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the reference forward model.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline); the product
+ * library never links or calls this code.  See oracle.h for what pins each
+ * function.  Every function cites the reference src/ file:line whose
+ * arithmetic it restates.  Deviations from the reference, all outside the
+ * reference's fixtures, are marked "DEVIATION" below.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <strings.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "continua_coeffs.inc"
+
+/* ------------------------------------------------------------------------
+   Helpers: error exit, interpolation, GSL 1.15 arithmetic
+   ------------------------------------------------------------------------ */
+
+/* Error exit with the reference's message format (jurassic.h:48-52). */
+#define OFAIL(msg)                                                              \
+  do {                                                                         \
+    printf("\nError (%s, %s, l%d): %s\n\n", __FILE__, __func__, __LINE__, msg); \
+    exit(EXIT_FAILURE);                                                        \
+  } while (0)
+
+/* Linear interpolation, same operation order as LIN (jurassic.h:73-74). */
+static inline double o_lin(double x0, double y0, double x1, double y1, double x) {
+  return y0 + (y1 - y0) / (x1 - x0) * (x - x0);
+}
+
+/* Exponential interpolation falling back to linear (EXP, jurassic.h:55-58). */
+static inline double o_exp(double x0, double y0, double x1, double y1, double x) {
+  if (y0 > 0 && y1 > 0)
+    return y0 * exp(log(y1 / y0) / (x1 - x0) * (x - x0));
+  return o_lin(x0, y0, x1, y1, x);
+}
+
+/* GSL_MIN / GSL_MAX semantics: ((a)<(b)?(a):(b)). */
+static inline double o_min(double a, double b) { return a < b ? a : b; }
+static inline double o_max(double a, double b) { return a > b ? a : b; }
+
+static inline double o_dot(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static inline double o_norm(const double *a) { return sqrt(o_dot(a, a)); }
+static inline double o_dist(const double *a, const double *b) {
+  return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+              (a[2] - b[2]) * (a[2] - b[2]));
+}
+/* ANGLE macro (jurassic.h:32-34). */
+static inline double o_angle(const double *a, const double *b) {
+  return acos(o_min(o_max(o_dot(a, b) / o_norm(a) / o_norm(b), -1), 1));
+}
+
+/* gsl_expm1 of GSL 1.15 (sys/expm1.c): Taylor series below ln 2. */
+static double o_expm1(double x) {
+  if (fabs(x) < M_LN2) {
+    double i = 1.0, sum = x, term = x / 1.0;
+    do {
+      i++;
+      term *= x / i;
+      sum += term;
+    } while (fabs(term) > fabs(sum) * 2.2204460492503131e-16);
+    return sum;
+  }
+  return exp(x) - 1;
+}
+
+/* gsl_log1p of GSL 1.15 (sys/log1p.c). */
+static double o_log1p(double x) {
+  volatile double y, z;
+  y = 1 + x;
+  z = y - 1;
+  return log(y) - (z - x) / y;
+}
+
+/* Bisection over a monotone double array (misc.c:237-265).  The direction
+   is decided from xx[mid] and xx[mid+1] only. */
+static int o_locate(const double *xx, int n, double x) {
+  int ilo = 0, ihi = n - 1, i = (ihi + ilo) >> 1;
+  if (xx[i] < xx[i + 1]) {
+    while (ihi > ilo + 1) {
+      i = (ihi + ilo) >> 1;
+      if (xx[i] > x) ihi = i; else ilo = i;
+    }
+  } else {
+    while (ihi > ilo + 1) {
+      i = (ihi + ilo) >> 1;
+      if (xx[i] <= x) ihi = i; else ilo = i;
+    }
+  }
+  return ilo;
+}
+
+/* Bisection over an increasing float table column (forwardmodel.c:546-565). */
+static int o_locate_tbl(const float *xx, int n, double x) {
+  int ilo = 0, ihi = n - 1, i;
+  while (ihi > ilo + 1) {
+    i = (ihi + ilo) >> 1;
+    if (xx[i] > x) ihi = i; else ilo = i;
+  }
+  return ilo;
+}
+
+/* Spherical Earth <-> Cartesian (misc.c:5-16, 61-72). */
+static void o_cart2geo(const double *x, double *z, double *lon, double *lat) {
+  double radius = o_norm(x);
+  *lat = asin(x[2] / radius) * 180 / M_PI;
+  *lon = atan2(x[1], x[0]) * 180 / M_PI;
+  *z = radius - RE;
+}
+static void o_geo2cart(double z, double lon, double lat, double *x) {
+  double radius = z + RE;
+  x[0] = radius * cos(lat / 180 * M_PI) * cos(lon / 180 * M_PI);
+  x[1] = radius * cos(lat / 180 * M_PI) * sin(lon / 180 * M_PI);
+  x[2] = radius * sin(lat / 180 * M_PI);
+}
+
+/* Case-insensitive emitter lookup (atmosphere.c:139-149). */
+static int o_find_emitter(const ctl_t *ctl, const char *name) {
+  for (int ig = 0; ig < ctl->ng; ig++)
+    if (strcasecmp(ctl->emitter[ig], name) == 0) return ig;
+  return -1;
+}
+
+/* ------------------------------------------------------------------------
+   Control file (control.c:7-171)
+   ------------------------------------------------------------------------ */
+
+double oracle_scan_ctl(int argc, char *argv[], const char *varname, int arridx,
+                       const char *defvalue, char *value) {
+  FILE *in = NULL;
+  char dummy[LEN], name1[LEN], name2[LEN], line[LEN], rname[LEN], rval[LEN];
+  int found = 0;
+  if (argv[1][0] != '-')
+    if (!(in = fopen(argv[1], "r"))) OFAIL("Cannot open file!");
+  if (arridx >= 0) {
+    sprintf(name1, "%s[%d]", varname, arridx);
+    sprintf(name2, "%s[*]", varname);
+  } else {
+    sprintf(name1, "%s", varname);
+    sprintf(name2, "%s", varname);
+  }
+  if (in != NULL)
+    while (fgets(line, LEN, in))
+      if (sscanf(line, "%s %s %s", rname, dummy, rval) == 3)
+        if (strcasecmp(rname, name1) == 0 || strcasecmp(rname, name2) == 0) {
+          found = 1;
+          break;
+        }
+  for (int i = 1; i < argc - 1; i++)
+    if (strcasecmp(argv[i], name1) == 0 || strcasecmp(argv[i], name2) == 0) {
+      sprintf(rval, "%s", argv[i + 1]);
+      found = 1;
+      break;
+    }
+  if (in != NULL) fclose(in);
+  if (!found) {
+    if (strlen(defvalue) > 0)
+      sprintf(rval, "%s", defvalue);
+    else {
+      char msg[2 * LEN];
+      sprintf(msg, "Missing variable %s!\n", name1);
+      OFAIL(msg);
+    }
+  }
+  if (value != NULL) sprintf(value, "%s", rval);
+  return atof(rval);
+}
+
+void oracle_read_ctl(int argc, char *argv[], ctl_t *ctl) {
+  ctl->ng = (int)oracle_scan_ctl(argc, argv, "NG", -1, "0", NULL);
+  if (ctl->ng < 0 || ctl->ng > NGMAX) OFAIL("Set 0 <= NG <= MAX!");
+  for (int ig = 0; ig < ctl->ng; ig++)
+    oracle_scan_ctl(argc, argv, "EMITTER", ig, "", ctl->emitter[ig]);
+  ctl->nd = (int)oracle_scan_ctl(argc, argv, "ND", -1, "0", NULL);
+  if (ctl->nd < 0 || ctl->nd > NDMAX) OFAIL("Set 0 <= ND <= MAX!");
+  for (int id = 0; id < ctl->nd; id++)
+    ctl->nu[id] = oracle_scan_ctl(argc, argv, "NU", id, "", NULL);
+  ctl->nw = (int)oracle_scan_ctl(argc, argv, "NW", -1, "1", NULL);
+  if (ctl->nw < 0 || ctl->nw > NWMAX) OFAIL("Set 0 <= NW <= NWMAX!");
+  for (int id = 0; id < ctl->nd; id++)
+    ctl->window[id] = (int)oracle_scan_ctl(argc, argv, "WINDOW", id, "0", NULL);
+  oracle_scan_ctl(argc, argv, "TBLBASE", -1, "-", ctl->tblbase);
+  ctl->hydz = oracle_scan_ctl(argc, argv, "HYDZ", -1, "-999", NULL);
+  ctl->ctm_co2 = (int)oracle_scan_ctl(argc, argv, "CTM_CO2", -1, "1", NULL);
+  ctl->ctm_h2o = (int)oracle_scan_ctl(argc, argv, "CTM_H2O", -1, "1", NULL);
+  ctl->ctm_n2 = (int)oracle_scan_ctl(argc, argv, "CTM_N2", -1, "1", NULL);
+  ctl->ctm_o2 = (int)oracle_scan_ctl(argc, argv, "CTM_O2", -1, "1", NULL);
+  ctl->sca_n = (int)oracle_scan_ctl(argc, argv, "SCA_N", -1, "0", NULL);
+  if (ctl->sca_n < 0 || ctl->sca_n > SCAMOD) OFAIL("Set 0 <= SCA_NMOD <= MAX!");
+  ctl->sca_mult = (int)oracle_scan_ctl(argc, argv, "SCA_MULT", -1, "1", NULL);
+  oracle_scan_ctl(argc, argv, "SCA_EXT", -1, "beta_a", ctl->sca_ext);
+  ctl->ip = (int)oracle_scan_ctl(argc, argv, "IP", -1, "1", NULL);
+  ctl->cz = oracle_scan_ctl(argc, argv, "CZ", -1, "0", NULL);
+  ctl->cx = oracle_scan_ctl(argc, argv, "CX", -1, "0", NULL);
+  ctl->refrac = (int)oracle_scan_ctl(argc, argv, "REFRAC", -1, "1", NULL);
+  ctl->rayds = oracle_scan_ctl(argc, argv, "RAYDS", -1, "10", NULL);
+  ctl->raydz = oracle_scan_ctl(argc, argv, "RAYDZ", -1, "1", NULL);
+  ctl->transs = oracle_scan_ctl(argc, argv, "TRANSS", -1, "0.02", NULL);
+  oracle_scan_ctl(argc, argv, "FOV", -1, "-", ctl->fov);
+  ctl->retp_zmin = oracle_scan_ctl(argc, argv, "RETP_ZMIN", -1, "-999", NULL);
+  ctl->retp_zmax = oracle_scan_ctl(argc, argv, "RETP_ZMAX", -1, "-999", NULL);
+  ctl->rett_zmin = oracle_scan_ctl(argc, argv, "RETT_ZMIN", -1, "-999", NULL);
+  ctl->rett_zmax = oracle_scan_ctl(argc, argv, "RETT_ZMAX", -1, "-999", NULL);
+  for (int ig = 0; ig < ctl->ng; ig++) {
+    ctl->retq_zmin[ig] = oracle_scan_ctl(argc, argv, "RETQ_ZMIN", ig, "-999", NULL);
+    ctl->retq_zmax[ig] = oracle_scan_ctl(argc, argv, "RETQ_ZMAX", ig, "-999", NULL);
+  }
+  for (int iw = 0; iw < ctl->nw; iw++) {
+    ctl->retk_zmin[iw] = oracle_scan_ctl(argc, argv, "RETK_ZMIN", iw, "-999", NULL);
+    ctl->retk_zmax[iw] = oracle_scan_ctl(argc, argv, "RETK_ZMAX", iw, "-999", NULL);
+  }
+  ctl->retnn = (int)oracle_scan_ctl(argc, argv, "RETNN", -1, "0", NULL);
+  ctl->retrr = (int)oracle_scan_ctl(argc, argv, "RETRR", -1, "0", NULL);
+  ctl->retss = (int)oracle_scan_ctl(argc, argv, "RETSS", -1, "0", NULL);
+  ctl->retnn_zmin = (int)oracle_scan_ctl(argc, argv, "RETNN_ZMIN", -1, "-999", NULL);
+  ctl->retnn_zmax = (int)oracle_scan_ctl(argc, argv, "RETNN_ZMAX", -1, "-999", NULL);
+  ctl->write_bbt = (int)oracle_scan_ctl(argc, argv, "WRITE_BBT", -1, "0", NULL);
+  ctl->write_matrix = (int)oracle_scan_ctl(argc, argv, "WRITE_MATRIX", -1, "0", NULL);
+  ctl->queue.capacity = (int)oracle_scan_ctl(argc, argv, "MAX_QUEUE", -1, "-1", NULL);
+  ctl->queue.items = NULL;
+  ctl->queue.begin = ctl->queue.end = 0;
+  ctl->queue.state = Queue_Inactive;
+}
+
+/* ------------------------------------------------------------------------
+   File I/O
+   ------------------------------------------------------------------------ */
+
+static void o_path(char *file, const char *dirname, const char *filename) {
+  if (dirname != NULL) sprintf(file, "%s/%s", dirname, filename);
+  else sprintf(file, "%s", filename);
+}
+
+/* Whitespace tokeniser with the TOK macro's semantics (jurassic.h:93-98):
+   a token that fails to parse skips the line; a missing token is an error. */
+static char *o_tok(char *line, int *bad, const char *fmt, void *var) {
+  char *tok = strtok(line, " \t");
+  if (!tok) OFAIL("Error while reading!");
+  if (sscanf(tok, fmt, var) != 1) *bad = 1;
+  return tok;
+}
+
+/* atmosphere.c:487-541 */
+void oracle_read_atm(const char *dirname, const char *filename, ctl_t *ctl, atm_t *atm) {
+  char file[LEN], line[LEN];
+  FILE *in;
+  atm->init = 0;
+  atm->np = 0;
+  o_path(file, dirname, filename);
+  if (!(in = fopen(file, "r"))) OFAIL("Cannot open file!");
+  while (fgets(line, LEN, in)) {
+    int bad = 0, n = atm->np;
+    o_tok(line, &bad, "%lg", &atm->time[n]);
+    if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &atm->z[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &atm->lon[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &atm->lat[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &atm->p[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &atm->t[n]); if (bad) continue;
+    for (int ig = 0; ig < ctl->ng && !bad; ig++) o_tok(NULL, &bad, "%lg", &atm->q[ig][n]);
+    if (bad) continue;
+    for (int iw = 0; iw < ctl->nw && !bad; iw++) o_tok(NULL, &bad, "%lg", &atm->k[iw][n]);
+    if (bad) continue;
+    if ((++atm->np) > NPMAX) OFAIL("Too many data points!");
+  }
+  fclose(in);
+  if (atm->np < 1) OFAIL("Could not read any data!");
+}
+
+/* misc.c:269-326 */
+void oracle_read_obs(const char *dirname, const char *filename, ctl_t *ctl, obs_t *obs) {
+  char file[LEN], line[LEN];
+  FILE *in;
+  obs->nr = 0;
+  o_path(file, dirname, filename);
+  if (!(in = fopen(file, "r"))) OFAIL("Cannot open file!");
+  while (fgets(line, LEN, in)) {
+    int bad = 0, n = obs->nr;
+    o_tok(line, &bad, "%lg", &obs->time[n]);
+    if (bad) continue;
+    double *cols[9] = {&obs->obsz[n], &obs->obslon[n], &obs->obslat[n], &obs->vpz[n],
+                       &obs->vplon[n], &obs->vplat[n], &obs->tpz[n], &obs->tplon[n],
+                       &obs->tplat[n]};
+    for (int c = 0; c < 9 && !bad; c++) o_tok(NULL, &bad, "%lg", cols[c]);
+    for (int id = 0; id < ctl->nd && !bad; id++) o_tok(NULL, &bad, "%lg", &obs->rad[id][n]);
+    for (int id = 0; id < ctl->nd && !bad; id++) o_tok(NULL, &bad, "%lg", &obs->tau[id][n]);
+    if (bad) continue;
+    if ((++obs->nr) > NRMAX) OFAIL("Too many rays!");
+  }
+  fclose(in);
+  if (obs->nr < 1) OFAIL("Could not read any data!");
+}
+
+/* misc.c:330-390 */
+void oracle_write_obs(const char *dirname, const char *filename, ctl_t *ctl, obs_t *obs) {
+  char file[LEN];
+  FILE *out;
+  int n = 10;
+  o_path(file, dirname, filename);
+  if (!(out = fopen(file, "w"))) OFAIL("Cannot create file!");
+  fprintf(out,
+          "# $1 = time (seconds since 2000-01-01T00:00Z)\n"
+          "# $2 = observer altitude [km]\n"
+          "# $3 = observer longitude [deg]\n"
+          "# $4 = observer latitude [deg]\n"
+          "# $5 = view point altitude [km]\n"
+          "# $6 = view point longitude [deg]\n"
+          "# $7 = view point latitude [deg]\n"
+          "# $8 = tangent point altitude [km]\n"
+          "# $9 = tangent point longitude [deg]\n"
+          "# $10 = tangent point latitude [deg]\n");
+  for (int id = 0; id < ctl->nd; id++)
+    fprintf(out, "# $%d = channel %g: radiance [W/(m^2 sr cm^-1)]\n", ++n, ctl->nu[id]);
+  for (int id = 0; id < ctl->nd; id++)
+    fprintf(out, "# $%d = channel %g: transmittance\n", ++n, ctl->nu[id]);
+  for (int ir = 0; ir < obs->nr; ir++) {
+    if (ir == 0 || obs->time[ir] != obs->time[ir - 1]) fprintf(out, "\n");
+    fprintf(out, "%.2f %g %g %g %g %g %g %g %g %g", obs->time[ir], obs->obsz[ir],
+            obs->obslon[ir], obs->obslat[ir], obs->vpz[ir], obs->vplon[ir], obs->vplat[ir],
+            obs->tpz[ir], obs->tplon[ir], obs->tplat[ir]);
+    for (int id = 0; id < ctl->nd; id++) fprintf(out, " %g", obs->rad[id][ir]);
+    for (int id = 0; id < ctl->nd; id++) fprintf(out, " %g", obs->tau[id][ir]);
+    fprintf(out, "\n");
+  }
+  fclose(out);
+}
+
+/* scatter.c:583-639 */
+void oracle_read_aero(const char *dirname, const char *filename, ctl_t *ctl, aero_t *aero) {
+  char file[LEN], line[LEN];
+  FILE *in;
+  aero->nm = 0;
+  o_path(file, dirname, filename);
+  if (!(in = fopen(file, "r"))) OFAIL("Cannot open file!");
+  while (fgets(line, LEN, in)) {
+    int bad = 0, n = aero->nm;
+    o_tok(line, &bad, "%lg", &aero->top_mod[n]);
+    if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &aero->bottom_mod[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &aero->trans_mod[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%s", aero->type[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%s", aero->filepath[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &aero->nn[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &aero->rr[n]); if (bad) continue;
+    o_tok(NULL, &bad, "%lg", &aero->ss[n]); if (bad) continue;
+    if ((++aero->nm) > SCAMOD) OFAIL("Too many aerosol models!");
+  }
+  fclose(in);
+  if (aero->nm < 1) OFAIL("Could not read any data!");
+  if (ctl->sca_n != aero->nm)
+    OFAIL("Number of scattering models in control file and aerosol file does not match.");
+}
+
+/* read_shape (forwardmodel.c:577-606): two-column text file. */
+static void o_read_shape(const char *filename, double *x, double *y, int *n) {
+  FILE *in;
+  char line[LEN];
+  if (!(in = fopen(filename, "r"))) OFAIL("Cannot open file!");
+  *n = 0;
+  while (fgets(line, LEN, in))
+    if (sscanf(line, "%lg %lg", &x[*n], &y[*n]) == 2)
+      if ((++(*n)) > NSHAPE) OFAIL("Too many data points!");
+  if (*n < 1) OFAIL("Could not read any data!");
+  fclose(in);
+}
+
+/* ------------------------------------------------------------------------
+   Emissivity tables and source-function tables (cached per table set)
+   ------------------------------------------------------------------------ */
+
+typedef struct {
+  int np;
+  double p[TBLNPMAX];
+  int nt[TBLNPMAX];
+  double t[TBLNPMAX][TBLNTMAX];
+  int nu[TBLNPMAX][TBLNTMAX];
+  float *u[TBLNPMAX][TBLNTMAX];
+  float *eps[TBLNPMAX][TBLNTMAX];
+} otab_t;
+
+static struct {
+  int ready;
+  char tblbase[LEN];
+  int ng, nd;
+  char emitter[NGMAX][64];
+  double nu[NDMAX];
+  otab_t *tab;                     /* [ng][nd] */
+  double st[TBLNSMAX];             /* source-function temperatures */
+  double sr[NDMAX][TBLNSMAX];      /* filter-weighted Planck radiances */
+} OT;
+
+static void o_free_tables(void) {
+  if (OT.tab) {
+    for (int k = 0; k < OT.ng * OT.nd; k++)
+      for (int ip = 0; ip < TBLNPMAX; ip++)
+        for (int it = 0; it < TBLNTMAX; it++) {
+          free(OT.tab[k].u[ip][it]);
+          free(OT.tab[k].eps[ip][it]);
+        }
+    free(OT.tab);
+  }
+  OT.tab = NULL;
+  OT.ready = 0;
+}
+
+void oracle_reset(void) { o_free_tables(); }
+
+/* Binary table reader (forwardmodel.c:630-656).
+   DEVIATION: a column longer than TBLNUMAX is cut to TBLNUMAX entries
+   (the reference would read out of bounds). */
+static int o_read_tbl_bin(const char *filename, otab_t *tb) {
+  FILE *in = fopen(filename, "r");
+  if (!in) return 0;
+#define ORD(ptr, type, n) \
+  if (fread(ptr, sizeof(type), (size_t)(n), in) != (size_t)(n)) OFAIL("Error while reading!");
+  ORD(&tb->np, int, 1);
+  if (tb->np > TBLNPMAX) OFAIL("Too many pressure levels!");
+  ORD(tb->p, double, tb->np);
+  ORD(tb->nt, int, tb->np);
+  for (int ip = 0; ip < tb->np; ip++) {
+    if (tb->nt[ip] > TBLNTMAX) OFAIL("Too many temperatures!");
+    ORD(tb->t[ip], double, tb->nt[ip]);
+    ORD(tb->nu[ip], int, tb->nt[ip]);
+    for (int it = 0; it < tb->nt[ip]; it++) {
+      int n = tb->nu[ip][it] < TBLNUMAX ? tb->nu[ip][it] : TBLNUMAX;
+      if (n < 0) n = 0;
+      tb->u[ip][it] = (float *)calloc((size_t)n + 2, sizeof(float));
+      tb->eps[ip][it] = (float *)calloc((size_t)n + 2, sizeof(float));
+      ORD(tb->u[ip][it], float, n);
+      ORD(tb->eps[ip][it], float, n);
+      tb->nu[ip][it] = n;
+    }
+  }
+#undef ORD
+  fclose(in);
+  return 1;
+}
+
+/* ASCII table reader (forwardmodel.c:658-737): keeps a row only if both u
+   and eps increase within the current (p, T) column. */
+static int o_read_tbl_ascii(const char *filename, otab_t *tb) {
+  FILE *in = fopen(filename, "r");
+  char line[LEN];
+  double press, temp, u, eps, eps_old = -999, press_old = -999, temp_old = -999, u_old = -999;
+  static float bu[TBLNPMAX][TBLNTMAX][TBLNUMAX], be[TBLNPMAX][TBLNTMAX][TBLNUMAX];
+  if (!in) return 0;
+  tb->np = -1;
+  while (fgets(line, LEN, in)) {
+    if (sscanf(line, "%lg %lg %lg %lg", &press, &temp, &u, &eps) != 4) continue;
+    if (press != press_old) {
+      press_old = press;
+      if ((++tb->np) >= TBLNPMAX) OFAIL("Too many pressure levels!");
+      tb->nt[tb->np] = -1;
+    }
+    int ip = tb->np;
+    if (temp != temp_old) {
+      temp_old = temp;
+      if ((++tb->nt[ip]) >= TBLNTMAX) OFAIL("Too many temperatures!");
+      tb->nu[ip][tb->nt[ip]] = -1;
+    }
+    int it = tb->nt[ip];
+    if ((eps > eps_old && u > u_old) || tb->nu[ip][it] < 0) {
+      eps_old = eps;
+      u_old = u;
+      if ((++tb->nu[ip][it]) >= TBLNUMAX) {
+        tb->nu[ip][it]--;
+        continue;
+      }
+    }
+    tb->p[ip] = press;
+    tb->t[ip][it] = temp;
+    bu[ip][it][tb->nu[ip][it]] = (float)u;
+    be[ip][it][tb->nu[ip][it]] = (float)eps;
+  }
+  fclose(in);
+  tb->np++;
+  for (int ip = 0; ip < tb->np; ip++) {
+    tb->nt[ip]++;
+    for (int it = 0; it < tb->nt[ip]; it++) {
+      tb->nu[ip][it]++;
+      int n = tb->nu[ip][it];
+      tb->u[ip][it] = (float *)calloc((size_t)n + 2, sizeof(float));
+      tb->eps[ip][it] = (float *)calloc((size_t)n + 2, sizeof(float));
+      memcpy(tb->u[ip][it], bu[ip][it], (size_t)n * sizeof(float));
+      memcpy(tb->eps[ip][it], be[ip][it], (size_t)n * sizeof(float));
+    }
+  }
+  return 1;
+}
+
+/* Load all tables and build the source-function tables (forwardmodel.c:
+   610-739 and 743-786).  The reference freezes them after its first call;
+   the oracle reloads when the table set named by ctl changes. */
+static void o_init_tables(ctl_t *ctl) {
+  int same = OT.ready && strcmp(OT.tblbase, ctl->tblbase) == 0 && OT.ng == ctl->ng &&
+             OT.nd == ctl->nd;
+  for (int ig = 0; same && ig < ctl->ng; ig++) same = strcmp(OT.emitter[ig], ctl->emitter[ig]) == 0;
+  for (int id = 0; same && id < ctl->nd; id++) same = OT.nu[id] == ctl->nu[id];
+  if (same) return;
+  o_free_tables();
+  snprintf(OT.tblbase, LEN, "%s", ctl->tblbase);
+  OT.ng = ctl->ng;
+  OT.nd = ctl->nd;
+  for (int ig = 0; ig < ctl->ng; ig++) snprintf(OT.emitter[ig], 64, "%s", ctl->emitter[ig]);
+  for (int id = 0; id < ctl->nd; id++) OT.nu[id] = ctl->nu[id];
+  OT.tab = (otab_t *)calloc((size_t)(ctl->ng * ctl->nd > 0 ? ctl->ng * ctl->nd : 1), sizeof(otab_t));
+  char filename[2 * LEN];
+  for (int ig = 0; ig < ctl->ng; ig++)
+    for (int id = 0; id < ctl->nd; id++) {
+      otab_t *tb = &OT.tab[ig * ctl->nd + id];
+      sprintf(filename, "%s_%.4f_%s.bin", ctl->tblbase, ctl->nu[id], ctl->emitter[ig]);
+      if (o_read_tbl_bin(filename, tb)) continue;
+      sprintf(filename, "%s_%.4f_%s.tab", ctl->tblbase, ctl->nu[id], ctl->emitter[ig]);
+      if (!o_read_tbl_ascii(filename, tb)) {
+        printf("Missing emissivity table: %s\n", filename);
+        tb->np = -1; /* the reference leaves np at -1 for a missing table */
+      }
+    }
+  /* Source function: 1201 temperatures 100..400 K, filter-weighted Planck. */
+  static double fnu[NSHAPE], f[NSHAPE];
+  for (int it = 0; it < TBLNSMAX; it++)
+    OT.st[it] = o_lin(0.0, 100.0, TBLNSMAX - 1.0, 400.0, (double)it);
+  for (int id = 0; id < ctl->nd; id++) {
+    int n;
+    sprintf(filename, "%s_%.4f.filt", ctl->tblbase, ctl->nu[id]);
+    o_read_shape(filename, fnu, f, &n);
+    for (int it = 0; it < TBLNSMAX; it++) {
+      double fsum = 0, acc = 0;
+      for (int i = 0; i < n; i++) {
+        fsum += f[i];
+        acc += f[i] * oracle_planck(OT.st[it], fnu[i]);
+      }
+      OT.sr[id][it] = acc / fsum;
+    }
+  }
+  OT.ready = 1;
+}
+
+/* forwardmodel.c:569-573 */
+double oracle_planck(double t, double nu) { return C1 * (nu * nu * nu) / o_expm1(C2 * nu / t); }
+
+/* forwardmodel.c:7-11 */
+double oracle_brightness(double rad, double nu) { return C2 * nu / o_log1p(C1 * (nu * nu * nu) / rad); }
+
+/* forwardmodel.c:788-793: locate + linear interpolation in the table. */
+void oracle_srcfunc_planck(ctl_t *ctl, double t, double *src) {
+  int it = o_locate(OT.st, TBLNSMAX, t);
+  for (int id = 0; id < ctl->nd; id++)
+    src[id] = o_lin(OT.st[it], OT.sr[id][it], OT.st[it + 1], OT.sr[id][it + 1], t);
+}
+
+/* LIN on float table nodes: under C's usual arithmetic conversions the
+   reference's macro (jurassic.h:73-74) forms the differences y1-y0, x1-x0
+   and their quotient in single precision; only the offset x-x0 and the final
+   sum are double. */
+static inline double o_lin_f(float x0, float y0, float x1, float y1, double x) {
+  float slope = (y1 - y0) / (x1 - x0);
+  return (double)y0 + (double)slope * (x - (double)x0);
+}
+
+/* intpol_tbl_eps / intpol_tbl_u (forwardmodel.c:506-542). */
+static double o_tbl_eps(const otab_t *tb, int ip, int it, double u) {
+  const float *cu = tb->u[ip][it], *ce = tb->eps[ip][it];
+  int i = o_locate_tbl(cu, tb->nu[ip][it], u);
+  return o_lin_f(cu[i], ce[i], cu[i + 1], ce[i + 1], u);
+}
+static double o_tbl_u(const otab_t *tb, int ip, int it, double eps) {
+  const float *cu = tb->u[ip][it], *ce = tb->eps[ip][it];
+  int i = o_locate_tbl(ce, tb->nu[ip][it], eps);
+  return o_lin_f(ce[i], cu[i], ce[i + 1], cu[i + 1], eps);
+}
+
+/* Emissivity growth approximation for one LOS point (forwardmodel.c:409-502).
+   tau_path[ig][id] carries the path transmittance per emitter. */
+static void o_intpol_tbl(const ctl_t *ctl, double p, double t, const double *u, int first,
+                         double tau_path[NGMAX][NDMAX], double *tau_seg) {
+  if (first)
+    for (int ig = 0; ig < ctl->ng; ig++)
+      for (int id = 0; id < ctl->nd; id++) tau_path[ig][id] = 1;
+  for (int id = 0; id < ctl->nd; id++) {
+    tau_seg[id] = 1;
+    for (int ig = 0; ig < ctl->ng; ig++) {
+      const otab_t *tb = &OT.tab[ig * ctl->nd + id];
+      double eps;
+      if (tb->np < 2)
+        eps = 0;
+      else if (tau_path[ig][id] < 1e-9)
+        eps = 1;
+      else {
+        int ipr = o_locate(tb->p, tb->np, p);
+        int it0 = o_locate(tb->t[ipr], tb->nt[ipr], t);
+        int it1 = o_locate(tb->t[ipr + 1], tb->nt[ipr + 1], t);
+        if (tb->nt[ipr] < 2 || tb->nt[ipr + 1] < 2 || tb->nu[ipr][it0] < 2 ||
+            tb->nu[ipr][it0 + 1] < 2 || tb->nu[ipr + 1][it1] < 2 || tb->nu[ipr + 1][it1 + 1] < 2)
+          eps = 0;
+        else {
+          double e_path = 1 - tau_path[ig][id], uu, e00, e01, e10, e11;
+          uu = o_tbl_u(tb, ipr, it0, e_path);
+          e00 = o_max(o_min(o_tbl_eps(tb, ipr, it0, uu + u[ig]), 1), 0);
+          uu = o_tbl_u(tb, ipr, it0 + 1, e_path);
+          e01 = o_max(o_min(o_tbl_eps(tb, ipr, it0 + 1, uu + u[ig]), 1), 0);
+          uu = o_tbl_u(tb, ipr + 1, it1, e_path);
+          e10 = o_max(o_min(o_tbl_eps(tb, ipr + 1, it1, uu + u[ig]), 1), 0);
+          uu = o_tbl_u(tb, ipr + 1, it1 + 1, e_path);
+          e11 = o_max(o_min(o_tbl_eps(tb, ipr + 1, it1 + 1, uu + u[ig]), 1), 0);
+          e00 = o_max(o_min(o_lin(tb->t[ipr][it0], e00, tb->t[ipr][it0 + 1], e01, t), 1), 0);
+          e11 = o_max(o_min(o_lin(tb->t[ipr + 1][it1], e10, tb->t[ipr + 1][it1 + 1], e11, t), 1), 0);
+          e00 = o_max(o_min(o_lin(tb->p[ipr], e00, tb->p[ipr + 1], e11, p), 1), 0);
+          eps = 1 - (1 - e00) / tau_path[ig][id];
+        }
+      }
+      tau_path[ig][id] *= (1 - eps);
+      tau_seg[id] *= (1 - eps);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------
+   Continua (continua.c; coefficient data in continua_coeffs.inc)
+   ------------------------------------------------------------------------ */
+
+/* continua.c:5-872 (ctmco2): returns the optical depth of the segment. */
+static double o_ctmco2(const ctl_t *ctl, double nu, double p, double t, const double *u) {
+  int ig = o_find_emitter(ctl, "CO2");
+  if (ig < 0) return 0;
+  double xw = nu / 2 + 1;
+  if (!(xw >= 1 && xw < 2001)) return 0;
+  int iw = (int)xw, jw = iw + 1;
+  double dw = xw - iw, ew = 1 - dw;
+  double cw296 = ew * JUR_CTM_CO2_296[iw - 1] + dw * JUR_CTM_CO2_296[jw - 1];
+  double cw260 = ew * JUR_CTM_CO2_260[iw - 1] + dw * JUR_CTM_CO2_260[jw - 1];
+  double cw230 = ew * JUR_CTM_CO2_230[iw - 1] + dw * JUR_CTM_CO2_230[jw - 1];
+  double dt230 = t - 230, dt260 = t - 260, dt296 = t - 296;
+  double ctw = dt260 * 5.050505e-4 * dt296 * cw230 - dt230 * 9.259259e-4 * dt296 * cw260 +
+               dt230 * 4.208754e-4 * dt260 * cw296;
+  return u[ig] / JUR_AVOGADRO / 1000 * p / P0 * ctw;
+}
+
+/* continua.c:876-1931 (ctmh2o) */
+static double o_ctmh2o(const ctl_t *ctl, double nu, double p, double t, const double *q,
+                       const double *u) {
+  int ig = o_find_emitter(ctl, "H2O");
+  if (ig < 0) return 0;
+  double xw = nu / 10 + 1;
+  if (!(xw >= 1 && xw < 2001)) return 0;
+  int iw = (int)xw, jw = iw + 1;
+  double dw = xw - iw, ew = 1 - dw, sfac;
+  double cw296 = ew * JUR_CTM_H2O_SELF296[iw - 1] + dw * JUR_CTM_H2O_SELF296[jw - 1];
+  double cw260 = ew * JUR_CTM_H2O_SELF260[iw - 1] + dw * JUR_CTM_H2O_SELF260[jw - 1];
+  double cwfrn = ew * JUR_CTM_H2O_FOREIGN[iw - 1] + dw * JUR_CTM_H2O_FOREIGN[jw - 1];
+  if (nu <= 820 || nu >= 960)
+    sfac = 1;
+  else {
+    double xx = (nu - 820) / 10;
+    int ix = (int)xx;
+    double dx = xx - ix;
+    sfac = (1 - dx) * JUR_CTM_H2O_XFCREV[ix] + dx * JUR_CTM_H2O_XFCREV[ix + 1];
+  }
+  double ctwslf = sfac * cw296 * pow(cw260 / cw296, (296 - t) / (296 - 260));
+  double vf2 = (nu - 370) * (nu - 370);
+  double vf6 = vf2 * vf2 * vf2;
+  double fscal = 36100 / (vf2 + vf6 * 1e-8 + 36100) * -.25 + 1;
+  double ctwfrn = cwfrn * fscal;
+  double a1 = nu * u[ig] * tanh(.7193876 / t * nu);
+  double a2 = 296 / t;
+  double a3 = p / P0 * (q[ig] * ctwslf + (1 - q[ig]) * ctwfrn) * 1e-20;
+  return a1 * a2 * a3;
+}
+
+/* continua.c:1935-1993 (ctmn2): absorption coefficient [1/km]. */
+double oracle_ctmn2(double nu, double p, double t) {
+  const double *nua = JUR_CTM_N2_NU;
+  if (nu < nua[0] || nu > nua[97]) return 0;
+  int idx = o_locate(nua, 98, nu);
+  double b = o_lin(nua[idx], JUR_CTM_N2_B[idx], nua[idx + 1], JUR_CTM_N2_B[idx + 1], nu);
+  double beta = o_lin(nua[idx], JUR_CTM_N2_BETA[idx], nua[idx + 1], JUR_CTM_N2_BETA[idx + 1], nu);
+  double t0 = 273, tr = 296, q_n2 = 0.79;
+  return 0.1 * pow(p / P0, 2) * pow(t0 / t, 2) * q_n2 * b * exp(beta * (1 / tr - 1 / t)) *
+         (q_n2 + (1 - q_n2) * (1.294 - 0.4545 * t / tr));
+}
+
+/* continua.c:1997-2049 (ctmo2) */
+double oracle_ctmo2(double nu, double p, double t) {
+  const double *nua = JUR_CTM_O2_NU;
+  if (nu < nua[0] || nu > nua[89]) return 0;
+  int idx = o_locate(nua, 90, nu);
+  double b = o_lin(nua[idx], JUR_CTM_O2_B[idx], nua[idx + 1], JUR_CTM_O2_B[idx + 1], nu);
+  double beta = o_lin(nua[idx], JUR_CTM_O2_BETA[idx], nua[idx + 1], JUR_CTM_O2_BETA[idx + 1], nu);
+  double t0 = 273, tr = 296, q_o2 = 0.21;
+  return 0.1 * pow(p / P0, 2) * pow(t0 / t, 2) * q_o2 * b * exp(beta * (1 / tr - 1 / t));
+}
+
+/* formod_continua (forwardmodel.c:120-152). */
+void oracle_continua(ctl_t *ctl, double p, double t, const double *q, const double *u,
+                     double ds, const double *k, double *beta) {
+  for (int id = 0; id < ctl->nd; id++) beta[id] = k[ctl->window[id]];
+  if (ctl->ctm_co2)
+    for (int id = 0; id < ctl->nd; id++) beta[id] += o_ctmco2(ctl, ctl->nu[id], p, t, u) / ds;
+  if (ctl->ctm_h2o)
+    for (int id = 0; id < ctl->nd; id++) beta[id] += o_ctmh2o(ctl, ctl->nu[id], p, t, q, u) / ds;
+  if (ctl->ctm_n2)
+    for (int id = 0; id < ctl->nd; id++) beta[id] += oracle_ctmn2(ctl->nu[id], p, t);
+  if (ctl->ctm_o2)
+    for (int id = 0; id < ctl->nd; id++) beta[id] += oracle_ctmo2(ctl->nu[id], p, t);
+}
+
+/* ------------------------------------------------------------------------
+   Atmosphere
+   ------------------------------------------------------------------------ */
+
+/* intpol_atm_1d (atmosphere.c:298-320), the IP=1 branch of intpol_atm_geo
+   (atmosphere.c:269-294). */
+static void o_intpol_atm(const ctl_t *ctl, const atm_t *atm, double z0, double *p, double *t,
+                         double *q, double *k) {
+  if (ctl->ip != 1) OFAIL("Only IP=1 (vertical profile interpolation) is supported!");
+  int ip = o_locate(atm->z, atm->np, z0);
+  *p = o_exp(atm->z[ip], atm->p[ip], atm->z[ip + 1], atm->p[ip + 1], z0);
+  *t = o_lin(atm->z[ip], atm->t[ip], atm->z[ip + 1], atm->t[ip + 1], z0);
+  for (int ig = 0; ig < ctl->ng; ig++)
+    q[ig] = o_lin(atm->z[ip], atm->q[ig][ip], atm->z[ip + 1], atm->q[ig][ip + 1], z0);
+  for (int iw = 0; iw < ctl->nw; iw++)
+    k[iw] = o_lin(atm->z[ip], atm->k[iw][ip], atm->z[ip + 1], atm->k[iw][ip + 1], z0);
+}
+
+/* gravity (atmosphere.c:153-159) */
+static double o_gravity(double z, double lat) {
+  return 9.780318 * (1 + 0.0053024 * pow(sin(lat / 180 * M_PI), 2) -
+                     0.0000058 * pow(sin(2 * lat / 180 * M_PI), 2)) - 3.086e-3 * z;
+}
+
+/* hydrostatic_1d (atmosphere.c:188-243) */
+static void o_hydrostatic_1d(ctl_t *ctl, atm_t *atm, int ip0, int ip1) {
+  int ig_h2o = o_find_emitter(ctl, "H2O");
+  double dzmin = 1e99, e = 0, mmair = 28.96456e-3, mmh2o = 18.0153e-3;
+  int ipref = 0, ipts = 20;
+  for (int ip = ip0; ip < ip1; ip++)
+    if (fabs(atm->z[ip] - ctl->hydz) < dzmin) {
+      dzmin = fabs(atm->z[ip] - ctl->hydz);
+      ipref = ip;
+    }
+  for (int ip = ipref + 1; ip < ip1; ip++) {
+    double mean = 0;
+    for (int i = 0; i < ipts; i++) {
+      double z = o_lin(0.0, atm->z[ip - 1], ipts - 1.0, atm->z[ip], (double)i);
+      if (ig_h2o >= 0)
+        e = o_lin(0.0, atm->q[ig_h2o][ip - 1], ipts - 1.0, atm->q[ig_h2o][ip], (double)i);
+      mean += (e * mmh2o + (1 - e) * mmair) * o_gravity(z, atm->lat[ipref]) / JUR_RGAS /
+              o_lin(0.0, atm->t[ip - 1], ipts - 1.0, atm->t[ip], (double)i) / ipts;
+    }
+    atm->p[ip] = exp(log(atm->p[ip - 1]) - mean * 1000 * (atm->z[ip] - atm->z[ip - 1]));
+  }
+  for (int ip = ipref - 1; ip >= ip0; ip--) {
+    double mean = 0;
+    for (int i = 0; i < ipts; i++) {
+      double z = o_lin(0.0, atm->z[ip + 1], ipts - 1.0, atm->z[ip], (double)i);
+      if (ig_h2o >= 0)
+        e = o_lin(0.0, atm->q[ig_h2o][ip + 1], ipts - 1.0, atm->q[ig_h2o][ip], (double)i);
+      mean += (e * mmh2o + (1 - e) * mmair) * o_gravity(z, atm->lat[ipref]) / JUR_RGAS /
+              o_lin(0.0, atm->t[ip + 1], ipts - 1.0, atm->t[ip], (double)i) / ipts;
+    }
+    atm->p[ip] = exp(log(atm->p[ip + 1]) - mean * 1000 * (atm->z[ip] - atm->z[ip + 1]));
+  }
+}
+
+/* hydrostatic (atmosphere.c:163-184) */
+void oracle_hydrostatic(ctl_t *ctl, atm_t *atm) {
+  double lat0 = -999, lon0 = -999;
+  int ip0 = -999;
+  if (ctl->hydz < 0) return;
+  for (int ip = 0; ip < atm->np; ip++)
+    if (atm->lon[ip] != lon0 || atm->lat[ip] != lat0) {
+      if (ip > 0) o_hydrostatic_1d(ctl, atm, ip0, ip);
+      lon0 = atm->lon[ip];
+      lat0 = atm->lat[ip];
+      ip0 = ip;
+    }
+  o_hydrostatic_1d(ctl, atm, ip0, atm->np);
+}
+
+/* ------------------------------------------------------------------------
+   Line of sight
+   ------------------------------------------------------------------------ */
+
+/* One ray's geometry: the obs_t columns raytrace reads and writes. */
+typedef struct {
+  double time, obsz, obslon, obslat, vpz, vplon, vplat, tpz, tplon, tplat;
+} oray_t;
+
+/* intersection_point (lineofsight.c:385-407): point on the chord between
+   LOS points ip-1 and ip of `los` at altitude *znew, stored at los2[jp]. */
+static void o_intersection(const ctl_t *ctl, const atm_t *atm, double znew, const los_t *los,
+                           int ip, los_t *los2, int jp) {
+  double x1[3], x2[3];
+  double frac = (los->z[ip - 1] - znew) / (los->z[ip - 1] - los->z[ip]);
+  o_geo2cart(los->z[ip - 1], los->lon[ip - 1], los->lat[ip - 1], x1);
+  o_geo2cart(los->z[ip], los->lon[ip], los->lat[ip], x2);
+  for (int i = 0; i < 3; i++) x2[i] = x1[i] + frac * (x2[i] - x1[i]);
+  o_cart2geo(x2, &los2->z[jp], &los2->lon[jp], &los2->lat[jp]);
+  o_intpol_atm(ctl, atm, los2->z[jp], &los2->p[jp], &los2->t[jp], los2->q[jp], los2->k[jp]);
+}
+
+/* tangent_point (lineofsight.c:411-452) */
+static void o_tangent_point(const los_t *los, double *tpz, double *tplon, double *tplat) {
+  /* gsl_stats_min_index: first minimum; a NaN returns its index. */
+  size_t ip = 0;
+  double zmin = los->z[0];
+  for (size_t i = 0; i < (size_t)los->np; i++) {
+    if (los->z[i] < zmin) { zmin = los->z[i]; ip = i; }
+    if (isnan(los->z[i])) { ip = i; break; }
+  }
+  if (ip <= 0 || ip >= (size_t)los->np - 1) {
+    *tpz = los->z[los->np - 1];
+    *tplon = los->lon[los->np - 1];
+    *tplat = los->lat[los->np - 1];
+  } else {
+    double yy0 = los->z[ip - 1], yy1 = los->z[ip], yy2 = los->z[ip + 1], v[3], v0[3], v2[3], dummy;
+    double x1 = sqrt(pow(los->ds[ip], 2) - pow(yy1 - yy0, 2));
+    double x2 = x1 + sqrt(pow(los->ds[ip + 1], 2) - pow(yy2 - yy1, 2));
+    double a = 1 / (x1 - x2) * (-(yy0 - yy1) / x1 + (yy0 - yy2) / x2);
+    double b = -(yy0 - yy1) / x1 - a * x1;
+    double c = yy0;
+    double x = -b / (2 * a);
+    *tpz = a * x * x + b * x + c;
+    o_geo2cart(los->z[ip - 1], los->lon[ip - 1], los->lat[ip - 1], v0);
+    o_geo2cart(los->z[ip + 1], los->lon[ip + 1], los->lat[ip + 1], v2);
+    for (int i = 0; i < 3; i++) v[i] = o_lin(0.0, v0[i], x2, v2[i], x);
+    o_cart2geo(v, &dummy, tplon, tplat);
+  }
+}
+
+/* add_aerosol_layers (lineofsight.c:193-372).
+   DEVIATION: only the np used points are copied back (the reference copies
+   the whole 5 MB struct).  The ip=0 layer pre-test reads z[-1] in the
+   reference; here it is taken as passed (the assignment then depends on
+   z[0] alone, which is what the reference's read of the np field yields). */
+static void o_add_aerosol_layers(const ctl_t *ctl, const atm_t *atm, los_t *los,
+                                 const aero_t *aero, los_t *la) {
+  double alti[4 * NLMAX + 2 * 10000], altimax, altimin, x1[3], x2[3], x3[3], eps = 0.005;
+  int jl = 0;
+  for (int il = 0; il < aero->nl; il++) {
+    alti[jl] = aero->top[il] + eps;
+    alti[jl + 1] = aero->top[il] - eps;
+    alti[jl + 2] = aero->bottom[il] + eps;
+    alti[jl + 3] = aero->bottom[il] - eps;
+    jl += 4;
+    if (aero->trans[il] > ctl->transs) {
+      double tt = aero->trans[il] / ctl->transs;
+      alti[jl] = aero->top[il] + aero->trans[il] + eps;
+      alti[jl + 1] = aero->top[il] + aero->trans[il] - eps;
+      alti[jl + 2] = aero->bottom[il] - aero->trans[il] + eps;
+      alti[jl + 3] = aero->bottom[il] - aero->trans[il] - eps;
+      jl += 4;
+      for (int it = 1; it < (int)tt; it++) {
+        alti[jl++] = aero->top[il] + aero->trans[il] - eps - it * ctl->transs;
+        alti[jl++] = aero->bottom[il] - aero->trans[il] + eps + it * ctl->transs;
+      }
+    }
+  }
+  /* Sort descending (the reference negates, sorts ascending with gsl_sort
+     and negates back, lineofsight.c:237-241). */
+  for (int i = 1; i < jl; i++) {
+    double v = alti[i];
+    int j = i - 1;
+    while (j >= 0 && alti[j] < v) { alti[j + 1] = alti[j]; j--; }
+    alti[j + 1] = v;
+  }
+  /* With no layer edges the comparisons below must all fail. */
+  altimax = jl > 0 ? alti[0] : NAN;
+  altimin = jl > 0 ? alti[0] : NAN;
+  for (int i = 0; i < jl; i++) {
+    if (alti[i] > altimax) altimax = alti[i];
+    if (alti[i] < altimin) altimin = alti[i];
+  }
+  la->tsurf = los->tsurf;
+  la->z[0] = los->z[0];
+  la->lat[0] = los->lat[0];
+  la->lon[0] = los->lon[0];
+  la->ds[0] = los->ds[0];
+  la->t[0] = los->t[0];
+  la->p[0] = los->p[0];
+  for (int ig = 0; ig < ctl->ng; ig++) la->q[0][ig] = los->q[0][ig];
+  for (int iw = 0; iw < ctl->nw; iw++) la->k[0][iw] = los->k[0][iw];
+  la->np = 1;
+  for (int ip = 1; ip < los->np; ip++) {
+    if ((los->z[ip - 1] < altimax || los->z[ip] < altimax) &&
+        (los->z[ip - 1] > altimin || los->z[ip] > altimin)) {
+      for (int il = 0; il < jl; il++) {
+        if (los->z[ip - 1] > alti[il] && los->z[ip] < alti[il]) {
+          o_intersection(ctl, atm, alti[il], los, ip, la, la->np);
+          la->np++;
+        }
+        if (los->z[ip - 1] < alti[jl - il - 1] && los->z[ip] > alti[jl - il - 1]) {
+          o_intersection(ctl, atm, alti[jl - il - 1], los, ip, la, la->np);
+          la->np++;
+        }
+      }
+    }
+    int n = la->np;
+    la->z[n] = los->z[ip];
+    la->lat[n] = los->lat[ip];
+    la->lon[n] = los->lon[ip];
+    la->t[n] = los->t[ip];
+    la->p[n] = los->p[ip];
+    for (int ig = 0; ig < ctl->ng; ig++) la->q[n][ig] = los->q[ip][ig];
+    for (int iw = 0; iw < ctl->nw; iw++) la->k[n][iw] = los->k[ip][iw];
+    if ((la->np++) > NLOS) OFAIL("Too many LOS points!");
+  }
+  /* Segment lengths from Cartesian half-chords (lineofsight.c:310-324). */
+  o_geo2cart(la->z[0], la->lon[0], la->lat[0], x1);
+  o_geo2cart(la->z[1], la->lon[1], la->lat[1], x2);
+  la->ds[0] = 0.5 * o_dist(x1, x2);
+  for (int ip = 1; ip < la->np - 1; ip++) {
+    o_geo2cart(la->z[ip - 1], la->lon[ip - 1], la->lat[ip - 1], x1);
+    o_geo2cart(la->z[ip], la->lon[ip], la->lat[ip], x2);
+    o_geo2cart(la->z[ip + 1], la->lon[ip + 1], la->lat[ip + 1], x3);
+    la->ds[ip] = 0.5 * (o_dist(x1, x2) + o_dist(x2, x3));
+  }
+  o_geo2cart(la->z[la->np - 1], la->lon[la->np - 1], la->lat[la->np - 1], x1);
+  o_geo2cart(la->z[la->np - 2], la->lon[la->np - 2], la->lat[la->np - 2], x2);
+  la->ds[la->np - 1] = 0.5 * o_dist(x1, x2);
+  /* Column densities and layer membership (lineofsight.c:327-363). */
+  for (int ip = 0; ip < la->np; ip++) {
+    for (int ig = 0; ig < ctl->ng; ig++)
+      la->u[ip][ig] = 10 * la->q[ip][ig] * la->p[ip] / (JUR_KBOLTZ * la->t[ip]) * la->ds[ip];
+    la->aeroi[ip] = -999;
+    la->aerofac[ip] = 0.;
+    int pre = (ip == 0) ? 1
+                        : ((la->z[ip - 1] < altimax || la->z[ip] < altimax) &&
+                           (la->z[ip - 1] > altimin || la->z[ip] > altimin));
+    if (pre) {
+      for (int il = 0; il < aero->nl; il++) {
+        if (la->z[ip] <= aero->top[il] && la->z[ip] >= aero->bottom[il]) {
+          la->aeroi[ip] = il;
+          la->aerofac[ip] = 1.;
+        }
+        if (aero->trans[il] > ctl->transs && la->z[ip] <= (aero->top[il] + aero->trans[il]) &&
+            la->z[ip] > aero->top[il]) {
+          la->aeroi[ip] = il;
+          la->aerofac[ip] = (aero->top[il] + aero->trans[il] - la->z[ip]) / aero->trans[il];
+        }
+        if (aero->trans[il] > ctl->transs && la->z[ip] < aero->bottom[il] &&
+            la->z[ip] >= (aero->bottom[il] - aero->trans[il])) {
+          la->aeroi[ip] = il;
+          la->aerofac[ip] = fabs(aero->bottom[il] - aero->trans[il] - la->z[ip]) / aero->trans[il];
+        }
+      }
+    }
+  }
+  /* Copy back the points in use. */
+  los->np = la->np;
+  los->tsurf = la->tsurf;
+  size_t n = (size_t)la->np;
+  memcpy(los->z, la->z, n * sizeof(double));
+  memcpy(los->lon, la->lon, n * sizeof(double));
+  memcpy(los->lat, la->lat, n * sizeof(double));
+  memcpy(los->p, la->p, n * sizeof(double));
+  memcpy(los->t, la->t, n * sizeof(double));
+  memcpy(los->q, la->q, n * sizeof(la->q[0]));
+  memcpy(los->k, la->k, n * sizeof(la->k[0]));
+  memcpy(los->aeroi, la->aeroi, n * sizeof(int));
+  memcpy(los->aerofac, la->aerofac, n * sizeof(double));
+  memcpy(los->ds, la->ds, n * sizeof(double));
+  memcpy(los->u, la->u, n * sizeof(la->u[0]));
+}
+
+/* Per-thread scratch LOS structs (avoids 5 MB allocations per pencil). */
+static __thread los_t *o_scratch[64];
+static __thread int o_level;
+static los_t *o_los_push(void) {
+  if (o_level >= 64) OFAIL("Recursion too deep!");
+  if (!o_scratch[o_level]) {
+    o_scratch[o_level] = (los_t *)malloc(sizeof(los_t));
+    if (!o_scratch[o_level]) OFAIL("Out of memory!");
+  }
+  return o_scratch[o_level++];
+}
+static void o_los_pop(void) { o_level--; }
+
+/* raytrace (lineofsight.c:5-190) */
+static void o_raytrace(const ctl_t *ctl, const atm_t *atm, oray_t *ray, const aero_t *aero,
+                       los_t *los) {
+  double cosa, d, dmax, dmin = 0, ds, ex0[3], ex1[3], h = 0.02, k[NWMAX], lat, lon, n, naux,
+         ng[3], norm, p = 0, q[NGMAX], t = 0, x[3], xh[3], xobs[3], xvp[3], z = 1e99, zmax, zmin,
+         zrefrac = 60;
+  int stop = 0;
+  los->np = 0;
+  los->tsurf = -999;
+  ray->tpz = ray->vpz;
+  ray->tplon = ray->vplon;
+  ray->tplat = ray->vplat;
+  zmin = zmax = atm->z[0];
+  for (int i = 0; i < atm->np; i++) { /* gsl_stats_min / _max */
+    if (atm->z[i] < zmin || isnan(atm->z[i])) zmin = atm->z[i];
+    if (atm->z[i] > zmax || isnan(atm->z[i])) zmax = atm->z[i];
+  }
+  if (ray->obsz < zmin) OFAIL("Observer below surface!");
+  if (ray->vpz > zmax - 0.001) return;
+  o_geo2cart(ray->obsz, ray->obslon, ray->obslat, xobs);
+  o_geo2cart(ray->vpz, ray->vplon, ray->vplat, xvp);
+  for (int i = 0; i < 3; i++) ex0[i] = xvp[i] - xobs[i];
+  norm = o_norm(ex0);
+  for (int i = 0; i < 3; i++) ex0[i] /= norm;
+  for (int i = 0; i < 3; i++) x[i] = xobs[i];
+  /* Observer above the atmosphere: bisect for the entry point (:53-67). */
+  if (ray->obsz > zmax) {
+    dmax = norm;
+    while (fabs(dmin - dmax) > 0.001) {
+      d = (dmax + dmin) / 2;
+      for (int i = 0; i < 3; i++) x[i] = xobs[i] + d * ex0[i];
+      o_cart2geo(x, &z, &lon, &lat);
+      if (z <= zmax && z > zmax - 0.001) break;
+      if (z < zmax - 0.0005) dmax = d; else dmin = d;
+    }
+  }
+  /* March (:70-167). */
+  while (1) {
+    ds = ctl->rayds;
+    if (ctl->raydz > 0) {
+      norm = o_norm(x);
+      for (int i = 0; i < 3; i++) xh[i] = x[i] / norm;
+      cosa = fabs(o_dot(ex0, xh));
+      if (cosa != 0) ds = o_min(ctl->rayds, ctl->raydz / cosa);
+    }
+    o_cart2geo(x, &z, &lon, &lat);
+    if (z < zmin + 0.001 || z > zmax + 0.001) {
+      stop = (z < zmin + 0.001 ? 2 : 1);
+      los->z[los->np] = z;
+      los->lon[los->np] = lon;
+      los->lat[los->np] = lat;
+      /* DEVIATION: with no previous point the reference reads z[-1]; here
+         the boundary point is kept as found. */
+      if (los->np > 0)
+        o_intersection(ctl, atm, (z < zmin + 0.001 ? zmin : zmax), los, los->np, los, los->np);
+      else
+        o_intpol_atm(ctl, atm, z, &los->p[0], &los->t[0], los->q[0], los->k[0]);
+      los->ds[los->np] = 0.;
+    }
+    if (stop == 0) {
+      o_intpol_atm(ctl, atm, z, &p, &t, q, k);
+      los->lon[los->np] = lon;
+      los->lat[los->np] = lat;
+      los->z[los->np] = z;
+      los->p[los->np] = p;
+      los->t[los->np] = t;
+      for (int ig = 0; ig < ctl->ng; ig++) los->q[los->np][ig] = q[ig];
+      for (int iw = 0; iw < ctl->nw; iw++) los->k[los->np][iw] = k[iw];
+      los->ds[los->np] = ds;
+    }
+    if ((++los->np) > NLOS) OFAIL("Too many LOS points!");
+    if (stop) {
+      /* t is the temperature of the last atmosphere evaluation. */
+      los->tsurf = (stop == 2 ? t : -999);
+      break;
+    }
+    if (ctl->refrac && z <= zrefrac)
+      n = 1 + 7.753e-05 * p / t; /* refractivity (lineofsight.c:376-381) */
+    else
+      n = 1;
+    for (int i = 0; i < 3; i++) ex1[i] = ex0[i] * n;
+    if (ctl->refrac && z <= zrefrac) {
+      for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
+      o_cart2geo(xh, &z, &lon, &lat);
+      o_intpol_atm(ctl, atm, z, &p, &t, q, k);
+      n = 7.753e-05 * p / t;
+      for (int i = 0; i < 3; i++) {
+        xh[i] += h;
+        o_cart2geo(xh, &z, &lon, &lat);
+        o_intpol_atm(ctl, atm, z, &p, &t, q, k);
+        naux = 7.753e-05 * p / t;
+        ng[i] = (naux - n) / h;
+        xh[i] -= h;
+      }
+    } else
+      for (int i = 0; i < 3; i++) ng[i] = 0;
+    for (int i = 0; i < 3; i++) ex1[i] += ds * ng[i];
+    norm = o_norm(ex1);
+    for (int i = 0; i < 3; i++) ex1[i] /= norm;
+    for (int i = 0; i < 3; i++) x[i] += 0.5 * ds * (ex0[i] + ex1[i]);
+    for (int i = 0; i < 3; i++) ex0[i] = ex1[i];
+  }
+  /* Drop a vanishing last segment (:170-171). */
+  if (los->np >= 2 && los->ds[los->np - 2] < 1e-3 && los->np - 1 > 1) los->np--;
+  o_tangent_point(los, &ray->tpz, &ray->tplon, &ray->tplat);
+  /* Recursive half-sum of the step lengths, as in the reference (:177-179):
+     each ds uses the already updated predecessor. */
+  los->ds[0] = 0.5 * los->ds[0];
+  for (int ip = 1; ip < los->np; ip++) los->ds[ip] = 0.5 * (los->ds[ip - 1] + los->ds[ip]);
+  for (int ip = 0; ip < los->np; ip++)
+    for (int ig = 0; ig < ctl->ng; ig++)
+      los->u[ip][ig] = 10 * los->q[ip][ig] * los->p[ip] / (JUR_KBOLTZ * los->t[ip]) * los->ds[ip];
+  for (int ip = 0; ip < los->np; ip++) {
+    los->aeroi[ip] = -999;
+    los->aerofac[ip] = 0;
+  }
+  if (ctl->sca_n > 0) {
+    los_t *la = o_los_push();
+    o_add_aerosol_layers(ctl, atm, los, aero, la);
+    o_los_pop();
+  }
+}
+
+void oracle_raytrace(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, los_t *los, int ir) {
+  oray_t ray = {obs->time[ir], obs->obsz[ir], obs->obslon[ir], obs->obslat[ir], obs->vpz[ir],
+                obs->vplon[ir], obs->vplat[ir], 0, 0, 0};
+  o_raytrace(ctl, atm, &ray, aero, los);
+  obs->tpz[ir] = ray.tpz;
+  obs->tplon[ir] = ray.tplon;
+  obs->tplat[ir] = ray.tplat;
+}
+
+/* ------------------------------------------------------------------------
+   Mie optics
+   ------------------------------------------------------------------------ */
+
+typedef struct { double re, im; } ocx;
+static inline ocx cx(double a, double b) { ocx z = {a, b}; return z; }
+static inline ocx cx_add(ocx a, ocx b) { return cx(a.re + b.re, a.im + b.im); }
+static inline ocx cx_sub(ocx a, ocx b) { return cx(a.re - b.re, a.im - b.im); }
+static inline ocx cx_mul(ocx a, ocx b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static inline double cx_abs(ocx a) { return hypot(a.re, a.im); }
+/* gsl_complex_div of GSL 1.15: scaled by 1/|b|. */
+static inline ocx cx_div(ocx a, ocx b) {
+  double s = 1.0 / cx_abs(b), sbr = s * b.re, sbi = s * b.im;
+  return cx((a.re * sbr + a.im * sbi) * s, (a.im * sbr - a.re * sbi) * s);
+}
+
+/* bhmie: Bohren & Huffman Mie series (scatter.c:56-234). */
+void oracle_bhmie(double x, double n_real, double n_imag, double *phase, double *qext,
+                  double *qsca) {
+  enum { NT = (NTHETA + 1) / 2 };
+  ocx cxan, cxbn, cxxi, cxxi1, cxtemp, cxref, cxy, cxs1[NTHETA + 2], cxs2[NTHETA + 2];
+  static __thread ocx cxd[10001];
+  double amu[NT + 2], pi[NT + 2], pi0[NT + 2], pi1[NT + 2], tau[NT + 2];
+  double apsi, apsi1, chi, chi0, chi1, dang, fn, p, rn, t, psi, psi0, psi1, dx = x, xstop, ymod;
+  int nmx, nstop;
+  if ((NTHETA + 1) % 2 != 0) OFAIL("NTHETA needs to be odd!");
+  cxref = cx(n_real, n_imag);
+  cxy = cx_mul(cx(x, 0.0), cxref);
+  xstop = x + 4.E0 * pow(x, 0.3333) + 2.0;
+  nstop = (int)xstop;
+  ymod = cx_abs(cxy);
+  nmx = (int)((xstop > ymod ? xstop : ymod) + 15);
+  if (nmx > 10000) OFAIL("Too many Mie terms!");
+  dang = .5E0 * M_PI / (double)(NT - 1);
+  for (int j = 1; j <= NT; j++) amu[j] = cos((double)(j - 1) * dang);
+  /* Logarithmic derivative, downward recurrence from D(nmx) = 0. */
+  cxd[nmx] = cx(0, 0);
+  for (int n = 1; n <= nmx - 1; n++) {
+    rn = nmx - n + 1;
+    cxtemp = cx_add(cxd[nmx - n + 1], cx_div(cx(rn, 0.0), cxy));
+    cxtemp = cx_div(cx(1.0, 0.0), cxtemp);
+    cxd[nmx - n] = cx_sub(cx_div(cx(rn, 0.0), cxy), cxtemp);
+  }
+  for (int j = 1; j <= NT; j++) {
+    pi0[j] = 0.E0;
+    pi1[j] = 1.E0;
+  }
+  for (int j = 1; j <= 2 * NT - 1; j++) {
+    cxs1[j] = cx(0, 0);
+    cxs2[j] = cx(0, 0);
+  }
+  /* Riccati-Bessel functions, upward recurrence. */
+  psi0 = cos(dx);
+  psi1 = sin(dx);
+  chi0 = -sin(x);
+  chi1 = cos(x);
+  apsi1 = psi1;
+  cxxi1 = cx(apsi1, -chi1);
+  *qsca = 0.E0;
+  for (int n = 1; n <= nstop; n++) {
+    double dn = n;
+    rn = n;
+    fn = (2.E0 * rn + 1.E0) / (rn * (rn + 1.E0));
+    psi = (2.E0 * dn - 1.E0) * psi1 / dx - psi0;
+    apsi = psi;
+    chi = (2.E0 * rn - 1.E0) * chi1 / x - chi0;
+    cxxi = cx(apsi, -chi);
+    cxan = cx_div(cxd[n], cxref);
+    cxan = cx_add(cxan, cx(rn / x, 0.0));
+    cxan = cx_mul(cxan, cx(apsi, 0.0));
+    cxan = cx_sub(cxan, cx(apsi1, 0.0));
+    cxtemp = cx_div(cxd[n], cxref);
+    cxtemp = cx_add(cxtemp, cx(rn / x, 0.0));
+    cxtemp = cx_mul(cxtemp, cxxi);
+    cxtemp = cx_sub(cxtemp, cxxi1);
+    cxan = cx_div(cxan, cxtemp);
+    cxbn = cx_mul(cxref, cxd[n]);
+    cxbn = cx_add(cxbn, cx(rn / x, 0.0));
+    cxbn = cx_mul(cxbn, cx(apsi, 0.0));
+    cxbn = cx_sub(cxbn, cx(apsi1, 0.0));
+    cxtemp = cx_mul(cxref, cxd[n]);
+    cxtemp = cx_add(cxtemp, cx(rn / x, 0.0));
+    cxtemp = cx_mul(cxtemp, cxxi);
+    cxtemp = cx_sub(cxtemp, cxxi1);
+    cxbn = cx_div(cxbn, cxtemp);
+    *qsca = *qsca + (2. * rn + 1.) * (cx_abs(cxan) * cx_abs(cxan) + cx_abs(cxbn) * cx_abs(cxbn));
+    for (int j = 1; j <= NT; j++) {
+      int jj = 2 * NT - j;
+      pi[j] = pi1[j];
+      tau[j] = rn * amu[j] * pi[j] - (rn + 1.E0) * pi0[j];
+      p = pow(-1.0, n - 1);
+      cxtemp = cx_mul(cxan, cx(pi[j], 0.0));
+      cxtemp = cx_add(cxtemp, cx_mul(cxbn, cx(tau[j], 0.0)));
+      cxtemp = cx_mul(cx(fn, 0.0), cxtemp);
+      cxs1[j] = cx_add(cxs1[j], cxtemp);
+      t = pow(-1.0, n);
+      cxtemp = cx_mul(cxan, cx(tau[j], 0.0));
+      cxtemp = cx_add(cxtemp, cx_mul(cxbn, cx(pi[j], 0.0)));
+      cxtemp = cx_mul(cx(fn, 0.0), cxtemp);
+      cxs2[j] = cx_add(cxs2[j], cxtemp);
+      if (j != jj) {
+        cxtemp = cx_mul(cxan, cx(pi[j] * p, 0.0));
+        cxtemp = cx_add(cxtemp, cx_mul(cxbn, cx(tau[j] * t, 0.0)));
+        cxtemp = cx_mul(cx(fn, 0.0), cxtemp);
+        cxs1[jj] = cx_add(cxs1[jj], cxtemp);
+        cxtemp = cx_mul(cxan, cx(tau[j] * t, 0.0));
+        cxtemp = cx_add(cxtemp, cx_mul(cxbn, cx(pi[j] * p, 0.0)));
+        cxtemp = cx_mul(cx(fn, 0.0), cxtemp);
+        cxs2[jj] = cx_add(cxs2[jj], cxtemp);
+      }
+    }
+    psi0 = psi1;
+    psi1 = psi;
+    apsi1 = psi1;
+    chi0 = chi1;
+    chi1 = chi;
+    cxxi1 = cx(apsi1, -chi1);
+    for (int j = 1; j <= NT; j++) {
+      pi1[j] = ((2. * rn + 1.) * amu[j] * pi[j] - (rn + 1.) * pi0[j]) / rn;
+      pi0[j] = pi[j];
+    }
+  }
+  *qsca = (2.E0 / (x * x)) * (*qsca);
+  *qext = (4.E0 / (x * x)) * cxs1[1].re;
+  for (int i = 0; i < 2 * NT - 1; i++)
+    phase[i] = 2 / (x * x * (*qsca)) *
+               (cxs1[i + 1].re * cxs1[i + 1].re + cxs1[i + 1].im * cxs1[i + 1].im +
+                cxs2[i + 1].re * cxs2[i + 1].re + cxs2[i + 1].im * cxs2[i + 1].im);
+}
+
+/* gauher: Gauss-Hermite nodes and weights, n = NRAD (scatter.c:275-327;
+   Numerical Recipes 3rd ed. initial guesses and Newton iteration). */
+void oracle_gauher(double *x, double *w) {
+  const double EPS = 1.0e-14, PIM4 = 0.7511255444649425;
+  const int MAXIT = 10000, n = NRAD, m = (NRAD + 1) / 2;
+  double p1, p2, p3, pp = 0, z = 0, z1;
+  for (int i = 0; i < m; i++) {
+    if (i == 0)
+      z = sqrt((double)(2 * n + 1)) - 1.85575 * pow((double)(2 * n + 1), -0.16667);
+    else if (i == 1)
+      z -= 1.14 * pow((double)n, 0.426) / z;
+    else if (i == 2)
+      z = 1.86 * z - 0.86 * x[0];
+    else if (i == 3)
+      z = 1.91 * z - 0.91 * x[1];
+    else
+      z = 2.0 * z - x[i - 2];
+    int its;
+    for (its = 0; its < MAXIT; its++) {
+      p1 = PIM4;
+      p2 = 0.0;
+      for (int j = 0; j < n; j++) {
+        p3 = p2;
+        p2 = p1;
+        p1 = z * sqrt(2.0 / ((double)j + 1)) * p2 - sqrt((double)j / (j + 1.)) * p3;
+      }
+      pp = sqrt(2. * (double)n) * p2;
+      z1 = z;
+      z = z1 - p1 / pp;
+      if (fabs(z - z1) <= EPS) break;
+    }
+    if (its >= MAXIT) OFAIL("Too many iterations in gauher.");
+    x[i] = z;
+    x[n - 1 - i] = -z;
+    w[i] = 2.0 / (pp * pp);
+    w[n - 1 - i] = w[i];
+  }
+}
+
+/* opt_prop_mie_log (scatter.c:422-508): one log-normal mode. */
+static void o_mie_log(ctl_t *ctl, aero_t *aero, int count, double *beta_ext, double *beta_sca,
+                      double phase[NDMAX][NTHETA]) {
+  static double nr_c[NDMAX], ni_c[NDMAX], zs[NRAD], weights[NRAD];
+  static int gh_init = 0;
+  static char last_path[LEN] = "";
+  if (count == 0 || strcmp(aero->filepath[count], aero->filepath[count - 1]) != 0 ||
+      strcmp(last_path, aero->filepath[count]) != 0) {
+    static double fnu[REFMAX + 1], fnr[REFMAX + 1], fni[REFMAX + 1];
+    char line[LEN];
+    FILE *in;
+    int npts = 0;
+    if (!(in = fopen(aero->filepath[count], "r"))) OFAIL("Cannot open file!");
+    while (fgets(line, LEN, in))
+      if (sscanf(line, "%lg %lg %lg", &fnu[npts], &fnr[npts], &fni[npts]) == 3)
+        if ((++npts) > REFMAX) OFAIL("Too many data points!");
+    fclose(in);
+    for (int id = 0; id < ctl->nd; id++) {
+      int idx = o_locate(fnu, npts, ctl->nu[id]);
+      nr_c[id] = o_lin(fnu[idx], fnr[idx], fnu[idx + 1], fnr[idx + 1], ctl->nu[id]);
+      ni_c[id] = o_lin(fnu[idx], fni[idx], fnu[idx + 1], fni[idx + 1], ctl->nu[id]);
+    }
+    snprintf(last_path, LEN, "%s", aero->filepath[count]);
+  }
+  if (aero->nn[count] <= 0 || aero->rr[count] <= 0 || aero->ss[count] <= 1)
+    OFAIL("The log-normal parameters are nonsense. ((?_?)) ");
+  if (!gh_init) {
+    gh_init = 1;
+    oracle_gauher(zs, weights);
+  }
+  double K1 = aero->nn[count] * 1e-3 * sqrt(M_PI), qext, qsca, qphase[NTHETA];
+  for (int nn = 0; nn < NRAD; ++nn) {
+    double rad = exp(sqrt(2) * log(aero->ss[count]) * zs[nn] + log(aero->rr[count]));
+    if (rad >= 0.001 && rad <= 1000. && K1 > 0.) {
+      for (int id = 0; id < ctl->nd; id++) {
+        double lambda = 1. / (ctl->nu[id]) * pow(10, 4.);
+        double x = 2 * M_PI * rad / lambda;
+        oracle_bhmie(x, nr_c[id], ni_c[id], qphase, &qext, &qsca);
+        beta_ext[id] += K1 * pow(rad, 2) * qext * weights[nn];
+        beta_sca[id] += K1 * pow(rad, 2) * qsca * weights[nn];
+        for (int jj = 0; jj < NTHETA; ++jj)
+          phase[id][jj] += K1 * qsca * pow(rad, 2) * qphase[jj] * weights[nn];
+      }
+    }
+  }
+  for (int id = 0; id < ctl->nd; id++)
+    for (int jj = 0; jj < NTHETA; ++jj) phase[id][jj] /= beta_sca[id];
+}
+
+/* get_opt_prop (scatter.c:330-418).  Only the "MIE" model type is
+   restated; "Ext" and "Const" stop with an error. */
+void oracle_get_opt_prop(ctl_t *ctl, aero_t *aero) {
+  int nl = 1, nm = 1, count;
+  static double mbe[NDMAX], mbs[NDMAX], mp[NDMAX][NTHETA];
+  if (aero->top_mod[0] <= aero->bottom_mod[0])
+    OFAIL("Aerosol top altitude is smaller than bottom altitude. Please check aero.tab.");
+  aero->top[0] = aero->top_mod[0];
+  aero->bottom[0] = aero->bottom_mod[0];
+  aero->trans[0] = aero->trans_mod[0];
+  for (int ii = 1; ii < aero->nm; ii++) {
+    if (aero->top_mod[ii] > aero->top_mod[ii - 1] || aero->top_mod[ii] <= aero->bottom_mod[ii] ||
+        aero->bottom_mod[ii] > aero->bottom_mod[ii - 1])
+      OFAIL("Aerosol/Cloud altitudes and/or transition layers are wrong. Please check aero.tab.");
+    if (aero->top_mod[ii] != aero->top_mod[ii - 1]) {
+      aero->top[nl] = aero->top_mod[ii];
+      aero->bottom[nl] = aero->bottom_mod[ii];
+      aero->trans[nl] = aero->trans_mod[ii];
+      aero->nmod[nl - 1] = nm;
+      nl++;
+      nm = 0;
+    }
+    nm++;
+  }
+  aero->nmod[nl - 1] = nm;
+  aero->nl = nl;
+  count = 0;
+  for (int ii = 0; ii < aero->nl; ii++) {
+    for (int id = 0; id < ctl->nd; id++) {
+      aero->beta_e[ii][id] = aero->beta_a[ii][id] = aero->beta_s[ii][id] = 0.;
+      for (int it = 0; it < NTHETA; it++) aero->p[ii][id][it] = mp[id][it] = 0.;
+      mbe[id] = mbs[id] = 0.;
+    }
+    for (int jj = 0; jj < aero->nmod[ii]; jj++) {
+      if (strcasecmp(aero->type[count], "MIE") == 0)
+        o_mie_log(ctl, aero, count, mbe, mbs, mp);
+      else
+        OFAIL("Only MIE optical properties are supported!");
+      for (int id = 0; id < ctl->nd; id++) {
+        aero->beta_e[ii][id] += mbe[id];
+        aero->beta_s[ii][id] += mbs[id];
+        aero->beta_a[ii][id] += (mbe[id] - mbs[id]);
+        for (int it = 0; it < NTHETA; it++) {
+          aero->p[ii][id][it] += mp[id][it];
+          mp[id][it] = 0.;
+        }
+        mbe[id] = mbs[id] = 0.;
+      }
+      count++;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------
+   Scattering source
+   ------------------------------------------------------------------------ */
+
+/* bascoord (scatter.c:6-52) */
+static void o_bascoord(const double *dz, const double *dy, double *ex, double *ey, double *ez) {
+  double dotp, norm;
+  norm = o_norm(dz);
+  for (int i = 0; i < 3; i++) ez[i] = dz[i] / norm;
+  norm = o_norm(dy);
+  for (int i = 0; i < 3; i++) ey[i] = dy[i] / norm;
+  dotp = o_dot(ey, ez);
+  if (dotp == 1) {
+    ey[0] = 1; ey[1] = 0; ey[2] = 0;
+    dotp = ez[0];
+    if (dotp == 1) { ey[0] = 0; ey[1] = 1; dotp = ez[1]; }
+  }
+  for (int i = 0; i < 3; i++) ey[i] -= dotp * ez[i];
+  norm = o_norm(ey);
+  for (int i = 0; i < 3; i++) ey[i] /= norm;
+  ex[0] = ey[1] * ez[2] - ey[2] * ez[1];
+  ex[1] = ey[2] * ez[0] - ey[0] * ez[2];
+  ex[2] = ey[0] * ez[1] - ey[1] * ez[0];
+}
+
+/* suncoord (scatter.c:999-1041) */
+void oracle_suncoord(double sec, double lon, double lat, double *azi, double *sza) {
+  double D = sec / 86400 - 0.5;
+  double g = (357.529 + 0.98560028 * D) * M_PI / 180;
+  double q = 280.459 + 0.98564736 * D;
+  double L = (q + 1.915 * sin(g) + 0.020 * sin(2 * g)) * M_PI / 180;
+  double e = (23.439 - 0.00000036 * D) * M_PI / 180;
+  double dec = asin(sin(e) * sin(L));
+  double ra = atan2(cos(e) * sin(L), cos(L));
+  double GMST = 18.697374558 + 24.06570982441908 * D;
+  double LST = GMST + lon / 15;
+  double h = LST / 12 * M_PI - ra;
+  lat *= M_PI / 180;
+  *azi = atan2(-sin(h), cos(lat) * tan(dec) - sin(lat) * cos(h)) * 180 / M_PI;
+  *sza = acos(sin(lat) * sin(dec) + cos(lat) * cos(dec) * cos(h)) * 180 / M_PI;
+}
+
+/* The 28 zenith angles of the secondary-ray fan (scatter.c:698-715):
+   83..90 deg in 1 deg steps around the horizon, then steps growing along the
+   Fibonacci sequence (1+2, 3, 5, 8, 13, 13, ...) towards 0 and 180 deg. */
+static void o_alpha_grid(double *alpha) {
+  const int nalpha = 28;
+  int n1 = 1, n2 = 2;
+  double midang = 83, up = 92, down = 81, step = 1 + 2;
+  alpha[0] = 0;
+  for (int ir = nalpha / 2 - 4; ir < nalpha / 2 + 4; ++ir) {
+    alpha[ir] = midang * M_PI / 180.;
+    midang++;
+  }
+  for (int ir = 0; ir < nalpha / 2 - 5; ++ir) {
+    alpha[nalpha / 2 + ir + 4] = up * M_PI / 180.;
+    alpha[nalpha / 2 - ir - 5] = down * M_PI / 180.;
+    up += step;
+    down -= step;
+    if (step < 13) {
+      step = (double)n1 + (double)n2;
+      n1 = n2;
+      n2 = (int)step;
+    }
+  }
+  alpha[nalpha - 1] = M_PI;
+}
+
+static void o_pencil(ctl_t *ctl, const atm_t *atm, const aero_t *aero, oray_t *ray,
+                     int scattering, double *rad, double *tau);
+
+/* srcfunc_sca_1d (scatter.c:666-807): thermal scattering source at x. */
+static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double time,
+                             const double *x, const double *dx, int il, double *src_sca,
+                             int scattering) {
+  enum { NALPHA = 28 };
+  double alpha[NALPHA], dnorth[3], ek[3], lx[3], ly[3], lz[3], sx[3], sy[3], sz[3],
+         theta[NTHETA], xv[3], wsum[NDMAX];
+  double *rad2 = (double *)malloc(sizeof(double) * NALPHA * NDMAX);
+  double tau2[NDMAX];
+  for (int it = 0; it < NTHETA; it++) theta[it] = (double)it * M_PI / 180.;
+  dnorth[0] = -x[0];
+  dnorth[1] = -x[1];
+  dnorth[2] = 2 * RE - x[2];
+  o_bascoord(x, dnorth, lx, ly, lz);
+  o_alpha_grid(alpha);
+  for (int ir = 0; ir < NALPHA; ir++) {
+    for (int i = 0; i < 3; i++)
+      xv[i] = x[i] + 10. * cos(alpha[ir]) * (-1) * lz[i] + 10. * sin(alpha[ir]) * ly[i];
+    oray_t r2;
+    memset(&r2, 0, sizeof(r2));
+    /* DEVIATION: the reference leaves the time of the secondary pencils
+       uninitialised (malloc'd obs_t); it only matters for the solar term
+       at depth >= 2.  The primary's time is propagated here. */
+    r2.time = time;
+    o_cart2geo(x, &r2.obsz, &r2.obslon, &r2.obslat);
+    o_cart2geo(xv, &r2.vpz, &r2.vplon, &r2.vplat);
+    o_pencil(ctl, atm, aero, &r2, scattering - 1, &rad2[ir * NDMAX], tau2);
+  }
+  o_bascoord(dx, x, sx, sy, sz);
+  for (int id = 0; id < ctl->nd; id++) src_sca[id] = wsum[id] = 0;
+  double mlz[3] = {-lz[0], -lz[1], -lz[2]};
+  for (int itheta = 0; itheta < 180; itheta++) {
+    double theta2 = (0.5 + itheta) * M_PI / 180.;
+    for (int iphi = 0; iphi < 180; iphi++) {
+      double phi = 2. * (0.5 + iphi) * M_PI / 180.;
+      for (int i = 0; i < 3; i++)
+        ek[i] = sin(theta2) * sin(phi) * sx[i] + sin(theta2) * cos(phi) * sy[i] +
+                cos(theta2) * sz[i];
+      int idp = o_locate(theta, NTHETA, theta2);
+      double alpha2 = o_angle(mlz, ek);
+      int idx = o_locate(alpha, NALPHA, alpha2);
+      for (int id = 0; id < ctl->nd; id++) {
+        double phase2 = o_lin(theta[idp], aero->p[il][id][idp], theta[idp + 1],
+                              aero->p[il][id][idp + 1], theta2);
+        double w = sin(theta2) * phase2;
+        double r = o_lin(alpha[idx], rad2[idx * NDMAX + id], alpha[idx + 1],
+                         rad2[(idx + 1) * NDMAX + id], alpha2);
+        src_sca[id] += w * r;
+        wsum[id] += w;
+      }
+    }
+  }
+  for (int id = 0; id < ctl->nd; id++) src_sca[id] /= wsum[id];
+  free(rad2);
+}
+
+/* srcfunc_sca_sun (scatter.c:899-995): solar single scattering at x. */
+static void o_srcfunc_sca_sun(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double sec,
+                              const double *x, const double *dx, int il, double *src_sca) {
+  double azi, sza, sza_beam, sza_cor, dnorth[3], dout[3], ek[3] = {0, 0, 0}, lx[3], ly[3],
+         lz[3], theta[NTHETA], x0[3], x1[3];
+  oray_t r;
+  memset(&r, 0, sizeof(r));
+  los_t *los = o_los_push();
+  for (int it = 0; it < NTHETA; it++) theta[it] = M_PI * (double)it / (NTHETA - 1);
+  dnorth[0] = -x[0];
+  dnorth[1] = -x[1];
+  dnorth[2] = 2 * RE - x[2];
+  o_bascoord(x, dnorth, lx, ly, lz);
+  o_cart2geo(x, &r.obsz, &r.obslon, &r.obslat);
+  r.time = sec;
+  oracle_suncoord(sec, r.obslon, r.obslat, &azi, &sza);
+  sza_cor = sza;
+  los->np = 0;
+  los->tsurf = -999;
+  for (int i2 = 0; i2 < 10; i2++) {
+    for (int i = 0; i < 3; i++) {
+      ek[i] = sin(sza_cor * M_PI / 180) * sin(azi * M_PI / 180) * lx[i] +
+              sin(sza_cor * M_PI / 180) * cos(azi * M_PI / 180) * ly[i] +
+              cos(sza_cor * M_PI / 180) * lz[i];
+      x1[i] = x[i] + 10 * ek[i];
+    }
+    o_cart2geo(x1, &r.vpz, &r.vplon, &r.vplat);
+    o_raytrace(ctl, atm, &r, aero, los);
+    if (los->np < 2) break;
+    o_geo2cart(los->z[los->np - 2], los->lon[los->np - 2], los->lat[los->np - 2], x0);
+    o_geo2cart(los->z[los->np - 1], los->lon[los->np - 1], los->lat[los->np - 1], x1);
+    for (int i = 0; i < 3; i++) dout[i] = x1[i] - x0[i];
+    sza_beam = o_angle(x, dout) * 180 / M_PI;
+    if (fabs(sza_beam - sza) < 0.01) break;
+    sza_cor -= 0.61803 * (sza_beam - sza);
+    sza_cor = o_min(o_max(sza_cor, 0), 180);
+  }
+  if (los->tsurf < 0) {
+    double rad_s[NDMAX], tau_s[NDMAX];
+    o_pencil(ctl, atm, aero, &r, 0, rad_s, tau_s);
+    double theta2 = o_angle(ek, dx);
+    int idx = o_locate(theta, NTHETA, theta2);
+    for (int id = 0; id < ctl->nd; id++) {
+      double phase2 = o_lin(theta[idx], aero->p[il][id][idx], theta[idx + 1],
+                            aero->p[il][id][idx + 1], theta2);
+      src_sca[id] += 6.764e-5 * phase2 * oracle_planck(TSUN, ctl->nu[id]) * tau_s[id];
+    }
+  }
+  o_los_pop();
+}
+
+/* srcfunc_sca (scatter.c:643-662) */
+static void o_srcfunc_sca(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double sec,
+                          const double *x, const double *dx, int il, double *src_sca,
+                          int scattering) {
+  if (ctl->ip != 1) OFAIL("Only IP=1 scattering (srcfunc_sca_1d) is supported!");
+  o_srcfunc_sca_1d(ctl, atm, aero, sec, x, dx, il, src_sca, scattering);
+  if (TSUN > 0) o_srcfunc_sca_sun(ctl, atm, aero, sec, x, dx, il, src_sca);
+}
+
+/* ------------------------------------------------------------------------
+   Pencil beam and forward model
+   ------------------------------------------------------------------------ */
+
+/* formod_pencil, non-queue path (forwardmodel.c:224-405). */
+static void o_pencil(ctl_t *ctl, const atm_t *atm, const aero_t *aero, oray_t *ray,
+                     int scattering, double *rad, double *tau) {
+  double beta_ctm[NDMAX], src_planck[NDMAX], src_sca[NDMAX], tau_gas[NDMAX], x[3], x0[3], x1[3],
+         dx[3];
+  double (*tau_path)[NDMAX] = malloc(sizeof(double) * NGMAX * NDMAX);
+  los_t *los = o_los_push();
+  o_raytrace(ctl, atm, ray, aero, los);
+  for (int id = 0; id < ctl->nd; id++) {
+    rad[id] = 0;
+    tau[id] = 1;
+  }
+  for (int ip = 0; ip < los->np; ip++) {
+    o_intpol_tbl(ctl, los->p[ip], los->t[ip], los->u[ip], ip <= 0, tau_path, tau_gas);
+    oracle_continua(ctl, los->p[ip], los->t[ip], los->q[ip], los->u[ip], los->ds[ip], los->k[ip],
+                    beta_ctm);
+    oracle_srcfunc_planck(ctl, los->t[ip], src_planck);
+    if (scattering > 0 && los->aerofac[ip] > 0) {
+      int il = los->aeroi[ip];
+      int ip0 = (ip > 0 ? ip - 1 : ip);
+      /* DEVIATION: the reference takes ip+1 = np at the last point (a
+         point past the end); the last point is used here instead. */
+      int ip1 = (ip < los->np - 1 ? ip + 1 : ip);
+      o_geo2cart(los->z[ip], los->lon[ip], los->lat[ip], x);
+      o_geo2cart(los->z[ip0], los->lon[ip0], los->lat[ip0], x0);
+      o_geo2cart(los->z[ip1], los->lon[ip1], los->lat[ip1], x1);
+      for (int i = 0; i < 3; i++) dx[i] = x1[i] - x0[i];
+      o_srcfunc_sca(ctl, atm, aero, ray->time, x, dx, il, src_sca, scattering);
+      double af = los->aerofac[ip];
+      for (int id = 0; id < ctl->nd; id++)
+        if (tau_gas[id] > 0) {
+          double bt = (-1.) * log(tau_gas[id]) / los->ds[ip] + beta_ctm[id] + af * aero->beta_e[il][id];
+          double eps = 1 - exp(-1 * bt * los->ds[ip]);
+          double src_all = ((bt - af * aero->beta_s[il][id]) * src_planck[id] +
+                            af * aero->beta_s[il][id] * src_sca[id]) / bt;
+          rad[id] += tau[id] * eps * src_all;
+          tau[id] *= exp(-1. * bt * los->ds[ip]);
+        }
+    } else {
+      for (int id = 0; id < ctl->nd; id++)
+        if (tau_gas[id] > 0) {
+          double eps;
+          if (ctl->sca_n == 0 || los->aerofac[ip] == 0)
+            eps = 1 - tau_gas[id] * exp(-1. * beta_ctm[id] * los->ds[ip]);
+          else if (strcmp(ctl->sca_ext, "beta_a") == 0)
+            eps = 1 - tau_gas[id] * exp(-1. * (beta_ctm[id] + los->aerofac[ip] *
+                                                 aero->beta_a[los->aeroi[ip]][id]) * los->ds[ip]);
+          else
+            eps = 1 - tau_gas[id] * exp(-1. * (beta_ctm[id] + los->aerofac[ip] *
+                                                 aero->beta_e[los->aeroi[ip]][id]) * los->ds[ip]);
+          rad[id] += src_planck[id] * eps * tau[id];
+          tau[id] *= (1 - eps);
+        }
+    }
+  }
+  if (los->tsurf > 0) {
+    oracle_srcfunc_planck(ctl, los->tsurf, src_planck);
+    for (int id = 0; id < ctl->nd; id++) rad[id] += src_planck[id] * tau[id];
+  }
+  o_los_pop();
+  free(tau_path);
+}
+
+void oracle_formod_pencil(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, int scattering,
+                          int ir) {
+  double rad[NDMAX], tau[NDMAX];
+  oray_t ray = {obs->time[ir], obs->obsz[ir], obs->obslon[ir], obs->obslat[ir], obs->vpz[ir],
+                obs->vplon[ir], obs->vplat[ir], 0, 0, 0};
+  o_init_tables(ctl);
+  o_pencil(ctl, atm, aero, &ray, scattering, rad, tau);
+  obs->tpz[ir] = ray.tpz;
+  obs->tplon[ir] = ray.tplon;
+  obs->tplat[ir] = ray.tplat;
+  for (int id = 0; id < ctl->nd; id++) {
+    obs->rad[id][ir] = rad[id];
+    obs->tau[id][ir] = tau[id];
+  }
+}
+
+/* formod_fov (forwardmodel.c:156-220) */
+static void o_formod_fov(ctl_t *ctl, obs_t *obs) {
+  static double dz[NSHAPE], w[NSHAPE];
+  int n;
+  if (ctl->fov[0] == '-') return;
+  o_read_shape(ctl->fov, dz, w, &n);
+  obs_t *obs2 = (obs_t *)malloc(sizeof(obs_t));
+  memcpy(obs2, obs, sizeof(obs_t));
+  double *z = (double *)malloc(sizeof(double) * NRMAX);
+  double (*rr)[NRMAX] = malloc(sizeof(double) * NDMAX * NRMAX);
+  double (*tt)[NRMAX] = malloc(sizeof(double) * NDMAX * NRMAX);
+  for (int ir = 0; ir < obs->nr; ir++) {
+    int nz = 0;
+    for (int ir2 = (ir - NFOV > 0 ? ir - NFOV : 0);
+         ir2 < (ir + 1 + NFOV < obs->nr ? ir + 1 + NFOV : obs->nr); ir2++)
+      if (obs->time[ir2] == obs->time[ir]) {
+        z[nz] = obs2->tpz[ir2];
+        for (int id = 0; id < ctl->nd; id++) {
+          rr[id][nz] = obs2->rad[id][ir2];
+          tt[id][nz] = obs2->tau[id][ir2];
+        }
+        nz++;
+      }
+    if (nz < 2) OFAIL("Cannot apply FOV convolution!");
+    double wsum = 0;
+    for (int id = 0; id < ctl->nd; id++) obs->rad[id][ir] = obs->tau[id][ir] = 0;
+    for (int i = 0; i < n; i++) {
+      double zfov = obs->tpz[ir] + dz[i];
+      int idx = o_locate(z, nz, zfov);
+      for (int id = 0; id < ctl->nd; id++) {
+        obs->rad[id][ir] += w[i] * o_lin(z[idx], rr[id][idx], z[idx + 1], rr[id][idx + 1], zfov);
+        obs->tau[id][ir] += w[i] * o_lin(z[idx], tt[id][idx], z[idx + 1], tt[id][idx + 1], zfov);
+      }
+      wsum += w[i];
+    }
+    for (int id = 0; id < ctl->nd; id++) {
+      obs->rad[id][ir] /= wsum;
+      obs->tau[id][ir] /= wsum;
+    }
+  }
+  free(z);
+  free(rr);
+  free(tt);
+  free(obs2);
+}
+
+/* formod (forwardmodel.c:15-116), non-queue execution order. */
+void oracle_formod(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero) {
+  static int mask[NDMAX][NRMAX];
+  for (int id = 0; id < ctl->nd; id++)
+    for (int ir = 0; ir < obs->nr; ir++) mask[id][ir] = !isfinite(obs->rad[id][ir]);
+  oracle_hydrostatic(ctl, atm);
+  if (ctl->retnn || ctl->retrr || ctl->retss) oracle_get_opt_prop(ctl, aero);
+  o_init_tables(ctl);
+#pragma omp parallel for schedule(dynamic)
+  for (int ir = 0; ir < obs->nr; ir++) oracle_formod_pencil(ctl, atm, obs, aero, ctl->sca_mult, ir);
+  /* The reference releases its queue after use, leaving capacity 0. */
+  if (ctl->queue.capacity > 0) ctl->queue.capacity = 0;
+  ctl->queue.state = Queue_Inactive;
+  o_formod_fov(ctl, obs);
+  if (ctl->write_bbt)
+    for (int ir = 0; ir < obs->nr; ir++)
+      for (int id = 0; id < ctl->nd; id++)
+        obs->rad[id][ir] = oracle_brightness(obs->rad[id][ir], ctl->nu[id]);
+  for (int id = 0; id < ctl->nd; id++)
+    for (int ir = 0; ir < obs->nr; ir++)
+      if (mask[id][ir]) obs->rad[id][ir] = NAN;
+}
+
+/* Batched clear-air forward model, one atmosphere per ray. */
+void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z, const double *p,
+                         const double *t, const double *q, const double *k, const double *time,
+                         const double *obsz, const double *obslon, const double *obslat,
+                         const double *vpz, const double *vplon, const double *vplat, int ray0,
+                         int ray1, int nthreads, double *rad, double *tau, double *tpz,
+                         double *tplon, double *tplat, int *npts) {
+  (void)nray;
+  if (nlev > NPMAX) OFAIL("Too many data points!");
+  o_init_tables(ctl);
+  static aero_t aero0; /* no aerosol in the batched clear-air path */
+#ifdef _OPENMP
+  int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+  int nth = 1;
+  (void)nthreads;
+#endif
+#pragma omp parallel num_threads(nth)
+  {
+    atm_t *atm = (atm_t *)malloc(sizeof(atm_t));
+    double r[NDMAX], tt[NDMAX];
+#pragma omp for schedule(dynamic)
+    for (int ir = ray0; ir < ray1; ir++) {
+      size_t o = (size_t)ir * nlev;
+      atm->np = nlev;
+      atm->init = 0;
+      for (int l = 0; l < nlev; l++) {
+        atm->time[l] = time[ir];
+        atm->z[l] = z[o + l];
+        atm->lon[l] = 0;
+        atm->lat[l] = 0;
+        atm->p[l] = p[o + l];
+        atm->t[l] = t[o + l];
+      }
+      for (int ig = 0; ig < ctl->ng; ig++)
+        for (int l = 0; l < nlev; l++) atm->q[ig][l] = q[((size_t)ir * ctl->ng + ig) * nlev + l];
+      for (int iw = 0; iw < ctl->nw; iw++)
+        for (int l = 0; l < nlev; l++)
+          atm->k[iw][l] = k ? k[((size_t)ir * ctl->nw + iw) * nlev + l] : 0;
+      oray_t ray = {time[ir], obsz[ir], obslon[ir], obslat[ir], vpz[ir], vplon[ir], vplat[ir], 0, 0, 0};
+      o_pencil(ctl, atm, &aero0, &ray, 0, r, tt);
+      for (int id = 0; id < ctl->nd; id++) {
+        rad[(size_t)ir * ctl->nd + id] = r[id];
+        tau[(size_t)ir * ctl->nd + id] = tt[id];
+      }
+      if (tpz) tpz[ir] = ray.tpz;
+      if (tplon) tplon[ir] = ray.tplon;
+      if (tplat) tplat[ir] = ray.tplat;
+      if (npts) npts[ir] = o_scratch[0] ? o_scratch[0]->np : 0;
+    }
+    free(atm);
+  }
+}

@@ -1,0 +1,102 @@
+/*
+ * oracle.h -- FP64 CPU restatement of the reference forward model.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU library is
+ * compared against (and the CPU baseline that the benchmark times beside it).
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+ * load it; the product library never links or calls it.
+ *
+ * Every function restates the arithmetic of the reference file:line cited at
+ * its definition in oracle.c, in FP64, without GSL.  The functions that touch
+ * the reference's GSL call sites restate GSL 1.15's published algorithms
+ * (gsl_expm1, gsl_log1p, gsl_complex_div/abs, gsl_stats_min_index, gsl_sort).
+ *
+ * Parity pinning (what checks each part; see tests/test_oracle_*.py):
+ *   formod clear air, raytrace, intpol_tbl, planck table, continua (CO2/H2O):
+ *       reference fixtures examples/org/rad_clear.tab, los.0, obs*.tab.
+ *   add_aerosol_layers, get_opt_prop, bhmie, gauher (beta_a path):
+ *       examples/org/rad_ext1.tab; bhmie also against Rayleigh limit,
+ *       k=0 energy conservation, phase normalisation; gauher against exact
+ *       Gaussian moments.
+ *   srcfunc_sca_1d, srcfunc_sca_sun, scattering radiance (depth 1):
+ *       examples/org/rad_aero0.tab, rad_aero1.tab; isotropic-field closure.
+ *   suncoord: solstice/equinox declination and local-noon checks.
+ *   brightness: closed-form inverse of planck (no reference fixture uses it).
+ *   hydrostatic: closed form for an isothermal dry column.
+ *   ctmn2, ctmo2: parity unpinned by fixtures (no fixture channel lies in
+ *       their bands); checked against direct evaluation at table nodes.
+ *   formod_fov: parity unpinned by fixtures; checked against a delta-shaped
+ *       field of view reproducing the unconvolved radiance.
+ *   Scattering depth >= 2: parity unpinned by fixtures (no reference test
+ *       uses SCA_MULT >= 2); checked for the isotropic-field closure only.
+ */
+#ifndef JURASSIC_ORACLE_H
+#define JURASSIC_ORACLE_H
+
+#include "jurassic_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Control file parsing (reference src/control.c:7-171). */
+void oracle_read_ctl(int argc, char *argv[], ctl_t *ctl);
+double oracle_scan_ctl(int argc, char *argv[], const char *varname, int arridx,
+                       const char *defvalue, char *value);
+
+/* File I/O (reference src/atmosphere.c:487-541, src/misc.c:269-390,
+   src/scatter.c:583-639). */
+void oracle_read_atm(const char *dirname, const char *filename, ctl_t *ctl, atm_t *atm);
+void oracle_read_obs(const char *dirname, const char *filename, ctl_t *ctl, obs_t *obs);
+void oracle_write_obs(const char *dirname, const char *filename, ctl_t *ctl, obs_t *obs);
+void oracle_read_aero(const char *dirname, const char *filename, ctl_t *ctl, aero_t *aero);
+
+/* Optical properties of the aerosol layers (reference src/scatter.c:330-508). */
+void oracle_get_opt_prop(ctl_t *ctl, aero_t *aero);
+void oracle_bhmie(double x, double n_real, double n_imag, double *phase,
+                  double *qext, double *qsca);
+void oracle_gauher(double *x, double *w);
+
+/* The forward model (reference src/forwardmodel.c:15-116). */
+void oracle_formod(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero);
+/* One pencil beam (reference src/forwardmodel.c:224-405, non-queue path). */
+void oracle_formod_pencil(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero,
+                          int scattering, int ir);
+/* Ray tracing into a caller-owned los_t (reference src/lineofsight.c:5-190). */
+void oracle_raytrace(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, los_t *los, int ir);
+
+/* Point functions. */
+double oracle_planck(double t, double nu);          /* forwardmodel.c:569-573 */
+double oracle_brightness(double rad, double nu);    /* forwardmodel.c:7-11 */
+void oracle_suncoord(double sec, double lon, double lat, double *azi, double *sza); /* scatter.c:999-1041 */
+void oracle_hydrostatic(ctl_t *ctl, atm_t *atm);    /* atmosphere.c:163-243 */
+/* Continuum absorption of one LOS point for all channels (forwardmodel.c:120-152). */
+void oracle_continua(ctl_t *ctl, double p, double t, const double *q, const double *u,
+                     double ds, const double *k, double *beta);
+double oracle_ctmn2(double nu, double p, double t);  /* continua.c:1935-1993 */
+double oracle_ctmo2(double nu, double p, double t);  /* continua.c:1997-2049 */
+/* Filter-weighted Planck source function for all channels (forwardmodel.c:743-794). */
+void oracle_srcfunc_planck(ctl_t *ctl, double t, double *src);
+
+/*
+ * Batched clear-air / extinction forward model with one atmosphere per ray
+ * (the shape of the batched GPU entry point).  Profiles are ray-major:
+ * z,p,t [nray][nlev], q [nray][ng][nlev], k [nray][nw][nlev].  Outputs rad,
+ * tau [nray][nd] and the tangent point [nray].  Rays ray0..ray1-1 only.
+ * nthreads <= 0 uses the OpenMP default.
+ */
+void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z,
+                         const double *p, const double *t, const double *q,
+                         const double *k, const double *time, const double *obsz,
+                         const double *obslon, const double *obslat, const double *vpz,
+                         const double *vplon, const double *vplat, int ray0, int ray1,
+                         int nthreads, double *rad, double *tau, double *tpz,
+                         double *tplon, double *tplat, int *npts);
+
+/* Drop the cached tables, filter functions and Planck tables. */
+void oracle_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
