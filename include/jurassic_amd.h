@@ -1,0 +1,117 @@
+/*
+ * jurassic_amd.h -- C ABI of the MI355X forward-model library
+ * (libjurassic_amd.so).
+ *
+ * The first group of functions has the reference's names and signatures, so
+ * a caller of the reference forward model links against this library
+ * instead of the reference objects: the structs are layout-identical
+ * (jurassic_types.h), errors stop the process with the reference's message
+ * format, and the work runs on the GPU (device 0 unless jur_init() picked
+ * another).  Not reentrant: one caller thread, like the reference.
+ *
+ * The second group (jur_*) is new: a batched entry point with one atmosphere
+ * per ray and any number of rays (the reference's obs_t stops at NRMAX=1000
+ * rays and one atmosphere), its device-pointer variant, and control / stats.
+ */
+#ifndef JURASSIC_AMD_H
+#define JURASSIC_AMD_H
+
+#include "jurassic_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Drop-in entry points (reference interfaces) ---------------------- */
+
+/* Forward model for all rays of obs (reference src/forwardmodel.h:16-19,
+   forwardmodel.c:15-116): NaN mask, hydrostatic equilibrium, optional Mie
+   refresh, ray tracing, path integration with single/multiple scattering,
+   FOV convolution, brightness temperature.  Writes obs->rad, obs->tau and
+   the tangent points. */
+void formod(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero);
+
+/* One pencil beam, ray ir, with the given scattering depth (reference
+   src/forwardmodel.h:32-37, forwardmodel.c:224-405, non-queue semantics). */
+void formod_pencil(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, int scattering, int ir);
+
+/* Ray tracing of ray ir into los (reference src/lineofsight.h, lineofsight.c:5-190,
+   including add_aerosol_layers :193-372 when SCA_N > 0). */
+void raytrace(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, los_t *los, int ir);
+
+/* Layer optical properties from the aerosol models with Mie theory
+   (reference src/scatter.h, scatter.c:330-508). */
+void get_opt_prop(ctl_t *ctl, aero_t *aero);
+
+/* Mie efficiencies and phase function of one sphere (reference scatter.c:56-234). */
+void bhmie(double x, double n_real, double n_imag, double *phase, double *qext, double *qsca);
+
+/* Hydrostatic pressure profile (reference src/atmosphere.c:163-184). */
+void hydrostatic(ctl_t *ctl, atm_t *atm);
+
+/* Control file and command-line keys (reference src/control.c:7-171). */
+void read_ctl(int argc, char *argv[], ctl_t *ctl);
+double scan_ctl(int argc, char *argv[], const char *varname, int arridx, const char *defvalue,
+                char *value);
+
+/* Text file formats (reference src/atmosphere.c:487-598, src/misc.c:269-390,
+   src/scatter.c:583-639 and 1045-1092). */
+void read_atm(const char *dirname, const char *filename, ctl_t *ctl, atm_t *atm);
+void write_atm(const char *dirname, const char *filename, ctl_t *ctl, atm_t *atm);
+void read_obs(const char *dirname, const char *filename, ctl_t *ctl, obs_t *obs);
+void write_obs(const char *dirname, const char *filename, ctl_t *ctl, obs_t *obs);
+void read_aero(const char *dirname, const char *filename, ctl_t *ctl, aero_t *aero);
+void write_aero(const char *dirname, const char *filename, aero_t *aero);
+
+/* Point formulas (reference src/forwardmodel.c:7-11 and :569-573). */
+double brightness(double rad, double nu);
+double planck(double t, double nu);
+
+/* ---- Batched extension (no reference counterpart) --------------------- */
+
+/* Rays with their own atmospheres.  Profiles: z, p, t [natm][nlev];
+   q [natm][ng][nlev]; k [natm][nw][nlev] or NULL for zero extinction.
+   atm_of_ray [nray] selects a profile per ray; NULL means ray i uses profile
+   i when natm == nray, else profile 0.  Outputs: rad, tau [nd][nray]
+   (channel-major like obs_t, FP32); tpz/tplon/tplat [nray] and npts [nray]
+   (LOS points) are optional. */
+typedef struct {
+  int nray, nlev, natm;
+  const int *atm_of_ray;
+  const double *z, *p, *t, *q, *k;
+  const double *time, *obsz, *obslon, *obslat, *vpz, *vplon, *vplat;
+  float *rad, *tau;
+  double *tpz, *tplon, *tplat;
+  int *npts;
+} jur_batch_t;
+
+/* All pointers of b are host pointers.  aero may be NULL when ctl->sca_n is 0;
+   scattering is the recursion depth (ctl->sca_mult for formod semantics).
+   Returns 0. */
+int jur_formod_batch(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero, int scattering);
+
+/* All pointers of b are device pointers already resident in GPU memory;
+   stream is a hipStream_t (NULL: the library's stream).  Runs asynchronously
+   on that stream and returns 0 after enqueueing; errors found on the device
+   (LOS buffer growth aside) are reported by jur_sync(). */
+int jur_formod_batch_device(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero, int scattering,
+                            void *stream);
+
+/* Wait for the library stream and check device-side error flags. */
+int jur_sync(void);
+
+/* Select the GPU (call before anything else; default device 0). */
+int jur_init(int device);
+
+/* Tuning and instrumentation.  Options: "cpw" channels per wave (1,2,4,8),
+   "wpb" waves per block, "cap" initial LOS points per ray, "graph" replay the
+   batched device call from a captured hipGraph (0/1). */
+int jur_set_option(const char *key, double value);
+/* Statistics of the last call: "points" (LOS points of primaries),
+   "pencils" (all pencils), "path_ms" (path kernel time), "trace_ms". */
+double jur_stat(const char *key);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,483 @@
+// engine.cpp -- device context of the forward model: emissivity tables,
+// source-function tables and continuum constants resident in HBM, aerosol
+// layer optics, work buffers, and the clear-air / extinction pipeline
+// (ray tracing -> sort by path length -> path integration).
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <strings.h>
+
+#include "continua_coeffs.inc"
+#include "jur_host.h"
+#include "jur_kernels_misc.h"
+#include "jur_scatter.h"
+
+namespace {
+
+struct Engine {
+  bool ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // table set
+  std::string tbl_key, chan_key;
+  int ng = 0, nd = 0;
+  DBuf t_np, t_p, t_nt, t_t, t_ncol, t_col, t_ue, planck, chan, nu_d;
+  // aerosol layers
+  DBuf beta_e, beta_s, beta_a, alti;
+  int nalti = 0;
+  double altimax = 0, altimin = 0;
+  int nl = 0;
+  double top[NLMAX] = {0}, bottom[NLMAX] = {0}, trans[NLMAX] = {0};
+  // work buffers of the primary pass
+  DBuf geo, geo2, rec, npts, tsurf, status, ids, order, keys, sorttmp;
+  int cap = 512;
+  std::map<std::string, double> opt, stats;
+};
+
+Engine& E() {
+  static Engine e;
+  if (!e.ready) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      JUR_FAIL("No GPU found: the forward model needs an MI355X (gfx950) device.");
+    HIP_OK(hipSetDevice(e.device));
+    HIP_OK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    e.ready = true;
+  }
+  return e;
+}
+
+int find_emitter(const ctl_t* ctl, const char* name) {
+  for (int ig = 0; ig < ctl->ng; ig++)
+    if (strcasecmp(ctl->emitter[ig], name) == 0) return ig;
+  return -1;
+}
+
+template <typename T>
+T* upload(DBuf& b, const std::vector<T>& v, hipStream_t s) {
+  T* d = (T*)b.need(v.size() * sizeof(T) + sizeof(T));
+  if (!v.empty()) HIP_OK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return d;
+}
+
+}  // namespace
+
+// Continuum coefficients of channel id (continua.c): the nu-dependent table
+// interpolation is done once here in FP64; the kernel evaluates the T, p
+// and column-density dependence per point.
+ChanConst make_chan_const(const ctl_t* ctl, int id) {
+  ChanConst c;
+  memset(&c, 0, sizeof c);
+  const double nu = ctl->nu[id];
+  c.nu = (float)nu;
+  c.window = ctl->window[id];
+  int ig_co2 = find_emitter(ctl, "CO2"), ig_h2o = find_emitter(ctl, "H2O");
+  double xw = nu / 2 + 1;  // ctmco2 (continua.c:854-869)
+  if (ctl->ctm_co2 && ig_co2 >= 0 && xw >= 1 && xw < 2001) {
+    int iw = (int)xw, jw = iw + 1;
+    double dw = xw - iw, ew = 1 - dw;
+    double cw296 = ew * JUR_CTM_CO2_296[iw - 1] + dw * JUR_CTM_CO2_296[jw - 1];
+    double cw260 = ew * JUR_CTM_CO2_260[iw - 1] + dw * JUR_CTM_CO2_260[jw - 1];
+    double cw230 = ew * JUR_CTM_CO2_230[iw - 1] + dw * JUR_CTM_CO2_230[jw - 1];
+    c.co2_on = 1;
+    c.co2_ig = ig_co2;
+    c.co2_k230 = (float)(5.050505e-4 * cw230);
+    c.co2_k260 = (float)(9.259259e-4 * cw260);
+    c.co2_k296 = (float)(4.208754e-4 * cw296);
+  }
+  xw = nu / 10 + 1;  // ctmh2o (continua.c:1898-1928)
+  if (ctl->ctm_h2o && ig_h2o >= 0 && xw >= 1 && xw < 2001) {
+    int iw = (int)xw, jw = iw + 1;
+    double dw = xw - iw, ew = 1 - dw, sfac;
+    double cw296 = ew * JUR_CTM_H2O_SELF296[iw - 1] + dw * JUR_CTM_H2O_SELF296[jw - 1];
+    double cw260 = ew * JUR_CTM_H2O_SELF260[iw - 1] + dw * JUR_CTM_H2O_SELF260[jw - 1];
+    double cwfrn = ew * JUR_CTM_H2O_FOREIGN[iw - 1] + dw * JUR_CTM_H2O_FOREIGN[jw - 1];
+    if (nu <= 820 || nu >= 960) {
+      sfac = 1;
+    } else {
+      double xx = (nu - 820) / 10;
+      int ix = (int)xx;
+      double dx = xx - ix;
+      sfac = (1 - dx) * JUR_CTM_H2O_XFCREV[ix] + dx * JUR_CTM_H2O_XFCREV[ix + 1];
+    }
+    double vf2 = (nu - 370) * (nu - 370), vf6 = vf2 * vf2 * vf2;
+    double fscal = 36100 / (vf2 + vf6 * 1e-8 + 36100) * -.25 + 1;
+    c.h2o_on = 1;
+    c.h2o_ig = ig_h2o;
+    c.h2o_s296 = (float)(sfac * cw296);
+    c.h2o_lnratio = (float)log(cw260 / cw296);
+    c.h2o_frn = (float)(cwfrn * fscal);
+    c.h2o_tanhk = (float)(.7193876 * nu);
+  }
+  auto band = [&](const double* nua, const double* ba, const double* bb, int n, float& b, float& beta) {
+    if (nu < nua[0] || nu > nua[n - 1]) return 0;
+    int idx = locate_mono<double>(nua, n, nu);
+    b = (float)lin(nua[idx], ba[idx], nua[idx + 1], ba[idx + 1], nu);
+    beta = (float)lin(nua[idx], bb[idx], nua[idx + 1], bb[idx + 1], nu);
+    return 1;
+  };
+  if (ctl->ctm_n2) c.n2_on = band(JUR_CTM_N2_NU, JUR_CTM_N2_B, JUR_CTM_N2_BETA, 98, c.n2_b, c.n2_beta);
+  if (ctl->ctm_o2) c.o2_on = band(JUR_CTM_O2_NU, JUR_CTM_O2_B, JUR_CTM_O2_BETA, 90, c.o2_b, c.o2_beta);
+  return c;
+}
+
+hipStream_t engine_stream() { return E().stream; }
+
+int engine_option_int(const char* key, int dflt) {
+  auto& m = E().opt;
+  auto it = m.find(key);
+  return it == m.end() ? dflt : (int)it->second;
+}
+double engine_stat(const char* key) {
+  auto& m = E().stats;
+  auto it = m.find(key);
+  return it == m.end() ? NAN : it->second;
+}
+void engine_set_stat(const char* key, double v) { E().stats[key] = v; }
+
+void engine_prepare(const ctl_t* ctl) {
+  Engine& e = E();
+  hipStream_t s = e.stream;
+  // Tables and source-function tables: loaded once per table set (the
+  // reference loads them at its first call, forwardmodel.c:286-292).
+  std::string key = std::string(ctl->tblbase) + "|";
+  for (int ig = 0; ig < ctl->ng; ig++) key += std::string(ctl->emitter[ig]) + ",";
+  char buf[64];
+  for (int id = 0; id < ctl->nd; id++) {
+    snprintf(buf, sizeof buf, "%.17g,", ctl->nu[id]);
+    key += buf;
+  }
+  if (key != e.tbl_key) {
+    const int ng = ctl->ng, nd = ctl->nd, ntbl = ng * nd;
+    std::vector<int> np(ntbl, -1), nt((size_t)ntbl * TBLNPMAX, 0), ncol((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0),
+        col((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0);
+    std::vector<float> p((size_t)ntbl * TBLNPMAX, 0.f), t((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0.f);
+    std::vector<float2> ue;
+    static char fn[2 * LEN + 64];
+    for (int id = 0; id < nd; id++)
+      for (int ig = 0; ig < ng; ig++) {
+        HostTable tb;
+        snprintf(fn, sizeof fn, "%s_%.4f_%s.bin", ctl->tblbase, ctl->nu[id], ctl->emitter[ig]);
+        bool ok = read_table_bin(fn, tb);
+        if (!ok) {
+          snprintf(fn, sizeof fn, "%s_%.4f_%s.tab", ctl->tblbase, ctl->nu[id], ctl->emitter[ig]);
+          ok = read_table_ascii(fn, tb);
+          if (!ok) printf("Missing emissivity table: %s\n", fn);
+        }
+        const int k = id * ng + ig;  // channel-contiguous table order
+        if (!ok) continue;
+        np[k] = tb.np;
+        for (int ip = 0; ip < tb.np; ip++) {
+          p[(size_t)k * TBLNPMAX + ip] = (float)tb.p[ip];
+          nt[(size_t)k * TBLNPMAX + ip] = tb.nt[ip];
+          for (int it = 0; it < tb.nt[ip]; it++) {
+            size_t c = ((size_t)k * TBLNPMAX + ip) * TBLNTMAX + it;
+            t[c] = (float)tb.t[ip][it];
+            ncol[c] = tb.nu[ip][it];
+            col[c] = (int)ue.size();
+            for (int iu = 0; iu < tb.nu[ip][it]; iu++) ue.push_back(make_float2(tb.u[ip][it][iu], tb.eps[ip][it][iu]));
+            ue.push_back(make_float2(0.f, 0.f));  // pad so col[i+1] stays in bounds
+          }
+        }
+      }
+    upload(e.t_np, np, s);
+    upload(e.t_p, p, s);
+    upload(e.t_nt, nt, s);
+    upload(e.t_t, t, s);
+    upload(e.t_ncol, ncol, s);
+    upload(e.t_col, col, s);
+    upload(e.t_ue, ue, s);
+    // Filter functions -> Planck table on the device.
+    std::vector<double> fnu, fw;
+    std::vector<int> foff, fnn;
+    std::vector<double> x, y;
+    for (int id = 0; id < nd; id++) {
+      snprintf(fn, sizeof fn, "%s_%.4f.filt", ctl->tblbase, ctl->nu[id]);
+      read_shape_file(fn, x, y);
+      foff.push_back((int)fnu.size());
+      fnn.push_back((int)x.size());
+      fnu.insert(fnu.end(), x.begin(), x.end());
+      fw.insert(fw.end(), y.begin(), y.end());
+    }
+    DBuf dfnu, dfw, dfoff, dfn;
+    upload(dfnu, fnu, s);
+    upload(dfw, fw, s);
+    upload(dfoff, foff, s);
+    upload(dfn, fnn, s);
+    e.planck.need((size_t)nd * JUR_NPLANCK * sizeof(float));
+    launch_planck_table(dfnu.as<double>(), dfw.as<double>(), dfoff.as<int>(), dfn.as<int>(), nullptr, nd,
+                        e.planck.as<float>(), s);
+    HIP_OK(hipStreamSynchronize(s));
+    dfnu.release();
+    dfw.release();
+    dfoff.release();
+    dfn.release();
+    e.tbl_key = key;
+    e.ng = ng;
+    e.nd = nd;
+    printf("Initialize source function table...\n");
+  }
+  // Channel constants (depend on the continua switches and windows too).
+  std::string ck = key;
+  snprintf(buf, sizeof buf, "|%d%d%d%d", ctl->ctm_co2, ctl->ctm_h2o, ctl->ctm_n2, ctl->ctm_o2);
+  ck += buf;
+  for (int id = 0; id < ctl->nd; id++) ck += std::to_string(ctl->window[id]) + ",";
+  if (ck != e.chan_key) {
+    std::vector<ChanConst> cc(ctl->nd);
+    std::vector<double> nu(ctl->nd);
+    for (int id = 0; id < ctl->nd; id++) {
+      cc[id] = make_chan_const(ctl, id);
+      nu[id] = ctl->nu[id];
+    }
+    upload(e.chan, cc, s);
+    upload(e.nu_d, nu, s);
+    HIP_OK(hipStreamSynchronize(s));
+    e.chan_key = ck;
+  }
+}
+
+// Aerosol layers for the ray tracer (layer-edge altitudes of
+// add_aerosol_layers, lineofsight.c:211-244) and layer optics for the path
+// integration.
+void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
+  Engine& e = E();
+  hipStream_t s = e.stream;
+  const int nd = ctl->nd;
+  std::vector<float> be((size_t)NLMAX * nd, 0.f), bs((size_t)NLMAX * nd, 0.f), ba((size_t)NLMAX * nd, 0.f);
+  std::vector<double> alti;
+  e.nl = 0;
+  if (aero && ctl->sca_n > 0) {
+    e.nl = aero->nl;
+    const double eps = 0.005;
+    for (int il = 0; il < aero->nl && il < NLMAX; il++) {
+      e.top[il] = aero->top[il];
+      e.bottom[il] = aero->bottom[il];
+      e.trans[il] = aero->trans[il];
+      for (int id = 0; id < nd; id++) {
+        be[(size_t)il * nd + id] = (float)aero->beta_e[il][id];
+        bs[(size_t)il * nd + id] = (float)aero->beta_s[il][id];
+        ba[(size_t)il * nd + id] = (float)aero->beta_a[il][id];
+      }
+      alti.push_back(aero->top[il] + eps);
+      alti.push_back(aero->top[il] - eps);
+      alti.push_back(aero->bottom[il] + eps);
+      alti.push_back(aero->bottom[il] - eps);
+      if (aero->trans[il] > ctl->transs) {
+        double tt = aero->trans[il] / ctl->transs;
+        alti.push_back(aero->top[il] + aero->trans[il] + eps);
+        alti.push_back(aero->top[il] + aero->trans[il] - eps);
+        alti.push_back(aero->bottom[il] - aero->trans[il] + eps);
+        alti.push_back(aero->bottom[il] - aero->trans[il] - eps);
+        for (int it = 1; it < (int)tt; it++) {
+          alti.push_back(aero->top[il] + aero->trans[il] - eps - it * ctl->transs);
+          alti.push_back(aero->bottom[il] - aero->trans[il] + eps + it * ctl->transs);
+        }
+      }
+    }
+    // descending order (the reference sorts the negated values ascending)
+    for (size_t i = 1; i < alti.size(); i++) {
+      double v = alti[i];
+      size_t j = i;
+      while (j > 0 && alti[j - 1] < v) {
+        alti[j] = alti[j - 1];
+        j--;
+      }
+      alti[j] = v;
+    }
+  }
+  e.nalti = (int)alti.size();
+  e.altimax = e.nalti ? alti.front() : 0;
+  e.altimin = e.nalti ? alti.back() : 0;
+  upload(e.beta_e, be, s);
+  upload(e.beta_s, bs, s);
+  upload(e.beta_a, ba, s);
+  upload(e.alti, alti, s);
+  scatter_set_phase(ctl, aero, s);
+  HIP_OK(hipStreamSynchronize(s));
+}
+
+// Trace a set of rays into the given buffers; grows the per-ray capacity
+// until every ray fits.  Returns the capacity used.
+int engine_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on,
+                 TraceBufs& tb, double* tpz, double* tplon, double* tplat, hipStream_t s) {
+  Engine& e = E();
+  const int R = rec_len(ctl->ng, ctl->nw);
+  int cap = tb.cap > 0 ? tb.cap : e.cap;
+  int ig_h2o = find_emitter(ctl, "H2O");
+  while (true) {
+    const size_t npt = (size_t)rays.nray * cap;
+    TraceArgs A;
+    memset(&A, 0, sizeof A);
+    A.nray = rays.nray;
+    A.obsz = rays.obsz;
+    A.obslon = rays.obslon;
+    A.obslat = rays.obslat;
+    A.vpz = rays.vpz;
+    A.vplon = rays.vplon;
+    A.vplat = rays.vplat;
+    A.atm_idx = rays.atm_idx;
+    A.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw};
+    A.rayds = ctl->rayds;
+    A.raydz = ctl->raydz;
+    A.transs = ctl->transs;
+    A.refrac = ctl->refrac;
+    A.sca_n = sca_on ? ctl->sca_n : 0;
+    A.alti = e.alti.as<double>();
+    A.nalti = e.nalti;
+    A.altimax = e.altimax;
+    A.altimin = e.altimin;
+    A.nl = e.nl;
+    for (int il = 0; il < NLMAX; il++) {
+      A.top[il] = e.top[il];
+      A.bottom[il] = e.bottom[il];
+      A.trans[il] = e.trans[il];
+    }
+    A.nw = ctl->nw;
+    A.ig_h2o = ig_h2o;
+    A.R = R;
+    A.cap = cap;
+    A.geo = (double4*)tb.geo->need(npt * sizeof(double4));
+    A.geo2 = A.sca_n > 0 ? (double4*)tb.geo2->need(npt * sizeof(double4)) : nullptr;
+    A.rec = (float*)tb.rec->need(npt * R * sizeof(float));
+    A.np_out = (int*)tb.npts->need(rays.nray * sizeof(int));
+    A.tsurf_out = (float*)tb.tsurf->need(rays.nray * sizeof(float));
+    A.tsurf64 = tb.tsurf64 ? (double*)tb.tsurf64->need(rays.nray * sizeof(double)) : nullptr;
+    A.geo_final = tb.geo_final ? (int*)tb.geo_final->need(rays.nray * sizeof(int)) : nullptr;
+    A.tpz = tpz;
+    A.tplon = tplon;
+    A.tplat = tplat;
+    A.status = (int*)e.status.need(sizeof(int) * 4);
+    launch_zero_int(A.status, 1, s);
+    launch_raytrace(A, s);
+    int st = 0;
+    HIP_OK(hipMemcpyAsync(&st, A.status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (st & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
+    if (st & JUR_ST_NLOS) JUR_FAIL("Too many LOS points!");
+    if (st & JUR_ST_CAP) {
+      if (cap > NLOS) JUR_FAIL("Too many LOS points!");
+      cap = cap * 2 > NLOS + 2 ? NLOS + 2 : cap * 2;
+      continue;
+    }
+    tb.cap = cap;
+    if (cap > e.cap) e.cap = cap;
+    return cap;
+  }
+}
+
+void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int scattering,
+                    const OutDev& out, hipStream_t s) {
+  Engine& e = E();
+  if (rays.nray <= 0) return;
+  if (ctl->ip != 1) JUR_FAIL("Only IP=1 (vertical profile interpolation) is supported!");
+  const int sca_on = ctl->sca_n > 0;
+  if (sca_on && scattering > 0) {
+    scatter_forward(ctl, atm, rays, scattering, out, s);
+    return;
+  }
+  hipEvent_t ev[3];
+  for (auto& x : ev) HIP_OK(hipEventCreate(&x));
+  HIP_OK(hipEventRecord(ev[0], s));
+  TraceBufs tb{&e.geo, &e.geo2, &e.rec, &e.npts, &e.tsurf, nullptr, nullptr, 0};
+  int cap = engine_trace(ctl, atm, rays, sca_on, tb, out.tpz, out.tplon, out.tplat, s);
+  HIP_OK(hipEventRecord(ev[1], s));
+  engine_integrate(ctl, rays.nray, cap, e.rec.as<float>(), e.npts.as<int>(), e.tsurf.as<float>(), 0, nullptr,
+                   nullptr, out.rad, out.tau, rays.nray, 0, s);
+  HIP_OK(hipEventRecord(ev[2], s));
+  if (out.npts) HIP_OK(hipMemcpyAsync(out.npts, e.npts.p, rays.nray * sizeof(int), hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipEventSynchronize(ev[2]));
+  float t1 = 0, t2 = 0;
+  HIP_OK(hipEventElapsedTime(&t1, ev[0], ev[1]));
+  HIP_OK(hipEventElapsedTime(&t2, ev[1], ev[2]));
+  e.stats["trace_ms"] = t1;
+  e.stats["path_ms"] = t2;
+  e.stats["pencils"] = rays.nray;
+  for (auto& x : ev) HIP_OK(hipEventDestroy(x));
+}
+
+// Sort rays by point count and run the path integration.
+void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, const int* npts,
+                      const float* tsurf, int scattering, const float* J, const int* jbase, float* rad,
+                      float* tau, int out_ld, int out_off, hipStream_t s) {
+  Engine& e = E();
+  int* ids = (int*)e.ids.need(nray * sizeof(int));
+  int* order = (int*)e.order.need(nray * sizeof(int));
+  int* keys = (int*)e.keys.need(nray * sizeof(int));
+  size_t tb = sort_temp_bytes(nray);
+  void* tmp = e.sorttmp.need(tb);
+  launch_iota(ids, nray, s);
+  launch_sort_rays(npts, nullptr, keys, ids, order, tmp, tb, nray, s);
+  PathArgs P;
+  memset(&P, 0, sizeof P);
+  P.nray = nray;
+  P.nd = ctl->nd;
+  P.ng = ctl->ng;
+  P.nw = ctl->nw;
+  P.R = rec_len(ctl->ng, ctl->nw);
+  P.cap = cap;
+  P.rec = rec;
+  P.np = npts;
+  P.tsurf = tsurf;
+  P.order = order;
+  P.T = {e.t_np.as<int>(), e.t_p.as<float>(), e.t_nt.as<int>(), e.t_t.as<float>(), e.t_ncol.as<int>(),
+         e.t_col.as<int>(), e.t_ue.as<float2>()};
+  P.cc = e.chan.as<ChanConst>();
+  P.planck = e.planck.as<float>();
+  P.beta_e = e.beta_e.as<float>();
+  P.beta_s = e.beta_s.as<float>();
+  P.beta_a = e.beta_a.as<float>();
+  P.sca_n = ctl->sca_n;
+  P.ext_beta_e = strcmp(ctl->sca_ext, "beta_a") != 0;
+  P.scattering = scattering;
+  P.J = J;
+  P.jbase = jbase;
+  P.rad = rad;
+  P.tau = tau;
+  P.out_ld = out_ld;
+  P.out_off = out_off;
+  P.cpw = engine_option_int("cpw", 4);
+  launch_path(P, engine_option_int("wpb", 4), s);
+}
+
+extern "C" int jur_set_option(const char* key, double value) {
+  E().opt[key] = value;
+  return 0;
+}
+extern "C" double jur_stat(const char* key) { return engine_stat(key); }
+extern "C" int jur_init(int device) {
+  Engine& e = E();
+  if (device != e.device) {
+    e.device = device;
+    HIP_OK(hipSetDevice(device));
+    HIP_OK(hipStreamDestroy(e.stream));
+    HIP_OK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    e.tbl_key.clear();
+    e.chan_key.clear();
+  }
+  return 0;
+}
+extern "C" int jur_sync(void) {
+  Engine& e = E();
+  HIP_OK(hipStreamSynchronize(e.stream));
+  if (e.status.p) {
+    int st = 0;
+    HIP_OK(hipMemcpy(&st, e.status.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (st & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
+    if (st & (JUR_ST_NLOS | JUR_ST_CAP)) JUR_FAIL("Too many LOS points!");
+    if (st & JUR_ST_QUEUE) JUR_FAIL("Too many queue items!");
+  }
+  return 0;
+}
+
+DevTables engine_tables() {
+  Engine& e = E();
+  return {e.t_np.as<int>(), e.t_p.as<float>(), e.t_nt.as<int>(), e.t_t.as<float>(), e.t_ncol.as<int>(),
+          e.t_col.as<int>(), e.t_ue.as<float2>()};
+}
+const double* engine_nu_dev() { return E().nu_d.as<double>(); }
+AeroGeom engine_aero_geom() {
+  Engine& e = E();
+  return {e.alti.as<double>(), e.nalti, e.altimax, e.altimin};
+}
+int engine_cap() { return E().cap; }

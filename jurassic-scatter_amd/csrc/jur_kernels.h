@@ -1,0 +1,76 @@
+// jur_kernels.h -- kernel argument blocks and launchers (host <-> device).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "jur_dev.h"
+
+// Status bits a kernel raises in a device word; the host turns them into the
+// reference's error exits.
+enum {
+  JUR_ST_CAP = 1,            // LOS buffer too small (host retries bigger)
+  JUR_ST_NLOS = 2,           // "Too many LOS points!"
+  JUR_ST_BELOW_SURFACE = 4,  // "Observer below surface!"
+  JUR_ST_QUEUE = 8,          // device queue overflow
+};
+
+struct TraceArgs {
+  int nray;
+  const double *obsz, *obslon, *obslat, *vpz, *vplon, *vplat;
+  const int* atm_idx;  // atmosphere of each ray (null: all use atmosphere 0)
+  DevAtm atm;
+  double rayds, raydz, transs;
+  int refrac;
+  int sca_n;           // > 0: insert aerosol-layer edge points
+  const double* alti;  // layer-edge altitudes sorted descending (device)
+  int nalti;
+  double altimax, altimin;
+  int nl;
+  double top[NLMAX], bottom[NLMAX], trans[NLMAX];
+  int nw, ig_h2o, R, cap;
+  double4* geo;        // [nray][cap] march points (z, lon, lat, ds)
+  double4* geo2;       // [nray][cap] points after layer insertion
+  float* rec;          // [nray][cap][R] FP32 point records
+  int* np_out;
+  float* tsurf_out;
+  double* tsurf64;     // optional
+  double *tpz, *tplon, *tplat;
+  int* geo_final;      // optional: 1 if the final points are in geo2
+  int* status;
+};
+void launch_raytrace(const TraceArgs& A, hipStream_t s);
+
+struct PathArgs {
+  int nray, nd, ng, nw, R, cap;
+  const float* rec;
+  const int* np;
+  const float* tsurf;
+  const int* order;        // rays sorted by descending point count
+  DevTables T;
+  const ChanConst* cc;     // [nd]
+  const float* planck;     // [nd][JUR_NPLANCK]
+  const float *beta_e, *beta_s, *beta_a;  // [NLMAX][nd] layer optics
+  int sca_n;
+  int ext_beta_e;          // leaf extinction: 1 = beta_e, 0 = beta_a
+  int scattering;          // > 0: points with aerofac > 0 use the source J
+  const float* J;          // [npoints][nd]
+  const int* jbase;        // [nray] index of the ray's first scattering point
+  float* rad;              // [nd][out_ld], ray r at column out_off + r
+  float* tau;
+  int out_ld, out_off;
+  int cpw;                 // channels per wave
+};
+// Launch the path integration with the given waves per block.
+void launch_path(const PathArgs& A, int waves_per_block, hipStream_t s);
+
+// Filter-weighted Planck table [nd][JUR_NPLANCK] (forwardmodel.c:743-786).
+void launch_planck_table(const double* filt_nu, const double* filt_w, const int* filt_off,
+                         const int* filt_n, const double* nu_unused, int nd, float* table,
+                         hipStream_t s);
+
+// Sort ray ids by descending point count (stable).  temp sized by
+// sort_temp_bytes.
+size_t sort_temp_bytes(int n);
+void launch_sort_rays(const int* np, int* keys_tmp, int* keys_out, int* ids_in, int* ids_out,
+                      void* temp, size_t temp_bytes, int n, hipStream_t s);
+void launch_iota(int* ids, int n, hipStream_t s);
+void launch_zero_int(int* p, int n, hipStream_t s);

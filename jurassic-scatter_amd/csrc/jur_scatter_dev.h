@@ -1,0 +1,63 @@
+// jur_scatter_dev.h -- data blocks and launchers of kernels_scatter.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "jur_dev.h"
+
+// Secondary pencils per scattering point (reference scatter.c:680, nalpha).
+#define NALPHA 28
+
+// One scattering point: Cartesian position, LOS tangent, layer, atmosphere
+// and time of its pencil.
+struct ScaPt {
+  double x[3];
+  double dx[3];
+  double time;
+  int il;
+  int atm;
+};
+
+// Ray geometry arrays of a ray batch (device).
+struct RayArrays {
+  double *time, *obsz, *obslon, *obslat, *vpz, *vplon, *vplat;
+  int* atm;
+};
+
+struct SunArgs {
+  DevAtm atm;
+  double rayds, raydz;
+  int refrac;
+  const double* alti;  // layer-edge altitudes (descending)
+  int nalti;
+  double altimax, altimin;
+  int* status;
+};
+
+struct AngArgs {
+  int npt, nd;
+  const ScaPt* pts;
+  const double* alpha;       // [NALPHA] fan zenith angles
+  const double* phase;       // [NLMAX][nd][NTHETA] layer phase functions (FP64)
+  const float* rad;          // [nd][nchild] radiances of the fans (child 28 k + j)
+  int nchild;
+  const int* sun_idx;        // [npt] solar pencil of the point or -1
+  const double* sun_theta;   // [npt] scattering angle of the solar beam
+  const float* tau_sun;      // [nd][nsun] solar pencil transmittances
+  int nsun;
+  const double* theta_grid;  // [NTHETA] pi i / 180
+  const double* bsun;        // [nd] Planck radiance of the Sun (TSUN)
+  float* J;                  // [npt][nd]
+};
+
+void launch_count_points(int nray, const int* np, const float* rec, int R, int cap, int* count, hipStream_t s);
+void launch_emit_points(int nray, const int* np, const float* rec, int R, int cap, const double4* geo,
+                        const double4* geo2, const int* geo_final, const int* jbase, const double* time,
+                        const int* atm_idx, ScaPt* pts, hipStream_t s);
+void launch_children(int npt, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s);
+void launch_sun(int npt, const ScaPt* pts, const SunArgs& S, int* valid, double* theta, RayArrays out,
+                hipStream_t s);
+void launch_angular(const AngArgs& G, hipStream_t s);
+void launch_sun_compact(int npt, const int* valid, const int* pos, int* sun_idx, hipStream_t s);
+void launch_gather_rays(int npt, const int* valid, const int* pos, RayArrays in, RayArrays out, hipStream_t s);
+size_t scan_temp_bytes(int n);
+void launch_exclusive_scan(const int* in, int* out, void* tmp, size_t bytes, int n, hipStream_t s);

@@ -1,0 +1,387 @@
+// kernels_path.hip -- path integration: emissivity growth approximation on
+// the look-up tables, continua, Planck source and the radiance /
+// transmittance recursion along each line of sight (reference
+// src/forwardmodel.c:303-399 with intpol_tbl :409-565, formod_continua
+// :120-152 and srcfunc_planck :743-794), in FP32 with Kahan-compensated
+// radiance.
+//
+// Work mapping: one wavefront = 64 rays (lanes) of the ray list sorted by
+// descending point count, times CPW channels (wave-uniform).  All lanes of a
+// wave read the SAME emissivity tables (those of its channels), and
+// neighbouring rays in the sorted list traverse similar altitudes, so the
+// table columns a wave touches overlap strongly in L1/L2.  Blocks are
+// mapped XCD-aware: the blocks covering the channel chunks of one ray group
+// are dispatched to the same XCD so they share its L2 copy of the group's
+// point records.
+//
+// EGA in FP32: per (channel, emitter) the lane carries the cumulative path
+// emissivity ep (= 1 - tau_path of the reference) and the path
+// transmittance tp.  The inverse lookup uses ep directly, and the segment
+// emissivity is formed as (eps_ext - ep) / tp, which equals the reference's
+// 1 - (1 - eps_ext) / tau_path without subtracting numbers close to one.
+#include <hipcub/hipcub.hpp>
+
+#include "jur_dev.h"
+#include "jur_kernels.h"
+
+namespace {
+
+// Emissivity and its complement 1 - eps, carried side by side through the
+// interpolations: eps is exact for transparent paths, the complement for
+// nearly opaque ones (1 - eps of FP32 table nodes >= 0.5 is exact).
+struct EC {
+  float e, c;
+};
+
+// GSL_MAX(GSL_MIN(eps, 1), 0) on the pair; NaN becomes 1 like the reference.
+__device__ inline EC clamp_ec(float e, float c) {
+  if (e != e) return {1.f, 0.f};
+  if (e <= 0.f) return {0.f, 1.f};
+  if (c <= 0.f) return {1.f, 0.f};
+  return {e < 1.f ? e : 1.f, c};
+}
+
+// LIN between two (eps, 1 - eps) nodes.
+__device__ inline EC lin_ec(float x0, EC y0, float x1, EC y1, float x) {
+  float w = (x - x0) / (x1 - x0);
+  return clamp_ec(y0.e + (y1.e - y0.e) * w, y0.c + (y1.c - y0.c) * w);
+}
+
+// One table corner: u at the path emissivity (inverse lookup), then the
+// emissivity of the extended path (forward lookup), clamped to [0, 1]
+// (forwardmodel.c:460-474, 506-542).
+__device__ inline EC corner(const float2* col, int n, float ep, float du) {
+  int i = locate_col<true>(col, n, ep);
+  float2 a = col[i], b = col[i + 1];
+  float u = lin(a.y, a.x, b.y, b.x, ep);
+  float ux = u + du;
+  int j = locate_col<false>(col, n, ux);
+  float2 c = col[j], d = col[j + 1];
+  float s = (d.y - c.y) / (d.x - c.x), dx = ux - c.x;
+  return clamp_ec(c.y + s * dx, (1.f - c.y) - s * dx);
+}
+
+// One EGA step of one emitter (forwardmodel.c:433-500).  ep, tp: path
+// emissivity and transmittance.  Returns the segment emissivity `seg` and
+// transmittance `fac` (= 1 - seg), each formed in the precise domain: the
+// reference's eps = 1 - (1 - eps_ext) / tau_path.
+__device__ inline void ega_step(const DevTables& T, int tb, float p, float t, float du, float& ep,
+                                float& tp, float& seg, float& fac) {
+  const int ntp = T.np[tb];
+  seg = 0.f;
+  fac = 1.f;
+  if (ntp < 2) return;
+  if (tp < 1e-9f) {
+    seg = 1.f;
+    fac = 0.f;
+    ep = 1.f;
+    tp = 0.f;
+    return;
+  }
+  const float* pg = T.p + (size_t)tb * TBLNPMAX;
+  const int ipr = locate_mono<float>(pg, ntp, p);
+  const size_t b0 = (size_t)tb * TBLNPMAX + ipr;
+  const int nt0 = T.nt[b0], nt1 = T.nt[b0 + 1];
+  if (nt0 < 2 || nt1 < 2) return;
+  const float* t0g = T.t + b0 * TBLNTMAX;
+  const float* t1g = t0g + TBLNTMAX;
+  const int it0 = locate_mono<float>(t0g, nt0, t);
+  const int it1 = locate_mono<float>(t1g, nt1, t);
+  const size_t k00 = b0 * TBLNTMAX + it0, k10 = (b0 + 1) * TBLNTMAX + it1;
+  const int n00 = T.ncol[k00], n01 = T.ncol[k00 + 1], n10 = T.ncol[k10], n11 = T.ncol[k10 + 1];
+  if (n00 < 2 || n01 < 2 || n10 < 2 || n11 < 2) return;
+  EC e00 = corner(T.ue + T.col[k00], n00, ep, du);
+  EC e01 = corner(T.ue + T.col[k00 + 1], n01, ep, du);
+  EC e10 = corner(T.ue + T.col[k10], n10, ep, du);
+  EC e11 = corner(T.ue + T.col[k10 + 1], n11, ep, du);
+  e00 = lin_ec(t0g[it0], e00, t0g[it0 + 1], e01, t);
+  e11 = lin_ec(t1g[it1], e10, t1g[it1 + 1], e11, t);
+  e00 = lin_ec(pg[ipr], e00, pg[ipr + 1], e11, p);
+  if (e00.e < 0.5f) {
+    seg = (e00.e - ep) / tp;
+    fac = 1.f - seg;
+  } else {
+    fac = e00.c / tp;
+    seg = 1.f - fac;
+  }
+  ep = e00.e;
+  tp = e00.c;
+}
+
+__device__ inline void kahan_add(float& sum, float& comp, float term) {
+  float y = term - comp;
+  float s = sum + y;
+  comp = (s - sum) - y;
+  sum = s;
+}
+
+// Continuum extinction of one point for one channel [1/km]
+// (forwardmodel.c:120-152; continua.c ctmco2 :854-869, ctmh2o :1898-1928,
+// ctmn2 :1980-1992, ctmo2 :2037-2048).
+__device__ inline float continua(const ChanConst& c, float p, float t, float ds, float qh2o,
+                                 float uco2, float uh2o, float kw) {
+  float beta = kw;
+  if (c.co2_on) {
+    float d230 = t - 230.f, d260 = t - 260.f, d296 = t - 296.f;
+    float ctw = d260 * d296 * c.co2_k230 - d230 * d296 * c.co2_k260 + d230 * d260 * c.co2_k296;
+    beta += uco2 * (float)(1.0 / (JUR_AVOGADRO * 1000.0 * P0)) * p * ctw / ds;
+  }
+  if (c.h2o_on) {
+    float ctwslf = c.h2o_s296 * expf(c.h2o_lnratio * ((296.f - t) / 36.f));
+    float a1 = c.nu * uh2o * tanhf(c.h2o_tanhk / t);
+    float a2 = 296.f / t;
+    float a3 = p * (float)(1.0 / P0) * (qh2o * ctwslf + (1.f - qh2o) * c.h2o_frn) * 1e-20f;
+    beta += a1 * a2 * a3 / ds;
+  }
+  if (c.n2_on || c.o2_on) {
+    float pp = p * (float)(1.0 / P0), tt = 273.f / t;
+    float base = 0.1f * pp * pp * tt * tt;
+    float ex = (1.f / 296.f - 1.f / t);
+    if (c.n2_on)
+      beta += base * 0.79f * c.n2_b * expf(c.n2_beta * ex) *
+              (0.79f + 0.21f * (1.294f - 0.4545f * t / 296.f));
+    if (c.o2_on) beta += base * 0.21f * c.o2_b * expf(c.o2_beta * ex);
+  }
+  return beta;
+}
+
+// Planck table row lookup; the temperature grid is 100 + 0.25 i, i < 1201
+// (forwardmodel.c:771-793: locate on that grid, then linear).
+__device__ inline void planck_index(float t, int& it, float& dt) {
+  float x = (t - 100.f) * 4.f;
+  if (!(x == x)) {
+    it = JUR_NPLANCK - 2;
+  } else {
+    x = floorf(x);
+    it = x < 0.f ? 0 : (x > (float)(JUR_NPLANCK - 2) ? JUR_NPLANCK - 2 : (int)x);
+  }
+  dt = t - (100.f + 0.25f * (float)it);
+}
+__device__ inline float planck_at(const float* row, int it, float dt) {
+  float y0 = row[it], y1 = row[it + 1];
+  return y0 + (y1 - y0) / 0.25f * dt;
+}
+
+template <int NGT, int CPW>
+__global__ void __launch_bounds__(256) k_path(PathArgs A) {
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  // XCD-aware block order: dispatch round-robins blocks over the 8 XCDs, so
+  // logical block L = xcd * per + slot keeps consecutive logical blocks (the
+  // channel chunks of one ray group) on one XCD.
+  const int per = gridDim.x >> 3;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int nchunk = (A.nd + CPW - 1) / CPW;
+  const int bpg = (nchunk + wpb - 1) / wpb;
+  const int group = L / bpg;
+  const int chunk = (L - group * bpg) * wpb + wv;
+  if (chunk >= nchunk || group * 64 >= A.nray) return;
+  const int slot = group * 64 + lane;
+  const bool valid = slot < A.nray;
+  const int ray = valid ? A.order[slot] : 0;
+  const int npts = valid ? A.np[ray] : 0;
+  const int c0 = chunk * CPW;
+  const int uo = rec_u(A.nw);
+
+  float ep[CPW][NGT], tp[CPW][NGT], radv[CPW], comp[CPW], tau[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; c++) {
+#pragma unroll
+    for (int g = 0; g < NGT; g++) {
+      ep[c][g] = 0.f;
+      tp[c][g] = 1.f;
+    }
+    radv[c] = 0.f;
+    comp[c] = 0.f;
+    tau[c] = 1.f;
+  }
+  int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
+  const float* rp = A.rec + (size_t)ray * A.cap * A.R;
+
+  for (int ip = 0;; ip++) {
+    const bool act = ip < npts;
+    if (!__any(act)) break;
+    if (act) {
+      const float* pr = rp + (size_t)ip * A.R;
+      const float p = pr[REC_P], t = pr[REC_T], ds = pr[REC_DS], af = pr[REC_AF];
+      const float qh = pr[REC_QH2O];
+      float uu[NGT];
+#pragma unroll
+      for (int g = 0; g < NGT; g++) uu[g] = (g < A.ng) ? pr[uo + g] : 0.f;
+      int itp;
+      float dtp;
+      planck_index(t, itp, dtp);
+      const bool scpt = A.scattering > 0 && af > 0.f;
+      const int ai = scpt || (A.sca_n > 0 && af != 0.f) ? (int)pr[REC_AI] : 0;
+#pragma unroll
+      for (int c = 0; c < CPW; c++) {
+        const int id = c0 + c;
+        if (id < A.nd) {
+          const ChanConst& cc = A.cc[id];
+          float tg = 1.f, et = 0.f;
+#pragma unroll
+          for (int g = 0; g < NGT; g++) {
+            if (g < A.ng) {
+              float s, f;
+              ega_step(A.T, id * A.ng + g, p, t, uu[g], ep[c][g], tp[c][g], s, f);
+              et = et + s - et * s;
+              tg *= f;
+            }
+          }
+          const float uco2 = cc.co2_on ? pr[uo + cc.co2_ig] : 0.f;
+          const float uh2o = cc.h2o_on ? pr[uo + cc.h2o_ig] : 0.f;
+          const float kw = A.nw > 0 ? pr[REC_K + cc.window] : 0.f;
+          const float beta = continua(cc, p, t, ds, qh, uco2, uh2o, kw);
+          const float B = planck_at(A.planck + (size_t)id * JUR_NPLANCK, itp, dtp);
+          if (tg > 0.f) {
+            if (scpt) {
+              const float be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
+              const float J = A.J[(size_t)kj * A.nd + id];
+              const float lg = et < 0.5f ? -log1pf(-et) : -logf(tg);
+              const float bt = lg / ds + beta + af * be;
+              const float x = bt * ds;
+              const float eps = -expm1f(-x);
+              const float src = ((bt - af * bs) * B + af * bs * J) / bt;
+              kahan_add(radv[c], comp[c], tau[c] * eps * src);
+              tau[c] *= expf(-x);
+            } else {
+              float bx = beta;
+              if (A.sca_n > 0 && af != 0.f)
+                bx += af * (A.ext_beta_e ? A.beta_e[ai * A.nd + id] : A.beta_a[ai * A.nd + id]);
+              const float x = bx * ds;
+              const float eps = et - tg * expm1f(-x);
+              kahan_add(radv[c], comp[c], B * eps * tau[c]);
+              tau[c] *= tg * expf(-x);
+            }
+          }
+        }
+      }
+      if (scpt) kj++;
+    }
+  }
+  if (!valid) return;
+  const float ts = A.tsurf[ray];
+  int its = 0;
+  float dts = 0.f;
+  if (ts > 0.f) planck_index(ts, its, dts);
+#pragma unroll
+  for (int c = 0; c < CPW; c++) {
+    const int id = c0 + c;
+    if (id < A.nd) {
+      if (ts > 0.f)
+        kahan_add(radv[c], comp[c], planck_at(A.planck + (size_t)id * JUR_NPLANCK, its, dts) * tau[c]);
+      A.rad[(size_t)id * A.out_ld + A.out_off + ray] = radv[c];
+      A.tau[(size_t)id * A.out_ld + A.out_off + ray] = tau[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Filter-weighted Planck source-function table, FP64 (forwardmodel.c:
+// 756-786, planck :569-573 with gsl_expm1's series below ln 2).
+// ---------------------------------------------------------------------------
+__device__ inline double expm1_gsl(double x) {
+  if (fabs(x) < M_LN2) {
+    double i = 1.0, sum = x, term = x;
+    do {
+      i++;
+      term *= x / i;
+      sum += term;
+    } while (fabs(term) > fabs(sum) * 2.2204460492503131e-16);
+    return sum;
+  }
+  return exp(x) - 1;
+}
+
+__global__ void k_planck_table(const double* fnu, const double* fw, const int* foff, const int* fn,
+                               int nd, float* table) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nd * JUR_NPLANCK) return;
+  int id = idx / JUR_NPLANCK, it = idx % JUR_NPLANCK;
+  double temp = lin(0.0, 100.0, JUR_NPLANCK - 1.0, 400.0, (double)it);
+  const double* nu = fnu + foff[id];
+  const double* w = fw + foff[id];
+  double fsum = 0, acc = 0;
+  for (int i = 0; i < fn[id]; i++) {
+    fsum += w[i];
+    double v = nu[i];
+    acc += w[i] * (C1 * (v * v * v) / expm1_gsl(C2 * v / temp));
+  }
+  table[(size_t)id * JUR_NPLANCK + it] = (float)(acc / fsum);
+}
+
+__global__ void k_iota(int* ids, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ids[i] = i;
+}
+__global__ void k_zero_int(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <int NGT, int CPW>
+void launch_path_t(const PathArgs& A, int wpb, hipStream_t s) {
+  int nchunk = (A.nd + CPW - 1) / CPW;
+  int bpg = (nchunk + wpb - 1) / wpb;
+  int ngroups = (A.nray + 63) / 64;
+  long nb = (long)ngroups * bpg;
+  nb = (nb + 7) / 8 * 8;  // a multiple of the XCD count for the logical remap
+  hipLaunchKernelGGL((k_path<NGT, CPW>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+}
+
+template <int NGT>
+void launch_path_cpw(const PathArgs& A, int wpb, hipStream_t s) {
+  switch (A.cpw) {
+    case 1: launch_path_t<NGT, 1>(A, wpb, s); break;
+    case 2: launch_path_t<NGT, 2>(A, wpb, s); break;
+    case 8: launch_path_t<NGT, 8>(A, wpb, s); break;
+    default: launch_path_t<NGT, 4>(A, wpb, s); break;
+  }
+}
+
+}  // namespace
+
+void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
+  if (A.nray <= 0 || A.nd <= 0) return;
+  if (A.ng <= 2) launch_path_cpw<2>(A, wpb, s);
+  else if (A.ng <= 3) launch_path_cpw<3>(A, wpb, s);
+  else if (A.ng <= 5) launch_path_cpw<5>(A, wpb, s);
+  else if (A.ng <= 8) {
+    PathArgs B = A;
+    B.cpw = A.cpw > 2 ? 2 : A.cpw;
+    launch_path_cpw<8>(B, wpb, s);
+  } else {
+    PathArgs B = A;
+    B.cpw = 1;
+    launch_path_t<NGMAX, 1>(B, wpb, s);
+  }
+}
+
+void launch_planck_table(const double* fnu, const double* fw, const int* foff, const int* fn,
+                         const double*, int nd, float* table, hipStream_t s) {
+  int n = nd * JUR_NPLANCK;
+  hipLaunchKernelGGL(k_planck_table, dim3((n + 255) / 256), dim3(256), 0, s, fnu, fw, foff, fn, nd,
+                     table);
+}
+
+size_t sort_temp_bytes(int n) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const int*)nullptr, (int*)nullptr,
+                                               (const int*)nullptr, (int*)nullptr, n);
+  return bytes;
+}
+
+void launch_sort_rays(const int* np, int* keys_tmp, int* keys_out, int* ids_in, int* ids_out,
+                      void* temp, size_t temp_bytes, int n, hipStream_t s) {
+  (void)keys_tmp;
+  (void)hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, np, keys_out, ids_in, ids_out, n,
+                                               0, 32, s);
+}
+
+void launch_iota(int* ids, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, ids, n);
+}
+void launch_zero_int(int* p, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}

@@ -1,0 +1,201 @@
+// kernels_raytrace.hip -- refracted ray tracing of many lines of sight.
+//
+// One thread traces one ray in FP64 (reference src/lineofsight.c:5-190):
+// the march of jur_march.h, then the reference's post-processing: drop of a
+// vanishing last segment, tangent point (:411-452), recursive half-sum of
+// the step lengths (:177-179), and, with aerosol layers, the layer-edge point
+// insertion and chord-based segment lengths of add_aerosol_layers
+// (:193-372).  The march keeps only FP64 (z, lon, lat, ds) per point;
+// pressure, temperature, mixing ratios and extinction are re-interpolated at
+// each final point's altitude (the same function of z the reference
+// evaluates there), and stored as the FP32 point records of the
+// path-integration kernel together with the column densities
+// u = 10 q p / (k_B T) ds (:182-185, :330-332).
+#include "jur_march.h"
+
+namespace {
+
+struct BufSink {
+  double4* G;
+  __device__ void put(int i, const G4& g) { G[i] = make_double4(g.z, g.lon, g.lat, g.ds); }
+};
+
+__device__ inline G4 load_g(const double4* G, int i) {
+  double4 v = G[i];
+  return {v.x, v.y, v.z, v.w};
+}
+
+struct InsertSink {
+  double4* G2;
+  int n, cap, status;
+  __device__ void operator()(const G4& g) {
+    if (n >= cap) {
+      status |= JUR_ST_CAP;
+      return;
+    }
+    G2[n++] = make_double4(g.z, g.lon, g.lat, 0.0);
+  }
+};
+
+}  // namespace
+
+__global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.nray) return;
+  const int a = A.atm_idx ? A.atm_idx[r] : 0;
+  const DevAtm& M = A.atm;
+  double4* G = A.geo + (size_t)r * A.cap;
+  double tpz = A.vpz[r], tplon = A.vplon[r], tplat = A.vplat[r];
+
+  BufSink sink{G};
+  MarchOut mo = march_ray(M, a, A.obsz[r], A.obslon[r], A.obslat[r], A.vpz[r], A.vplon[r], A.vplat[r],
+                          A.rayds, A.raydz, A.refrac, A.cap, sink);
+  int np = mo.np, status = mo.status;
+  const int fatal = JUR_ST_CAP | JUR_ST_NLOS | JUR_ST_BELOW_SURFACE;
+  if (!(status & fatal) && np > 0) {
+    // Drop a vanishing last segment (lineofsight.c:170-171).
+    if (np >= 2 && G[np - 2].w < 1e-3 && np - 1 > 1) np--;
+    // Tangent point from the pre-trapezoid steps (lineofsight.c:411-452):
+    // first minimum of z; limb rays get a quadratic fit.
+    int im = 0;
+    double zm = G[0].x;
+    for (int i = 0; i < np; i++) {
+      double zi = G[i].x;
+      if (zi < zm) { zm = zi; im = i; }
+      if (isnan(zi)) { im = i; break; }
+    }
+    if (im <= 0 || im >= np - 1) {
+      G4 l = load_g(G, np - 1);
+      tpz = l.z; tplon = l.lon; tplat = l.lat;
+    } else {
+      G4 g0 = load_g(G, im - 1), g1 = load_g(G, im), g2 = load_g(G, im + 1);
+      double yy0 = g0.z, yy1 = g1.z, yy2 = g2.z;
+      double x1 = sqrt(g1.ds * g1.ds - (yy1 - yy0) * (yy1 - yy0));
+      double x2 = x1 + sqrt(g2.ds * g2.ds - (yy2 - yy1) * (yy2 - yy1));
+      double ca = 1 / (x1 - x2) * (-(yy0 - yy1) / x1 + (yy0 - yy2) / x2);
+      double cb = -(yy0 - yy1) / x1 - ca * x1;
+      double xt = -cb / (2 * ca);
+      tpz = ca * xt * xt + cb * xt + yy0;
+      double v0[3], v2[3], v[3], dummy;
+      geo2cart(g0.z, g0.lon, g0.lat, v0);
+      geo2cart(g2.z, g2.lon, g2.lat, v2);
+      for (int i = 0; i < 3; i++) v[i] = lin(0.0, v0[i], x2, v2[i], xt);
+      cart2geo(v, &dummy, &tplon, &tplat);
+    }
+    // Recursive half-sum of steps, as the reference (lineofsight.c:177-179):
+    // each ds is averaged with its already updated predecessor.
+    double dprev = 0.5 * G[0].w;
+    G[0].w = dprev;
+    for (int i = 1; i < np; i++) {
+      dprev = 0.5 * (dprev + G[i].w);
+      G[i].w = dprev;
+    }
+  }
+
+  // Aerosol-layer point insertion (lineofsight.c:193-372) into geo2.
+  const double4* F = G;
+  int nf = np;
+  float* rec = A.rec + (size_t)r * A.cap * A.R;
+  if (A.sca_n > 0 && np > 0 && !(status & fatal)) {
+    double4* G2 = A.geo2 + (size_t)r * A.cap;
+    const int jl = A.nalti;
+    const double altimax = jl > 0 ? A.altimax : NAN, altimin = jl > 0 ? A.altimin : NAN;
+    InsertSink ins{G2, 0, A.cap, 0};
+    G2[ins.n++] = G[0];
+    for (int ip = 1; ip < np && !ins.status; ip++) {
+      G4 g0 = load_g(G, ip - 1), g1 = load_g(G, ip);
+      edge_points(g0, g1, A.alti, jl, altimax, altimin, ins);
+      if (ins.status) break;
+      if (ins.n >= A.cap) { ins.status |= JUR_ST_CAP; break; }
+      if (ins.n > NLOS) { ins.status |= JUR_ST_NLOS; break; }
+      G2[ins.n++] = G[ip];
+    }
+    status |= ins.status;
+    const int n2 = ins.n;
+    if (!(status & fatal)) {
+      // Segment lengths from Cartesian half-chords (lineofsight.c:310-324).
+      double xa[3], xb[3], xc[3];
+      if (n2 >= 2) {
+        G4 g0 = load_g(G2, 0), g1 = load_g(G2, 1);
+        geo2cart(g0.z, g0.lon, g0.lat, xa);
+        geo2cart(g1.z, g1.lon, g1.lat, xb);
+        G2[0].w = 0.5 * dist3(xa, xb);
+        for (int ip = 1; ip < n2 - 1; ip++) {
+          G4 p0 = load_g(G2, ip - 1), p1 = load_g(G2, ip), p2 = load_g(G2, ip + 1);
+          geo2cart(p0.z, p0.lon, p0.lat, xa);
+          geo2cart(p1.z, p1.lon, p1.lat, xb);
+          geo2cart(p2.z, p2.lon, p2.lat, xc);
+          G2[ip].w = 0.5 * (dist3(xa, xb) + dist3(xb, xc));
+        }
+        G4 gl = load_g(G2, n2 - 1), gm = load_g(G2, n2 - 2);
+        geo2cart(gl.z, gl.lon, gl.lat, xa);
+        geo2cart(gm.z, gm.lon, gm.lat, xb);
+        G2[n2 - 1].w = 0.5 * dist3(xa, xb);
+      }
+      // Layer membership and transition factor (lineofsight.c:334-363).
+      for (int ip = 0; ip < n2; ip++) {
+        double zi = G2[ip].x;
+        int ai = -999;
+        double af = 0;
+        bool pre = (ip == 0) || ((G2[ip - 1].x < altimax || zi < altimax) &&
+                                 (G2[ip - 1].x > altimin || zi > altimin));
+        if (pre) {
+          for (int il = 0; il < A.nl; il++) {
+            double top = A.top[il], bot = A.bottom[il], tr = A.trans[il];
+            if (zi <= top && zi >= bot) { ai = il; af = 1.; }
+            if (tr > A.transs && zi <= top + tr && zi > top) { ai = il; af = (top + tr - zi) / tr; }
+            if (tr > A.transs && zi < bot && zi >= bot - tr) { ai = il; af = fabs(bot - tr - zi) / tr; }
+          }
+        }
+        rec[(size_t)ip * A.R + REC_AF] = (float)af;
+        rec[(size_t)ip * A.R + REC_AI] = (float)ai;
+      }
+      F = G2;
+      nf = n2;
+    }
+  } else {
+    for (int ip = 0; ip < np; ip++) {
+      rec[(size_t)ip * A.R + REC_AF] = 0.f;
+      rec[(size_t)ip * A.R + REC_AI] = -999.f;
+    }
+  }
+
+  // FP32 point records: atmosphere at each final point and column densities.
+  if (!(status & fatal)) {
+    const int uo = rec_u(A.nw);
+    for (int ip = 0; ip < nf; ip++) {
+      double4 g = F[ip];
+      int il = atm_locate(M, a, g.x);
+      AtmPoint ap = atm_pt(M, a, g.x, il);
+      float* pr = rec + (size_t)ip * A.R;
+      pr[REC_P] = (float)ap.p;
+      pr[REC_T] = (float)ap.t;
+      pr[REC_DS] = (float)g.w;
+      double qh = 0;
+      for (int ig = 0; ig < M.ng; ig++) {
+        double q = atm_q(M, a, ig, g.x, il);
+        if (ig == A.ig_h2o) qh = q;
+        pr[uo + ig] = (float)(10 * q * ap.p / (JUR_KBOLTZ * ap.t) * g.w);
+      }
+      pr[REC_QH2O] = (float)qh;
+      for (int iw = 0; iw < M.nw; iw++) pr[REC_K + iw] = (float)atm_k(M, a, iw, g.x, il);
+    }
+  } else {
+    nf = 0;
+  }
+
+  A.np_out[r] = nf;
+  A.tsurf_out[r] = (float)mo.tsurf;
+  if (A.tsurf64) A.tsurf64[r] = mo.tsurf;
+  A.tpz[r] = tpz;
+  A.tplon[r] = tplon;
+  A.tplat[r] = tplat;
+  if (A.geo_final) A.geo_final[r] = (F == G) ? 0 : 1;
+  if (status) atomicOr(A.status, status);
+}
+
+void launch_raytrace(const TraceArgs& A, hipStream_t s) {
+  if (A.nray <= 0) return;
+  int bs = 128;
+  hipLaunchKernelGGL(k_raytrace, dim3((A.nray + bs - 1) / bs), dim3(bs), 0, s, A);
+}

@@ -1,0 +1,79 @@
+"""Load libjurassic_amd.so (the GPU forward model) and declare its C ABI.
+
+Fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parents[1]
+LIBPATH = PKG / "libjurassic_amd.so"
+
+# Every symbol include/jurassic_amd.h declares, with (restype, argtypes).
+P = C.c_void_p
+D = C.POINTER(C.c_double)
+SIGNATURES = {
+    "formod": (None, [P, P, P, P]),
+    "formod_pencil": (None, [P, P, P, P, C.c_int, C.c_int]),
+    "raytrace": (None, [P, P, P, P, P, C.c_int]),
+    "get_opt_prop": (None, [P, P]),
+    "bhmie": (None, [C.c_double, C.c_double, C.c_double, D, D, D]),
+    "hydrostatic": (None, [P, P]),
+    "read_ctl": (None, [C.c_int, P, P]),
+    "scan_ctl": (C.c_double, [C.c_int, P, C.c_char_p, C.c_int, C.c_char_p, C.c_char_p]),
+    "read_atm": (None, [C.c_char_p, C.c_char_p, P, P]),
+    "write_atm": (None, [C.c_char_p, C.c_char_p, P, P]),
+    "read_obs": (None, [C.c_char_p, C.c_char_p, P, P]),
+    "write_obs": (None, [C.c_char_p, C.c_char_p, P, P]),
+    "read_aero": (None, [C.c_char_p, C.c_char_p, P, P]),
+    "write_aero": (None, [C.c_char_p, C.c_char_p, P]),
+    "brightness": (C.c_double, [C.c_double, C.c_double]),
+    "planck": (C.c_double, [C.c_double, C.c_double]),
+    "jur_formod_batch": (C.c_int, [P, P, P, C.c_int]),
+    "jur_formod_batch_device": (C.c_int, [P, P, P, C.c_int, P]),
+    "jur_sync": (C.c_int, []),
+    "jur_init": (C.c_int, [C.c_int]),
+    "jur_set_option": (C.c_int, [C.c_char_p, C.c_double]),
+    "jur_stat": (C.c_double, [C.c_char_p]),
+}
+
+
+class jur_batch_t(C.Structure):
+    _fields_ = [("nray", C.c_int), ("nlev", C.c_int), ("natm", C.c_int),
+                ("atm_of_ray", C.c_void_p),
+                ("z", C.c_void_p), ("p", C.c_void_p), ("t", C.c_void_p), ("q", C.c_void_p),
+                ("k", C.c_void_p),
+                ("time", C.c_void_p), ("obsz", C.c_void_p), ("obslon", C.c_void_p),
+                ("obslat", C.c_void_p), ("vpz", C.c_void_p), ("vplon", C.c_void_p),
+                ("vplat", C.c_void_p),
+                ("rad", C.c_void_p), ("tau", C.c_void_p),
+                ("tpz", C.c_void_p), ("tplon", C.c_void_p), ("tplat", C.c_void_p),
+                ("npts", C.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    """The GPU library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBPATH.exists():
+        raise RuntimeError("libjurassic_amd.so is not built: run __graft_entry__.build() "
+                           "(make -C jurassic-scatter_amd)")
+    lib = C.CDLL(str(LIBPATH))
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor (host or device)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data

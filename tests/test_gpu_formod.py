@@ -1,0 +1,87 @@
+"""GPU forward model against the oracle and the reference fixtures.
+
+Radiances are FP32 on the GPU (EGA, path integration with Kahan-compensated
+radiance) and FP64 in the oracle; the bar is 1e-4 relative radiance
+(<= 0.01 K brightness temperature) on every ray and channel, transmittance
+within 1e-5 absolute, tangent points within 1e-6 km (FP64 geometry).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ctypes
+from conftest import EXAMPLES
+from helpers import EXAMPLE_CASES, obs_rows, read_example
+from pyjurassic import jtypes as J
+from pyjurassic import lib as L
+
+pytestmark = pytest.mark.gpu
+
+RAD_RTOL = 1e-4
+TAU_ATOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def glib():
+    return L.load()
+
+
+@pytest.fixture(scope="module")
+def olib():
+    return oracle_ctypes.load()
+
+
+def arrays(ctl, obs):
+    nr, nd = obs.nr, ctl.nd
+    rad = np.array([[obs.rad[i][r] for r in range(nr)] for i in range(nd)])
+    tau = np.array([[obs.tau[i][r] for r in range(nr)] for i in range(nd)])
+    tp = np.array([[obs.tpz[r], obs.tplon[r], obs.tplat[r]] for r in range(nr)])
+    return rad, tau, tp
+
+
+@pytest.mark.parametrize("case", ["clear", "ext1"])
+def test_examples_vs_oracle(glib, olib, in_examples, tmp_path, case):
+    ctl, obs, atm, aero = read_example(olib, "oracle_", case)
+    olib.oracle_formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero))
+    ref = arrays(ctl, obs)
+    ctl2, obs2, atm2, aero2 = read_example(glib, "", case)
+    glib.formod(C.byref(ctl2), C.byref(atm2), C.byref(obs2), C.byref(aero2))
+    got = arrays(ctl2, obs2)
+    print(case, "max rel rad", np.max(np.abs(got[0] / ref[0] - 1)), "max abs tau",
+          np.max(np.abs(got[1] - ref[1])), flush=True)
+    np.testing.assert_allclose(got[0], ref[0], rtol=RAD_RTOL)
+    np.testing.assert_allclose(got[1], ref[1], atol=TAU_ATOL)
+    np.testing.assert_allclose(got[2], ref[2], atol=1e-6)
+    # and against the reference's own golden file (%g, 6 digits)
+    out = tmp_path / "rad.tab"
+    glib.write_obs(None, str(out).encode(), C.byref(ctl2), C.byref(obs2))
+    g = np.array(obs_rows(EXAMPLES / "org" / EXAMPLE_CASES[case][4]))
+    m = np.array(obs_rows(out))
+    np.testing.assert_allclose(m[:, 10], g[:, 10], rtol=RAD_RTOL)
+    np.testing.assert_allclose(m[:, 11], g[:, 11], atol=TAU_ATOL, rtol=1e-4)
+    np.testing.assert_allclose(m[:, 7:10], g[:, 7:10], rtol=1e-5, atol=1e-6)
+
+
+def test_raytrace_los_vs_oracle(glib, olib, in_examples):
+    ctl, obs, atm = J.ctl_t(), J.obs_t(), J.atm_t()
+    argc, argv = J.argv("clear-air.ctl")
+    olib.oracle_read_ctl(argc, argv, C.byref(ctl))
+    olib.oracle_read_obs(None, b"org/obs_raytrace.tab", C.byref(ctl), C.byref(obs))
+    olib.oracle_read_atm(None, b"atm_pwin_0_80.tab", C.byref(ctl), C.byref(atm))
+    aero = J.aero_t()
+    lo, lg = J.los_t(), J.los_t()
+    olib.oracle_raytrace(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero), C.byref(lo), 0)
+    tpo = (obs.tpz[0], obs.tplon[0], obs.tplat[0])
+    glib.raytrace(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero), C.byref(lg), 0)
+    assert lg.np == lo.np == 216
+    n = lo.np
+    for f in ["z", "lon", "lat", "p", "t", "ds"]:
+        a = np.frombuffer(getattr(lo, f), dtype=np.float64)[:n]
+        b = np.frombuffer(getattr(lg, f), dtype=np.float64)[:n]
+        np.testing.assert_allclose(b, a, rtol=1e-11, atol=1e-10, err_msg=f)
+    uo = np.array([[lo.u[i][g] for g in range(ctl.ng)] for i in range(n)])
+    ug = np.array([[lg.u[i][g] for g in range(ctl.ng)] for i in range(n)])
+    np.testing.assert_allclose(ug, uo, rtol=1e-11)
+    np.testing.assert_allclose((obs.tpz[0], obs.tplon[0], obs.tplat[0]), tpo, rtol=1e-11, atol=1e-12)
