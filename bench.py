@@ -29,6 +29,15 @@ sys.path.insert(0, str(ROOT / "jurassic-scatter_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the path kernel measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a plain copy
+# kernel of known size (bin/copy_probe); keyed by (workload, rays, variant,
+# waves per block).  The profile summary named beside each figure holds the
+# commands and the raw counter values.
+MEASURED_TRAFFIC = {
+    ("clear_batch_65536", 65536, 2, 4): (2.1814e11, "profiles/k_path_fast_clear_batch_65536.md"),
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -37,10 +46,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clear_batch_65536")
     ap.add_argument("--nray", type=int, default=None, help="truncate the workload's rays")
-    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpw", type=int, default=None)
-    ap.add_argument("--wpb", type=int, default=None)
+    ap.add_argument("--variant", type=int, default=None,
+                    help="path-kernel variant (default: autotuned on the first call)")
+    ap.add_argument("--wpb", type=int, default=None, help="waves per block (default: autotuned)")
     return ap.parse_args()
 
 
@@ -66,9 +76,9 @@ def main():
         lib.read_ctl(argc, argv, C.byref(ctl))
     finally:
         os.dup2(saved, 1)
-    if args.cpw:
-        lib.jur_set_option(b"cpw", args.cpw)
-    if args.wpb:
+    if args.variant is not None:
+        lib.jur_set_option(b"variant", args.variant)
+    if args.wpb is not None:
         lib.jur_set_option(b"wpb", args.wpb)
     nd, ng, nray = ctl.nd, ctl.ng, w.nray
     dev = torch.device("cuda:0")
@@ -117,6 +127,8 @@ def main():
     alg_bytes = pts * nd * (128 * ng + 8) + nray * nd * 8
     path_avg = float(np.mean(path_ms))
     achieved = alg_bytes / (path_avg * 1e-3) / 1e9
+    variant, wpb = lib.jur_stat(b"tuned_variant"), lib.jur_stat(b"tuned_wpb")
+    traffic, traffic_src = MEASURED_TRAFFIC.get((args.workload, nray, int(variant), int(wpb)), (None, None))
 
     line = {
         "metric": "ray x channel radiances/s (FP32 EGA + Kahan, FP64 geometry)",
@@ -134,9 +146,9 @@ def main():
         "config": {"workload": args.workload, "rays": nray, "channels": nd, "emitters": ng,
                    "levels": w.nlev, "los_points": pts, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_path", "kernel_ms": path_avg, "trace_ms": float(np.mean(trace_ms)),
-                     "alg_bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "k_path_fast", "kernel_ms": path_avg, "trace_ms": float(np.mean(trace_ms)),
+                     "alg_bytes_per_launch": alg_bytes, "variant": variant, "wpb": wpb},
     }
 
     if not args.no_cpu:

@@ -23,6 +23,8 @@ struct Engine {
   std::string tbl_key, chan_key;
   int ng = 0, nd = 0;
   DBuf t_np, t_p, t_nt, t_t, t_ncol, t_col, t_ue, planck, chan, nu_d;
+  DBuf g_np, g_p, g_nt, g_t, t_meta;
+  int shared = 0;
   // aerosol layers
   DBuf beta_e, beta_s, beta_a, alti;
   int nalti = 0;
@@ -181,6 +183,53 @@ void engine_prepare(const ctl_t* ctl) {
           }
         }
       }
+    // Shared-grid fast path: every channel of an emitter has the same
+    // increasing p and T grids, and all columns increase strictly in u and eps.
+    bool shared = true;
+    std::vector<int> gnp(ng, -1), gnt((size_t)ng * TBLNPMAX, 0);
+    std::vector<float> gp((size_t)ng * TBLNPMAX, 0.f), gt((size_t)ng * TBLNPMAX * TBLNTMAX, 0.f);
+    for (int ig = 0; ig < ng && shared; ig++) {
+      const int k0 = ig;  // channel 0
+      for (int id = 1; id < nd && shared; id++) {
+        const int k = id * ng + ig;
+        if (np[k] != np[k0]) shared = false;
+        for (int ip = 0; shared && ip < np[k0]; ip++) {
+          if (p[(size_t)k * TBLNPMAX + ip] != p[(size_t)k0 * TBLNPMAX + ip] ||
+              nt[(size_t)k * TBLNPMAX + ip] != nt[(size_t)k0 * TBLNPMAX + ip])
+            shared = false;
+          for (int it = 0; shared && it < nt[(size_t)k0 * TBLNPMAX + ip]; it++)
+            if (t[((size_t)k * TBLNPMAX + ip) * TBLNTMAX + it] != t[((size_t)k0 * TBLNPMAX + ip) * TBLNTMAX + it])
+              shared = false;
+        }
+      }
+      gnp[ig] = np[k0];
+      for (int ip = 0; ip < np[k0]; ip++) {
+        gp[(size_t)ig * TBLNPMAX + ip] = p[(size_t)k0 * TBLNPMAX + ip];
+        if (ip > 0 && !(gp[(size_t)ig * TBLNPMAX + ip] > gp[(size_t)ig * TBLNPMAX + ip - 1])) shared = false;
+        gnt[(size_t)ig * TBLNPMAX + ip] = nt[(size_t)k0 * TBLNPMAX + ip];
+        for (int it = 0; it < nt[(size_t)k0 * TBLNPMAX + ip]; it++) {
+          float v = t[((size_t)k0 * TBLNPMAX + ip) * TBLNTMAX + it];
+          gt[((size_t)ig * TBLNPMAX + ip) * TBLNTMAX + it] = v;
+          if (it > 0 && !(v > gt[((size_t)ig * TBLNPMAX + ip) * TBLNTMAX + it - 1])) shared = false;
+        }
+      }
+    }
+    for (size_t c = 0; c < col.size() && shared; c++)
+      for (int i = 1; i < ncol[c]; i++) {
+        const float2 a = ue[col[c] + i - 1], b = ue[col[c] + i];
+        if (!(b.x > a.x && b.y > a.y)) {
+          shared = false;
+          break;
+        }
+      }
+    std::vector<int2> meta(ncol.size());
+    for (size_t c = 0; c < ncol.size(); c++) meta[c] = make_int2(ncol[c], col[c]);
+    e.shared = shared ? 1 : 0;
+    upload(e.g_np, gnp, s);
+    upload(e.g_p, gp, s);
+    upload(e.g_nt, gnt, s);
+    upload(e.g_t, gt, s);
+    upload(e.t_meta, meta, s);
     upload(e.t_np, np, s);
     upload(e.t_p, p, s);
     upload(e.t_nt, nt, s);
@@ -214,6 +263,7 @@ void engine_prepare(const ctl_t* ctl) {
     dfoff.release();
     dfn.release();
     e.tbl_key = key;
+    e.stats["shared_grids"] = e.shared;
     e.ng = ng;
     e.nd = nd;
     printf("Initialize source function table...\n");
@@ -420,8 +470,8 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   P.np = npts;
   P.tsurf = tsurf;
   P.order = order;
-  P.T = {e.t_np.as<int>(), e.t_p.as<float>(), e.t_nt.as<int>(), e.t_t.as<float>(), e.t_ncol.as<int>(),
-         e.t_col.as<int>(), e.t_ue.as<float2>()};
+  P.T = engine_tables();
+  if (engine_option_int("fast", 1) == 0) P.T.shared = 0;
   P.cc = e.chan.as<ChanConst>();
   P.planck = e.planck.as<float>();
   P.beta_e = e.beta_e.as<float>();
@@ -436,8 +486,51 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   P.tau = tau;
   P.out_ld = out_ld;
   P.out_off = out_off;
-  P.cpw = engine_option_int("cpw", 4);
-  launch_path(P, engine_option_int("wpb", 4), s);
+  int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
+  if (variant < 0 || wpb < 0) {
+    // Autotuner: the first large batch of a (emitters, channels, size class)
+    // shape times each kernel variant (channels per wave x occupancy) and
+    // waves per block once, with HIP events on this stream, and keeps the
+    // fastest.  Small batches use the default.
+    char key[96];
+    int cls = nray >= (1 << 16) ? 3 : nray >= (1 << 14) ? 2 : nray >= 4096 ? 1 : 0;
+    snprintf(key, sizeof key, "tune:%d:%d:%d:%d", ctl->ng, ctl->nd, cls, P.T.shared);
+    auto it = e.opt.find(key);
+    int best = 2 * 8 + 4;  // variant 2, 4 waves per block
+    if (it != e.opt.end()) {
+      best = (int)it->second;
+    } else if (cls >= 2 && engine_option_int("autotune", 1)) {
+      hipEvent_t a, b;
+      HIP_OK(hipEventCreate(&a));
+      HIP_OK(hipEventCreate(&b));
+      float tbest = 1e30f;
+      const int wpbs[3] = {1, 2, 4};
+      for (int v = 0; v < path_variants(); v++)
+        for (int w : wpbs) {
+          P.cpw = v;
+          HIP_OK(hipEventRecord(a, s));
+          launch_path(P, w, s);
+          HIP_OK(hipEventRecord(b, s));
+          HIP_OK(hipEventSynchronize(b));
+          float ms = 0;
+          HIP_OK(hipEventElapsedTime(&ms, a, b));
+          if (ms < tbest) {
+            tbest = ms;
+            best = v * 8 + w;
+          }
+        }
+      HIP_OK(hipEventDestroy(a));
+      HIP_OK(hipEventDestroy(b));
+      e.opt[key] = best;
+      e.stats["tuned_variant"] = best / 8;
+      e.stats["tuned_wpb"] = best % 8;
+      e.stats["tuned_ms"] = tbest;
+    }
+    if (variant < 0) variant = best / 8;
+    if (wpb < 0) wpb = best % 8;
+  }
+  P.cpw = variant;
+  launch_path(P, wpb, s);
 }
 
 extern "C" int jur_set_option(const char* key, double value) {
@@ -472,8 +565,21 @@ extern "C" int jur_sync(void) {
 
 DevTables engine_tables() {
   Engine& e = E();
-  return {e.t_np.as<int>(), e.t_p.as<float>(), e.t_nt.as<int>(), e.t_t.as<float>(), e.t_ncol.as<int>(),
-          e.t_col.as<int>(), e.t_ue.as<float2>()};
+  DevTables T;
+  T.np = e.t_np.as<int>();
+  T.p = e.t_p.as<float>();
+  T.nt = e.t_nt.as<int>();
+  T.t = e.t_t.as<float>();
+  T.ncol = e.t_ncol.as<int>();
+  T.col = e.t_col.as<int>();
+  T.ue = e.t_ue.as<float2>();
+  T.shared = e.shared;
+  T.gnp = e.g_np.as<int>();
+  T.gp = e.g_p.as<float>();
+  T.gnt = e.g_nt.as<int>();
+  T.gt = e.g_t.as<float>();
+  T.meta = e.t_meta.as<int2>();
+  return T;
 }
 const double* engine_nu_dev() { return E().nu_d.as<double>(); }
 AeroGeom engine_aero_geom() {

@@ -27,6 +27,15 @@ struct DevTables {
   const int* ncol;     // [ntbl][TBLNPMAX][TBLNTMAX]
   const int* col;      // [ntbl][TBLNPMAX][TBLNTMAX]
   const float2* ue;    // (u, eps) pairs
+  // When every channel of an emitter has the same increasing (p, T) grids
+  // and every column is strictly increasing (shared != 0): the grids once
+  // per emitter, and (ncol, col) packed per column.
+  int shared;
+  const int* gnp;      // [ng]
+  const float* gp;     // [ng][TBLNPMAX]
+  const int* gnt;      // [ng][TBLNPMAX]
+  const float* gt;     // [ng][TBLNPMAX][TBLNTMAX]
+  const int2* meta;    // [ntbl][TBLNPMAX][TBLNTMAX] = (ncol, col)
 };
 
 // Per-channel constants of the continua and the channel's window (filled on

@@ -57,8 +57,9 @@ struct PathArgs {
   float* rad;              // [nd][out_ld], ray r at column out_off + r
   float* tau;
   int out_ld, out_off;
-  int cpw;                 // channels per wave
+  int cpw;                 // kernel variant (channels per wave x occupancy), see launch_path
 };
+int path_variants();
 // Launch the path integration with the given waves per block.
 void launch_path(const PathArgs& A, int waves_per_block, hipStream_t s);
 

@@ -162,21 +162,86 @@ __device__ inline float planck_at(const float* row, int it, float dt) {
   return y0 + (y1 - y0) / 0.25f * dt;
 }
 
+// Work of a wave: ray group (64 consecutive rays of the sorted list) and
+// channel chunk (CPW channels).  A block's waves take consecutive ray
+// groups of ONE channel chunk, so they read the same tables (shared L1);
+// logical blocks are chunk-major and the hardware's round-robin over the 8
+// XCDs is undone (L = xcd * per + slot), so each XCD works on a contiguous
+// range of channel chunks and its L2 holds only those channels' tables.
+__device__ inline bool map_work(const PathArgs& A, int cpw, int& chunk, int& group) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  const int per = gridDim.x >> 3;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int ngroups = (A.nray + 63) >> 6;
+  const int bpc = (ngroups + wpb - 1) / wpb;  // blocks per channel chunk
+  chunk = L / bpc;
+  group = (L - chunk * bpc) * wpb + wv;
+  const int nchunk = (A.nd + cpw - 1) / cpw;
+  return chunk < nchunk && group < ngroups;
+}
+
+// Radiance and transmittance update of one ray x channel at one LOS point
+// (forwardmodel.c:332-389) given the gas segment transmittance tg = tau_gas
+// and its complement et = 1 - tau_gas.
+__device__ inline void point_radiance(const PathArgs& A, const ChanConst& cc, int id, const float* pr, int uo,
+                                      float p, float t, float ds, float qh, float af, int ai, bool scpt, int kj,
+                                      int itp, float dtp, float tg, float et, float& radv, float& comp,
+                                      float& tau) {
+  const float uco2 = cc.co2_on ? pr[uo + cc.co2_ig] : 0.f;
+  const float uh2o = cc.h2o_on ? pr[uo + cc.h2o_ig] : 0.f;
+  const float kw = A.nw > 0 ? pr[REC_K + cc.window] : 0.f;
+  const float beta = continua(cc, p, t, ds, qh, uco2, uh2o, kw);
+  const float B = planck_at(A.planck + (size_t)id * JUR_NPLANCK, itp, dtp);
+  if (!(tg > 0.f)) return;
+  if (scpt) {
+    // scattering source (forwardmodel.c:332-358)
+    const float be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
+    const float J = A.J[(size_t)kj * A.nd + id];
+    const float lg = et < 0.5f ? -log1pf(-et) : -logf(tg);
+    const float bt = lg / ds + beta + af * be;
+    const float x = bt * ds;
+    const float eps = -expm1f(-x);
+    const float src = ((bt - af * bs) * B + af * bs * J) / bt;
+    kahan_add(radv, comp, tau * eps * src);
+    tau *= expf(-x);
+  } else {
+    // gas, continua and aerosol extinction (forwardmodel.c:365-388)
+    float bx = beta;
+    if (A.sca_n > 0 && af != 0.f)
+      bx += af * (A.ext_beta_e ? A.beta_e[ai * A.nd + id] : A.beta_a[ai * A.nd + id]);
+    const float x = bx * ds;
+    const float eps = et - tg * expm1f(-x);
+    kahan_add(radv, comp, B * eps * tau);
+    tau *= tg * expf(-x);
+  }
+}
+
+// Surface term and store (forwardmodel.c:393-399).
+template <int CPW>
+__device__ inline void finish_ray(const PathArgs& A, int ray, int c0, const float* radv, float* comp,
+                                  const float* tau) {
+  const float ts = A.tsurf[ray];
+  int its = 0;
+  float dts = 0.f;
+  if (ts > 0.f) planck_index(ts, its, dts);
+#pragma unroll
+  for (int c = 0; c < CPW; c++) {
+    const int id = c0 + c;
+    if (id < A.nd) {
+      float r = radv[c];
+      if (ts > 0.f) kahan_add(r, comp[c], planck_at(A.planck + (size_t)id * JUR_NPLANCK, its, dts) * tau[c]);
+      A.rad[(size_t)id * A.out_ld + A.out_off + ray] = r;
+      A.tau[(size_t)id * A.out_ld + A.out_off + ray] = tau[c];
+    }
+  }
+}
+
 template <int NGT, int CPW>
 __global__ void __launch_bounds__(256) k_path(PathArgs A) {
   const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wpb = blockDim.x >> 6;
-  // XCD-aware block order: dispatch round-robins blocks over the 8 XCDs, so
-  // logical block L = xcd * per + slot keeps consecutive logical blocks (the
-  // channel chunks of one ray group) on one XCD.
-  const int per = gridDim.x >> 3;
-  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const int nchunk = (A.nd + CPW - 1) / CPW;
-  const int bpg = (nchunk + wpb - 1) / wpb;
-  const int group = L / bpg;
-  const int chunk = (L - group * bpg) * wpb + wv;
-  if (chunk >= nchunk || group * 64 >= A.nray) return;
+  int chunk, group;
+  if (!map_work(A, CPW, chunk, group)) return;
   const int slot = group * 64 + lane;
   const bool valid = slot < A.nray;
   const int ray = valid ? A.order[slot] : 0;
@@ -229,52 +294,222 @@ __global__ void __launch_bounds__(256) k_path(PathArgs A) {
               tg *= f;
             }
           }
-          const float uco2 = cc.co2_on ? pr[uo + cc.co2_ig] : 0.f;
-          const float uh2o = cc.h2o_on ? pr[uo + cc.h2o_ig] : 0.f;
-          const float kw = A.nw > 0 ? pr[REC_K + cc.window] : 0.f;
-          const float beta = continua(cc, p, t, ds, qh, uco2, uh2o, kw);
-          const float B = planck_at(A.planck + (size_t)id * JUR_NPLANCK, itp, dtp);
-          if (tg > 0.f) {
-            if (scpt) {
-              const float be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
-              const float J = A.J[(size_t)kj * A.nd + id];
-              const float lg = et < 0.5f ? -log1pf(-et) : -logf(tg);
-              const float bt = lg / ds + beta + af * be;
-              const float x = bt * ds;
-              const float eps = -expm1f(-x);
-              const float src = ((bt - af * bs) * B + af * bs * J) / bt;
-              kahan_add(radv[c], comp[c], tau[c] * eps * src);
-              tau[c] *= expf(-x);
-            } else {
-              float bx = beta;
-              if (A.sca_n > 0 && af != 0.f)
-                bx += af * (A.ext_beta_e ? A.beta_e[ai * A.nd + id] : A.beta_a[ai * A.nd + id]);
-              const float x = bx * ds;
-              const float eps = et - tg * expm1f(-x);
-              kahan_add(radv[c], comp[c], B * eps * tau[c]);
-              tau[c] *= tg * expf(-x);
-            }
-          }
+          point_radiance(A, cc, id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg, et, radv[c], comp[c],
+                         tau[c]);
         }
       }
       if (scpt) kj++;
     }
   }
   if (!valid) return;
-  const float ts = A.tsurf[ray];
-  int its = 0;
-  float dts = 0.f;
-  if (ts > 0.f) planck_index(ts, its, dts);
+  finish_ray<CPW>(A, ray, c0, radv, comp, tau);
+}
+
+
+// ---------------------------------------------------------------------------
+// Fast path: shared (p, T) grids and strictly increasing columns.  Brackets
+// are searched from the previous point's bracket (the LOS state moves
+// slowly), which returns the same index as the reference's bisection
+// (misc.c:237-265, forwardmodel.c:546-565) for increasing data; a miss after
+// a few steps falls back to the bisection.
+// ---------------------------------------------------------------------------
+__device__ inline int near_grid(const float* xx, int n, float x, int h) {
+  if (!(x == x)) return n - 2;
+  h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
+  const float a = xx[h], b = xx[h + 1];
+  const bool okl = (h == 0) || (a <= x), okh = (h == n - 2) || (x < b);
+  if (okl && okh) return h;
+  if (okl) {
+    const int g = h + 1;
+    if (g == n - 2 || x < xx[g + 1]) return g;
+  } else {
+    const int g = h - 1;
+    if (g == 0 || xx[g] <= x) return g;
+  }
+  return locate_mono<float>(xx, n, x);
+}
+
+template <bool KEY_EPS>
+__device__ inline float keyof(float2 v) { return KEY_EPS ? v.y : v.x; }
+
+template <bool KEY_EPS>
+__device__ inline int near_col(const float2* c, int n, float x, int h, float2& lo, float2& hi) {
+  if (!(x == x)) {
+    lo = c[n - 2];
+    hi = c[n - 1];
+    return n - 2;
+  }
+  h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
+  lo = c[h];
+  hi = c[h + 1];
+  const bool okl = (h == 0) || (keyof<KEY_EPS>(lo) <= x);
+  const bool okh = (h == n - 2) || (x < keyof<KEY_EPS>(hi));
+  if (okl && okh) return h;
+  if (okl) {
+    for (int s = 0; s < 3; s++) {
+      h++;
+      lo = hi;
+      hi = c[h + 1];
+      if (h == n - 2 || x < keyof<KEY_EPS>(hi)) return h;
+    }
+  } else {
+    h--;
+    hi = lo;
+    lo = c[h];
+    if (h == 0 || keyof<KEY_EPS>(lo) <= x) return h;
+  }
+  h = locate_col<KEY_EPS>(c, n, x);
+  lo = c[h];
+  hi = c[h + 1];
+  return h;
+}
+
+// Corner lookup (inverse then forward) starting from bracket hint h; returns
+// the inverse bracket in h.
+__device__ inline EC corner_h(const float2* col, int n, float ep, float du, int& h) {
+  float2 a, b, c, d;
+  const int i = near_col<true>(col, n, ep, h, a, b);
+  h = i;
+  const float u = lin(a.y, a.x, b.y, b.x, ep);
+  const float ux = u + du;
+  near_col<false>(col, n, ux, i, c, d);
+  const float s = (d.y - c.y) / (d.x - c.x), dx = ux - c.x;
+  return clamp_ec(c.y + s * dx, (1.f - c.y) - s * dx);
+}
+
+__device__ inline EC mix_ec(EC y0, EC y1, float w) {
+  return clamp_ec(y0.e + (y1.e - y0.e) * w, y0.c + (y1.c - y0.c) * w);
+}
+
+template <int NGT, int CPW, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) k_path_fast(PathArgs A) {
+  const int lane = threadIdx.x & 63;
+  int chunk, group;
+  if (!map_work(A, CPW, chunk, group)) return;
+  const int slot = group * 64 + lane;
+  const bool valid = slot < A.nray;
+  const int ray = valid ? A.order[slot] : 0;
+  const int npts = valid ? A.np[ray] : 0;
+  const int c0 = chunk * CPW;
+  const int uo = rec_u(A.nw);
+  const DevTables& T = A.T;
+
+  float ep[CPW][NGT], tp[CPW][NGT], radv[CPW], comp[CPW], tau[CPW];
+  int hint[CPW][NGT];
+  int hpt[NGT];  // packed (ipr, it0, it1) brackets of the previous point
 #pragma unroll
   for (int c = 0; c < CPW; c++) {
-    const int id = c0 + c;
-    if (id < A.nd) {
-      if (ts > 0.f)
-        kahan_add(radv[c], comp[c], planck_at(A.planck + (size_t)id * JUR_NPLANCK, its, dts) * tau[c]);
-      A.rad[(size_t)id * A.out_ld + A.out_off + ray] = radv[c];
-      A.tau[(size_t)id * A.out_ld + A.out_off + ray] = tau[c];
+#pragma unroll
+    for (int g = 0; g < NGT; g++) {
+      ep[c][g] = 0.f;
+      tp[c][g] = 1.f;
+      hint[c][g] = 0;
+    }
+    radv[c] = 0.f;
+    comp[c] = 0.f;
+    tau[c] = 1.f;
+  }
+#pragma unroll
+  for (int g = 0; g < NGT; g++) hpt[g] = 0;
+  int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
+  const float* rp = A.rec + (size_t)ray * A.cap * A.R;
+
+  for (int ip = 0;; ip++) {
+    const bool act = ip < npts;
+    if (!__any(act)) break;
+    if (act) {
+      const float* pr = rp + (size_t)ip * A.R;
+      const float p = pr[REC_P], t = pr[REC_T], ds = pr[REC_DS], af = pr[REC_AF];
+      const float qh = pr[REC_QH2O];
+      int itp;
+      float dtp;
+      planck_index(t, itp, dtp);
+      const bool scpt = A.scattering > 0 && af > 0.f;
+      const int ai = scpt || (A.sca_n > 0 && af != 0.f) ? (int)pr[REC_AI] : 0;
+      // Emitter-outer loop: the (p, T) brackets and weights of an emitter are
+      // shared by all channels of the wave; only the per-channel segment
+      // transmittance products (tg, et) stay live across emitters.
+      float tg[CPW], et[CPW];
+#pragma unroll
+      for (int c = 0; c < CPW; c++) {
+        tg[c] = 1.f;
+        et[c] = 0.f;
+      }
+#pragma unroll
+      for (int g = 0; g < NGT; g++) {
+        if (g >= A.ng) continue;
+        const float du = pr[uo + g];
+        const int np_ = T.gnp[g];
+        if (np_ < 2) continue;
+        const float* pg = T.gp + g * TBLNPMAX;
+        const int ipr = near_grid(pg, np_, p, hpt[g] & 0xff);
+        const int nt0 = T.gnt[g * TBLNPMAX + ipr], nt1 = T.gnt[g * TBLNPMAX + ipr + 1];
+        if (nt0 < 2 || nt1 < 2) continue;
+        const float* t0g = T.gt + (size_t)(g * TBLNPMAX + ipr) * TBLNTMAX;
+        const float* t1g = t0g + TBLNTMAX;
+        const int it0 = near_grid(t0g, nt0, t, (hpt[g] >> 8) & 0xff);
+        const int it1 = near_grid(t1g, nt1, t, (hpt[g] >> 16) & 0xff);
+        hpt[g] = ipr | (it0 << 8) | (it1 << 16);
+        const float wt0 = (t - t0g[it0]) / (t0g[it0 + 1] - t0g[it0]);
+        const float wt1 = (t - t1g[it1]) / (t1g[it1 + 1] - t1g[it1]);
+        const float wp = (p - pg[ipr]) / (pg[ipr + 1] - pg[ipr]);
+        const int gb0 = ipr * TBLNTMAX + it0, gb1 = (ipr + 1) * TBLNTMAX + it1;
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+          const int id = c0 + c;
+          if (id >= A.nd) continue;
+          float s = 0.f, f = 1.f;
+          if (tp[c][g] < 1e-9f) {
+            s = 1.f;
+            f = 0.f;
+            ep[c][g] = 1.f;
+            tp[c][g] = 0.f;
+          } else {
+            const size_t tb = (size_t)(id * A.ng + g) * TBLNPMAX * TBLNTMAX;
+            const int2 m00 = T.meta[tb + gb0], m01 = T.meta[tb + gb0 + 1];
+            const int2 m10 = T.meta[tb + gb1], m11 = T.meta[tb + gb1 + 1];
+            if (m00.x >= 2 && m01.x >= 2 && m10.x >= 2 && m11.x >= 2) {
+              const float e = ep[c][g];
+              const int h0 = hint[c][g];
+              int h = h0;
+              EC e00 = corner_h(T.ue + m00.y, m00.x, e, du, h);
+              hint[c][g] = h;
+              h = h0;
+              EC e01 = corner_h(T.ue + m01.y, m01.x, e, du, h);
+              h = h0;
+              EC e10 = corner_h(T.ue + m10.y, m10.x, e, du, h);
+              h = h0;
+              EC e11 = corner_h(T.ue + m11.y, m11.x, e, du, h);
+              e00 = mix_ec(e00, e01, wt0);
+              e11 = mix_ec(e10, e11, wt1);
+              e00 = mix_ec(e00, e11, wp);
+              if (e00.e < 0.5f) {
+                s = (e00.e - e) / tp[c][g];
+                f = 1.f - s;
+              } else {
+                f = e00.c / tp[c][g];
+                s = 1.f - f;
+              }
+              ep[c][g] = e00.e;
+              tp[c][g] = e00.c;
+            }
+          }
+          et[c] = et[c] + s - et[c] * s;
+          tg[c] *= f;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CPW; c++) {
+        const int id = c0 + c;
+        if (id < A.nd)
+          point_radiance(A, A.cc[id], id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg[c], et[c], radv[c],
+                         comp[c], tau[c]);
+      }
+      if (scpt) kj++;
     }
   }
+  if (!valid) return;
+  finish_ray<CPW>(A, ray, c0, radv, comp, tau);
 }
 
 // ---------------------------------------------------------------------------
@@ -320,42 +555,42 @@ __global__ void k_zero_int(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-template <int NGT, int CPW>
-void launch_path_t(const PathArgs& A, int wpb, hipStream_t s) {
+template <int NGT, int CPW, int OCC>
+void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   int nchunk = (A.nd + CPW - 1) / CPW;
-  int bpg = (nchunk + wpb - 1) / wpb;
   int ngroups = (A.nray + 63) / 64;
-  long nb = (long)ngroups * bpg;
+  long nb = (long)nchunk * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;  // a multiple of the XCD count for the logical remap
-  hipLaunchKernelGGL((k_path<NGT, CPW>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  if (A.T.shared)
+    hipLaunchKernelGGL((k_path_fast<NGT, CPW, OCC>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  else
+    hipLaunchKernelGGL((k_path<NGT, CPW>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
 }
 
+// Variants (channels per wave x occupancy target) picked by the tuner.
 template <int NGT>
-void launch_path_cpw(const PathArgs& A, int wpb, hipStream_t s) {
-  switch (A.cpw) {
-    case 1: launch_path_t<NGT, 1>(A, wpb, s); break;
-    case 2: launch_path_t<NGT, 2>(A, wpb, s); break;
-    case 8: launch_path_t<NGT, 8>(A, wpb, s); break;
-    default: launch_path_t<NGT, 4>(A, wpb, s); break;
+void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
+  switch (variant) {
+    case 0: launch_t<NGT, 1, 4>(A, wpb, s); break;
+    case 1: launch_t<NGT, 1, 8>(A, wpb, s); break;
+    case 2: launch_t<NGT, 2, 4>(A, wpb, s); break;
+    case 3: launch_t<NGT, 4, 2>(A, wpb, s); break;
+    default: launch_t<NGT, 4, 4>(A, wpb, s); break;
   }
 }
 
 }  // namespace
 
+int path_variants() { return 5; }
+
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;
-  if (A.ng <= 2) launch_path_cpw<2>(A, wpb, s);
-  else if (A.ng <= 3) launch_path_cpw<3>(A, wpb, s);
-  else if (A.ng <= 5) launch_path_cpw<5>(A, wpb, s);
-  else if (A.ng <= 8) {
-    PathArgs B = A;
-    B.cpw = A.cpw > 2 ? 2 : A.cpw;
-    launch_path_cpw<8>(B, wpb, s);
-  } else {
-    PathArgs B = A;
-    B.cpw = 1;
-    launch_path_t<NGMAX, 1>(B, wpb, s);
-  }
+  const int v = A.cpw;  // variant index
+  if (A.ng <= 2) launch_ng<2>(A, wpb, v, s);
+  else if (A.ng <= 3) launch_ng<3>(A, wpb, v, s);
+  else if (A.ng <= 5) launch_ng<5>(A, wpb, v, s);
+  else if (A.ng <= 8) launch_ng<8>(A, wpb, v < 2 ? v : 0, s);
+  else launch_t<NGMAX, 1, 2>(A, wpb, s);
 }
 
 void launch_planck_table(const double* fnu, const double* fw, const int* foff, const int* fn,

@@ -85,3 +85,52 @@ def test_raytrace_los_vs_oracle(glib, olib, in_examples):
     ug = np.array([[lg.u[i][g] for g in range(ctl.ng)] for i in range(n)])
     np.testing.assert_allclose(ug, uo, rtol=1e-11)
     np.testing.assert_allclose((obs.tpz[0], obs.tplon[0], obs.tplat[0]), tpo, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["aero0", "aero1"])
+def test_scattering_examples_vs_oracle(glib, olib, in_examples, tmp_path, case):
+    """Single scattering (SCA_MULT=1): nadir ice cloud (aerosol0.sh) and limb
+    ice + H2SO4 layers (aerosol1.sh), against the oracle and the golden."""
+    ctl, obs, atm, aero = read_example(olib, "oracle_", case)
+    olib.oracle_formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero))
+    ref = arrays(ctl, obs)
+    ctl2, obs2, atm2, aero2 = read_example(glib, "", case)
+    glib.formod(C.byref(ctl2), C.byref(atm2), C.byref(obs2), C.byref(aero2))
+    got = arrays(ctl2, obs2)
+    print(case, "max rel rad", np.max(np.abs(got[0] / ref[0] - 1)), "max abs tau",
+          np.max(np.abs(got[1] - ref[1])), "pencils", glib.jur_stat(b"pencils"), flush=True)
+    np.testing.assert_allclose(got[0], ref[0], rtol=RAD_RTOL)
+    np.testing.assert_allclose(got[1], ref[1], atol=TAU_ATOL)
+    out = tmp_path / "rad.tab"
+    glib.write_obs(None, str(out).encode(), C.byref(ctl2), C.byref(obs2))
+    g = np.array(obs_rows(EXAMPLES / "org" / EXAMPLE_CASES[case][4]))
+    m = np.array(obs_rows(out))
+    np.testing.assert_allclose(m[:, 10], g[:, 10], rtol=RAD_RTOL)
+    np.testing.assert_allclose(m[:, 11], g[:, 11], atol=TAU_ATOL, rtol=1e-4)
+
+
+def test_mie_layer_optics_vs_oracle(glib, olib, in_examples):
+    """get_opt_prop on the device (FP64 Mie) against the oracle: the six modes
+    in three layers of aero1.tab."""
+    ctl, _, _, aero = read_example(olib, "oracle_", "aero1")
+    ctl2, _, _, aero2 = read_example(glib, "", "aero1")
+    assert aero2.nl == aero.nl == 3
+    for il in range(aero.nl):
+        for f in ["beta_e", "beta_s", "beta_a"]:
+            a = getattr(aero, f)[il][0]
+            b = getattr(aero2, f)[il][0]
+            assert b == pytest.approx(a, rel=1e-12), (f, il)
+        pa = np.array(aero.p[il][0])
+        pb = np.array(aero2.p[il][0])
+        np.testing.assert_allclose(pb, pa, rtol=1e-11)
+
+
+def test_bhmie_vs_oracle(glib, olib):
+    for x, nr, ni in [(0.05, 1.33, 0.0), (2.5, 1.5, 0.01), (40.0, 1.31, 0.3), (785.0, 1.45, 0.2)]:
+        pa, pb = (C.c_double * 181)(), (C.c_double * 181)()
+        qa, sa, qb, sb = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        olib.oracle_bhmie(x, nr, ni, pa, C.byref(qa), C.byref(sa))
+        glib.bhmie(x, nr, ni, pb, C.byref(qb), C.byref(sb))
+        assert qb.value == pytest.approx(qa.value, rel=1e-11)
+        assert sb.value == pytest.approx(sa.value, rel=1e-11)
+        np.testing.assert_allclose(np.array(pb), np.array(pa), rtol=1e-10)
