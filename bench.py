@@ -29,6 +29,9 @@ sys.path.insert(0, str(ROOT / "jurassic-scatter_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# Rays of the CPU oracle sample per workload (about 10-30 s on 16 cores).
+CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 4096, "aerosol_limb_8192": 192, "cirrus_4096": 2}
+
 # HBM bytes per launch of the path kernel measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a plain copy
 # kernel of known size (bin/copy_probe); keyed by (workload, rays, variant,
@@ -46,7 +49,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clear_batch_65536")
     ap.add_argument("--nray", type=int, default=None, help="truncate the workload's rays")
-    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="rays of the CPU oracle sample (default: per workload, ~10-30 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", type=int, default=None,
                     help="path-kernel variant (default: autotuned on the first call)")
@@ -76,6 +80,14 @@ def main():
         lib.read_ctl(argc, argv, C.byref(ctl))
     finally:
         os.dup2(saved, 1)
+    aero, aero_p, scattering = None, None, 0
+    if w.aerofile:
+        # layer optics by the library's own Mie (get_opt_prop) before timing
+        aero = J.aero_t()
+        lib.read_aero(None, w.aerofile.encode(), C.byref(ctl), C.byref(aero))
+        lib.get_opt_prop(C.byref(ctl), C.byref(aero))
+        aero_p = C.byref(aero)
+        scattering = ctl.sca_mult
     if args.variant is not None:
         lib.jur_set_option(b"variant", args.variant)
     if args.wpb is not None:
@@ -101,18 +113,19 @@ def main():
     torch.cuda.synchronize()
 
     def step():
-        lib.jur_formod_batch_device(C.byref(ctl), C.byref(b), None, 0, None)
+        lib.jur_formod_batch_device(C.byref(ctl), C.byref(b), aero_p, scattering, None)
         lib.jur_sync()
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    path_ms, trace_ms = [], []
+    path_ms, trace_ms, ipts = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
         path_ms.append(lib.jur_stat(b"path_ms"))
         trace_ms.append(lib.jur_stat(b"trace_ms"))
+        ipts.append(lib.jur_stat(b"integrated_points"))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = 1e3 * dt / args.steps
@@ -120,11 +133,15 @@ def main():
 
     npts_h = npts.cpu().numpy().astype(np.int64)
     pts = int(npts_h.sum())
-    # Algorithmic bytes of the path-integration kernel per launch: per
+    pencils = int(lib.jur_stat(b"pencils"))
+    # Algorithmic bytes of the path-integration kernel per call: per
     # (LOS point x channel x emitter) 128 B of table nodes (4 (p,T) corners x
     # 2 lookups x 2 bracketing nodes x (u, eps) x 4 B), per (point x channel)
-    # 8 B of Planck-table nodes, per (ray x channel) 8 B of rad + tau out.
-    alg_bytes = pts * nd * (128 * ng + 8) + nray * nd * 8
+    # 8 B of Planck-table nodes, per (pencil x channel) 8 B of rad + tau out;
+    # summed over every pencil integrated (primaries, secondary fans, solar
+    # pencils), with the LOS points the library counted.
+    ipts_call = int(ipts[-1])
+    alg_bytes = ipts_call * nd * (128 * ng + 8) + pencils * nd * 8
     path_avg = float(np.mean(path_ms))
     achieved = alg_bytes / (path_avg * 1e-3) / 1e9
     variant, wpb = lib.jur_stat(b"tuned_variant"), lib.jur_stat(b"tuned_wpb")
@@ -144,11 +161,14 @@ def main():
         "dtype": "f32 (EGA, path integration), f64 (ray geometry)",
         "data": "synthetic (seeded generator, reference file formats)",
         "config": {"workload": args.workload, "rays": nray, "channels": nd, "emitters": ng,
-                   "levels": w.nlev, "los_points": pts, "parallelism": "single GPU"},
+                   "levels": w.nlev, "los_points": pts, "scattering_depth": scattering,
+                   "pencils": pencils, "integrated_points": ipts_call, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_path_fast", "kernel_ms": path_avg, "trace_ms": float(np.mean(trace_ms)),
-                     "alg_bytes_per_launch": alg_bytes, "variant": variant, "wpb": wpb},
+                     "kernel": "k_path_fast", "kernel_ms": path_avg,
+                     "trace_ms": float(np.mean(trace_ms)) if not scattering else None,
+                     "launches_per_call": int(lib.jur_stat(b"path_launches")),
+                     "alg_bytes_per_call": alg_bytes, "variant": variant, "wpb": wpb},
     }
 
     if not args.no_cpu:
@@ -161,7 +181,12 @@ def main():
             olib.oracle_read_ctl(argc, argv, C.byref(octl))
         finally:
             os.dup2(saved, 1)
-        ns = min(args.cpu_sample, nray)
+        oaero = None
+        if w.aerofile:
+            oaero = J.aero_t()
+            olib.oracle_read_aero(None, w.aerofile.encode(), C.byref(octl), C.byref(oaero))
+            olib.oracle_get_opt_prop(C.byref(octl), C.byref(oaero))
+        ns = min(args.cpu_sample or CPU_SAMPLE[args.workload], nray)
         # every (nray/ns)-th ray: the sample spans limb and nadir geometries
         sel = np.linspace(0, nray - 1, ns).astype(np.int64)
         arr = lambda a: np.ascontiguousarray(a)
@@ -177,11 +202,13 @@ def main():
             pass
         olib.oracle_formod_batch(C.byref(octl), ns, w.nlev, dp(zs), dp(ps), dp(ts), dp(qs), None,
                                  dp(gs[0]), dp(gs[1]), dp(gs[2]), dp(gs[3]), dp(gs[4]), dp(gs[5]), dp(gs[6]),
-                                 0, min(ns, 16), ncores, dp(orad), dp(otau), None, None, None, None)
+                                 0, 1, ncores, dp(orad), dp(otau), None, None, None, None,
+                                 C.byref(oaero) if oaero else None, scattering)
         c0 = time.perf_counter()
         olib.oracle_formod_batch(C.byref(octl), ns, w.nlev, dp(zs), dp(ps), dp(ts), dp(qs), None,
                                  dp(gs[0]), dp(gs[1]), dp(gs[2]), dp(gs[3]), dp(gs[4]), dp(gs[5]), dp(gs[6]),
-                                 0, ns, ncores, dp(orad), dp(otau), None, None, None, None)
+                                 0, ns, ncores, dp(orad), dp(otau), None, None, None, None,
+                                 C.byref(oaero) if oaero else None, scattering)
         cdt = time.perf_counter() - c0
         line["cpu_baseline"] = {"value": ns * nd / cdt, "unit": "ray*channel/s", "cores": ncores,
                                 "kind": "port",

@@ -35,6 +35,11 @@ struct Engine {
   DBuf geo, geo2, rec, npts, tsurf, status, ids, order, keys, sorttmp;
   int cap = 512;
   std::map<std::string, double> opt, stats;
+  // per-call accounting of the path-integration launches: event pairs
+  // around each launch and the LOS points integrated (device counter)
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> path_ev;
+  size_t path_ev_used = 0;
+  DBuf pts_acc;
 };
 
 Engine& E() {
@@ -416,14 +421,41 @@ int engine_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, 
   }
 }
 
+namespace {
+void begin_call(hipStream_t s) {
+  Engine& e = E();
+  e.path_ev_used = 0;
+  launch_zero_u64((unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
+}
+// After the stream has reached the end of the call: path-kernel time and
+// the LOS points integrated, summed over the call's launches.
+void end_call(hipStream_t s) {
+  Engine& e = E();
+  double ms = 0;
+  for (size_t i = 0; i < e.path_ev_used; i++) {
+    float t = 0;
+    HIP_OK(hipEventElapsedTime(&t, e.path_ev[i].first, e.path_ev[i].second));
+    ms += t;
+  }
+  unsigned long long pts = 0;
+  HIP_OK(hipMemcpyAsync(&pts, e.pts_acc.p, sizeof pts, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  e.stats["path_ms"] = ms;
+  e.stats["path_launches"] = (double)e.path_ev_used;
+  e.stats["integrated_points"] = (double)pts;
+}
+}  // namespace
+
 void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int scattering,
                     const OutDev& out, hipStream_t s) {
   Engine& e = E();
   if (rays.nray <= 0) return;
   if (ctl->ip != 1) JUR_FAIL("Only IP=1 (vertical profile interpolation) is supported!");
   const int sca_on = ctl->sca_n > 0;
+  begin_call(s);
   if (sca_on && scattering > 0) {
     scatter_forward(ctl, atm, rays, scattering, out, s);
+    end_call(s);
     return;
   }
   hipEvent_t ev[3];
@@ -437,12 +469,11 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
   HIP_OK(hipEventRecord(ev[2], s));
   if (out.npts) HIP_OK(hipMemcpyAsync(out.npts, e.npts.p, rays.nray * sizeof(int), hipMemcpyDeviceToDevice, s));
   HIP_OK(hipEventSynchronize(ev[2]));
-  float t1 = 0, t2 = 0;
+  float t1 = 0;
   HIP_OK(hipEventElapsedTime(&t1, ev[0], ev[1]));
-  HIP_OK(hipEventElapsedTime(&t2, ev[1], ev[2]));
   e.stats["trace_ms"] = t1;
-  e.stats["path_ms"] = t2;
   e.stats["pencils"] = rays.nray;
+  end_call(s);
   for (auto& x : ev) HIP_OK(hipEventDestroy(x));
 }
 
@@ -530,7 +561,17 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     if (wpb < 0) wpb = best % 8;
   }
   P.cpw = variant;
+  if (e.path_ev.size() <= e.path_ev_used) {
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    e.path_ev.push_back({a, b});
+  }
+  auto& ev = e.path_ev[e.path_ev_used++];
+  launch_sum_points(npts, nray, (unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
+  HIP_OK(hipEventRecord(ev.first, s));
   launch_path(P, wpb, s);
+  HIP_OK(hipEventRecord(ev.second, s));
 }
 
 extern "C" int jur_set_option(const char* key, double value) {

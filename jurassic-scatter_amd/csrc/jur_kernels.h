@@ -75,3 +75,5 @@ void launch_sort_rays(const int* np, int* keys_tmp, int* keys_out, int* ids_in, 
                       void* temp, size_t temp_bytes, int n, hipStream_t s);
 void launch_iota(int* ids, int n, hipStream_t s);
 void launch_zero_int(int* p, int n, hipStream_t s);
+void launch_sum_points(const int* np, int n, unsigned long long* acc, hipStream_t s);
+void launch_zero_u64(unsigned long long* p, hipStream_t s);

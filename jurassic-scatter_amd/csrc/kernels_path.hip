@@ -121,12 +121,16 @@ __device__ inline void kahan_add(float& sum, float& comp, float term) {
 __device__ inline float continua(const ChanConst& c, float p, float t, float ds, float qh2o,
                                  float uco2, float uh2o, float kw) {
   float beta = kw;
-  if (c.co2_on) {
+  // A zero-length segment (an aerosol edge point inserted onto the
+  // observer; u = 0) adds no CO2/H2O continuum: the reference's u/ds is
+  // 0/0 there.
+  const bool seg = ds > 0.f;
+  if (c.co2_on && seg) {
     float d230 = t - 230.f, d260 = t - 260.f, d296 = t - 296.f;
     float ctw = d260 * d296 * c.co2_k230 - d230 * d296 * c.co2_k260 + d230 * d260 * c.co2_k296;
     beta += uco2 * (float)(1.0 / (JUR_AVOGADRO * 1000.0 * P0)) * p * ctw / ds;
   }
-  if (c.h2o_on) {
+  if (c.h2o_on && seg) {
     float ctwslf = c.h2o_s296 * expf(c.h2o_lnratio * ((296.f - t) / 36.f));
     float a1 = c.nu * uh2o * tanhf(c.h2o_tanhk / t);
     float a2 = 296.f / t;
@@ -195,6 +199,8 @@ __device__ inline void point_radiance(const PathArgs& A, const ChanConst& cc, in
   const float B = planck_at(A.planck + (size_t)id * JUR_NPLANCK, itp, dtp);
   if (!(tg > 0.f)) return;
   if (scpt) {
+    // a zero-length segment adds nothing (the reference forms -log(1)/0)
+    if (!(ds > 0.f)) return;
     // scattering source (forwardmodel.c:332-358)
     const float be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
     const float J = A.J[(size_t)kj * A.nd + id];
@@ -616,6 +622,24 @@ void launch_sort_rays(const int* np, int* keys_tmp, int* keys_out, int* ids_in, 
 
 void launch_iota(int* ids, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, ids, n);
+}
+// Sum of the LOS point counts of a ray set into *acc (one atomic per block).
+__global__ void k_sum_points(const int* np, int n, unsigned long long* acc) {
+  __shared__ unsigned long long part[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = i < n ? (unsigned long long)np[i] : 0ull;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
+}
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+void launch_sum_points(const int* np, int n, unsigned long long* acc, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_sum_points, dim3((n + 255) / 256), dim3(256), 0, s, np, n, acc);
+}
+void launch_zero_u64(unsigned long long* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, p);
 }
 void launch_zero_int(int* p, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n);

@@ -27,6 +27,11 @@ Workloads (names used by bench.py and the tests):
   1.6, number density U(0.01, 1) cm^-3, n = U(1.1, 1.5), k = U(0.01, 0.5)),
   4,096 limb and nadir rays, 32 channels, scattering depth 2.
 
+The scattering workloads observe at 2000-01-01 12:00 UTC (time 43200 s), so
+the solar source term is active; the layer optics use the reference's fixed
+parameters (170 Gauss-Hermite radius nodes, 181 phase angles, 28 secondary
+pencils per scattering point).
+
 Emissivity tables: per emitter x channel, 20 log-spaced pressures
 (0.01-1013 hPa) x 12 temperatures (180-320 K) x 60 log-spaced column
 densities (1e14-1e24 molecules/cm^2), eps = 1 - exp(-(k u)^a) with
@@ -140,6 +145,26 @@ class Workload:
 
     def write_ctl(self, path):
         Path(path).write_text("\n".join(self.ctl_lines()) + "\n")
+
+    def select(self, rays=None, nd=None, ng=None):
+        """A smaller case of the same workload: the rays at indices `rays`
+        (with their atmospheres), the first nd channels and the first ng
+        emitters (their tables are already written)."""
+        kw = dict(self.__dict__)
+        if rays is not None:
+            rays = np.asarray(rays)
+            kw["geom"] = np.ascontiguousarray(self.geom[:, rays])
+            kw["nray"] = len(rays)
+            if self.natm > 1:
+                for f in ("z", "p", "t", "q"):
+                    kw[f] = np.ascontiguousarray(getattr(self, f)[rays])
+                kw["natm"] = len(rays)
+        if nd is not None:
+            kw["nu"] = self.nu[:nd]
+        if ng is not None:
+            kw["emitters"] = self.emitters[:ng]
+            kw["q"] = np.ascontiguousarray(kw["q"][:, :ng])
+        return Workload(**kw)
 
 
 def _rays_limb(th, obsz=800.0):
@@ -256,6 +281,8 @@ def make(name, outdir, nray=None, seed=None):
             aerofile.write_text("%g %g %g MIE %s %g %g %g\n" % (top, bot, 0.1, refrac, nn, rr, ss))
             aero = str(aerofile)
             extra = {"SCA_N": "1", "SCA_MULT": "1" if cfg == 2 else "2"}
+            # local noon at longitude 0 on 2000-01-01: the solar term is on
+            geom[0] = 43200.0
     w = Workload(
         name=name, cfg=cfg, emitters=emitters, nu=nu, tblbase=tblbase, nray=total, nlev=nlev,
         natm=natm, z=np.repeat(z[None], natm, 0).copy(), p=np.repeat(p0[None], natm, 0).copy(),

@@ -705,10 +705,18 @@ double oracle_ctmo2(double nu, double p, double t) {
 void oracle_continua(ctl_t *ctl, double p, double t, const double *q, const double *u,
                      double ds, const double *k, double *beta) {
   for (int id = 0; id < ctl->nd; id++) beta[id] = k[ctl->window[id]];
+  /* DEVIATION: the reference divides the CO2 and H2O continua by ds also
+     for a zero-length segment (0/0 = NaN).  Such a segment (u = 0) occurs
+     when add_aerosol_layers inserts an edge point onto the observer of a
+     pencil that starts at an edge altitude (vertical secondary pencils at
+     depth >= 2); it contributes nothing here.  Every finite reference
+     result is unchanged. */
+  if (!(ds > 0)) goto rest;
   if (ctl->ctm_co2)
     for (int id = 0; id < ctl->nd; id++) beta[id] += o_ctmco2(ctl, ctl->nu[id], p, t, u) / ds;
   if (ctl->ctm_h2o)
     for (int id = 0; id < ctl->nd; id++) beta[id] += o_ctmh2o(ctl, ctl->nu[id], p, t, q, u) / ds;
+rest:
   if (ctl->ctm_n2)
     for (int id = 0; id < ctl->nd; id++) beta[id] += oracle_ctmn2(ctl->nu[id], p, t);
   if (ctl->ctm_o2)
@@ -1461,22 +1469,30 @@ static void o_alpha_grid(double *alpha) {
 static void o_pencil(ctl_t *ctl, const atm_t *atm, const aero_t *aero, oray_t *ray,
                      int scattering, double *rad, double *tau);
 
+/* Threads per primary ray for the secondary fans of its scattering points
+   (oracle_formod_batch sets it when there are fewer rays than threads). */
+static int o_inner = 1;
+
 /* srcfunc_sca_1d (scatter.c:666-807): thermal scattering source at x. */
 static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double time,
                              const double *x, const double *dx, int il, double *src_sca,
                              int scattering) {
   enum { NALPHA = 28 };
   double alpha[NALPHA], dnorth[3], ek[3], lx[3], ly[3], lz[3], sx[3], sy[3], sz[3],
-         theta[NTHETA], xv[3], wsum[NDMAX];
+         theta[NTHETA], wsum[NDMAX];
   double *rad2 = (double *)malloc(sizeof(double) * NALPHA * NDMAX);
-  double tau2[NDMAX];
+  double *tau2 = (double *)malloc(sizeof(double) * NALPHA * NDMAX);
   for (int it = 0; it < NTHETA; it++) theta[it] = (double)it * M_PI / 180.;
   dnorth[0] = -x[0];
   dnorth[1] = -x[1];
   dnorth[2] = 2 * RE - x[2];
   o_bascoord(x, dnorth, lx, ly, lz);
   o_alpha_grid(alpha);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(o_inner) schedule(dynamic) if (o_inner > 1 && omp_get_level() == 1)
+#endif
   for (int ir = 0; ir < NALPHA; ir++) {
+    double xv[3];
     for (int i = 0; i < 3; i++)
       xv[i] = x[i] + 10. * cos(alpha[ir]) * (-1) * lz[i] + 10. * sin(alpha[ir]) * ly[i];
     oray_t r2;
@@ -1487,7 +1503,7 @@ static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, d
     r2.time = time;
     o_cart2geo(x, &r2.obsz, &r2.obslon, &r2.obslat);
     o_cart2geo(xv, &r2.vpz, &r2.vplon, &r2.vplat);
-    o_pencil(ctl, atm, aero, &r2, scattering - 1, &rad2[ir * NDMAX], tau2);
+    o_pencil(ctl, atm, aero, &r2, scattering - 1, &rad2[ir * NDMAX], &tau2[ir * NDMAX]);
   }
   o_bascoord(dx, x, sx, sy, sz);
   for (int id = 0; id < ctl->nd; id++) src_sca[id] = wsum[id] = 0;
@@ -1515,6 +1531,7 @@ static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, d
   }
   for (int id = 0; id < ctl->nd; id++) src_sca[id] /= wsum[id];
   free(rad2);
+  free(tau2);
 }
 
 /* srcfunc_sca_sun (scatter.c:899-995): solar single scattering at x. */
@@ -1610,8 +1627,10 @@ static void o_pencil(ctl_t *ctl, const atm_t *atm, const aero_t *aero, oray_t *r
       for (int i = 0; i < 3; i++) dx[i] = x1[i] - x0[i];
       o_srcfunc_sca(ctl, atm, aero, ray->time, x, dx, il, src_sca, scattering);
       double af = los->aerofac[ip];
+      /* DEVIATION: a zero-length segment (see oracle_continua) adds
+         nothing; the reference forms -log(1)/0 = NaN there. */
       for (int id = 0; id < ctl->nd; id++)
-        if (tau_gas[id] > 0) {
+        if (tau_gas[id] > 0 && los->ds[ip] > 0) {
           double bt = (-1.) * log(tau_gas[id]) / los->ds[ip] + beta_ctm[id] + af * aero->beta_e[il][id];
           double eps = 1 - exp(-1 * bt * los->ds[ip]);
           double src_all = ((bt - af * aero->beta_s[il][id]) * src_planck[id] +
@@ -1735,13 +1754,21 @@ void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z, const 
                          const double *obsz, const double *obslon, const double *obslat,
                          const double *vpz, const double *vplon, const double *vplat, int ray0,
                          int ray1, int nthreads, double *rad, double *tau, double *tpz,
-                         double *tplon, double *tplat, int *npts) {
+                         double *tplon, double *tplat, int *npts, const aero_t *aero,
+                         int scattering) {
   (void)nray;
   if (nlev > NPMAX) OFAIL("Too many data points!");
   o_init_tables(ctl);
-  static aero_t aero0; /* no aerosol in the batched clear-air path */
+  static aero_t aero0; /* clear air: no aerosol layers */
+  const aero_t *ae = aero ? aero : &aero0;
+  if (!aero) scattering = 0;
 #ifdef _OPENMP
   int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+  /* fewer rays than threads: the spare threads work on the secondary fans */
+  int nouter = ray1 - ray0 < nth ? (ray1 - ray0 > 0 ? ray1 - ray0 : 1) : nth;
+  o_inner = nth / nouter;
+  if (o_inner > 1) omp_set_max_active_levels(2);
+  nth = nouter;
 #else
   int nth = 1;
   (void)nthreads;
@@ -1769,7 +1796,7 @@ void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z, const 
         for (int l = 0; l < nlev; l++)
           atm->k[iw][l] = k ? k[((size_t)ir * ctl->nw + iw) * nlev + l] : 0;
       oray_t ray = {time[ir], obsz[ir], obslon[ir], obslat[ir], vpz[ir], vplon[ir], vplat[ir], 0, 0, 0};
-      o_pencil(ctl, atm, &aero0, &ray, 0, r, tt);
+      o_pencil(ctl, atm, ae, &ray, scattering, r, tt);
       for (int id = 0; id < ctl->nd; id++) {
         rad[(size_t)ir * ctl->nd + id] = r[id];
         tau[(size_t)ir * ctl->nd + id] = tt[id];
@@ -1781,4 +1808,5 @@ void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z, const 
     }
     free(atm);
   }
+  o_inner = 1;
 }

@@ -83,7 +83,9 @@ void oracle_srcfunc_planck(ctl_t *ctl, double t, double *src);
  * (the shape of the batched GPU entry point).  Profiles are ray-major:
  * z,p,t [nray][nlev], q [nray][ng][nlev], k [nray][nw][nlev].  Outputs rad,
  * tau [nray][nd] and the tangent point [nray].  Rays ray0..ray1-1 only.
- * nthreads <= 0 uses the OpenMP default.
+ * nthreads <= 0 uses the OpenMP default.  aero (NULL: clear air) must hold
+ * the layer optics (oracle_get_opt_prop); scattering is the recursion depth
+ * of formod_pencil (0 without aero).
  */
 void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z,
                          const double *p, const double *t, const double *q,
@@ -91,7 +93,8 @@ void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z,
                          const double *obslon, const double *obslat, const double *vpz,
                          const double *vplon, const double *vplat, int ray0, int ray1,
                          int nthreads, double *rad, double *tau, double *tpz,
-                         double *tplon, double *tplat, int *npts);
+                         double *tplon, double *tplat, int *npts, const aero_t *aero,
+                         int scattering);
 
 /* Drop the cached tables, filter functions and Planck tables. */
 void oracle_reset(void);

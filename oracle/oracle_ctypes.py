@@ -42,7 +42,7 @@ def load():
         "oracle_ctmo2": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "oracle_srcfunc_planck": (None, [P, C.c_double, D]),
         "oracle_formod_batch": (None, [P, C.c_int, C.c_int] + [D] * 12 + [C.c_int] * 3 +
-                                [D] * 5 + [C.POINTER(C.c_int)]),
+                                [D] * 5 + [C.POINTER(C.c_int), P, C.c_int]),
         "oracle_reset": (None, []),
     }
     for name, (res, args) in sig.items():
