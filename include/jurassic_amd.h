@@ -63,6 +63,14 @@ void write_obs(const char *dirname, const char *filename, ctl_t *ctl, obs_t *obs
 void read_aero(const char *dirname, const char *filename, ctl_t *ctl, aero_t *aero);
 void write_aero(const char *dirname, const char *filename, aero_t *aero);
 
+/* Two-column shape file, e.g. filter or FOV function, at most NSHAPE points
+   (reference src/forwardmodel.h:73-76, forwardmodel.c:577-606). */
+void read_shape(const char *filename, double *x, double *y, int *n);
+
+/* Copy an aerosol model; init clears its mode/layer parameters and optics
+   (reference src/scatter.h:34-37, scatter.c:237-271). */
+void copy_aero(ctl_t *ctl, aero_t *aero_dest, aero_t *aero_src, int init);
+
 /* Point formulas (reference src/forwardmodel.c:7-11 and :569-573). */
 double brightness(double rad, double nu);
 double planck(double t, double nu);
@@ -91,9 +99,10 @@ typedef struct {
 int jur_formod_batch(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero, int scattering);
 
 /* All pointers of b are device pointers already resident in GPU memory;
-   stream is a hipStream_t (NULL: the library's stream).  Runs asynchronously
-   on that stream and returns 0 after enqueueing; errors found on the device
-   (LOS buffer growth aside) are reported by jur_sync(). */
+   stream is a hipStream_t (NULL: the library's stream).  The work runs on
+   that stream; the call returns once the outputs are complete (the LOS
+   buffers are sized from a device-side overflow flag, and the scattering
+   tree from device-side point counts, both read back during the call). */
 int jur_formod_batch_device(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero, int scattering,
                             void *stream);
 
@@ -103,12 +112,21 @@ int jur_sync(void);
 /* Select the GPU (call before anything else; default device 0). */
 int jur_init(int device);
 
-/* Tuning and instrumentation.  Options: "cpw" channels per wave (1,2,4,8),
-   "wpb" waves per block, "cap" initial LOS points per ray, "graph" replay the
-   batched device call from a captured hipGraph (0/1). */
+/* Tuning.  Options: "variant" path-kernel variant (channels per wave x
+   occupancy target, 0-4; default: autotuned), "wpb" waves per block (1, 2,
+   4; default: autotuned), "autotune" (0/1, default 1), "fast" (0: always use
+   the generic table search), "cap" initial LOS points per ray, "chunk_mb"
+   trace-buffer budget of the scattering levels in MiB (default 6144).
+   Returns 0. */
 int jur_set_option(const char *key, double value);
-/* Statistics of the last call: "points" (LOS points of primaries),
-   "pencils" (all pencils), "path_ms" (path kernel time), "trace_ms". */
+/* Statistics of the last call: "path_ms" (path-integration kernel time,
+   summed over its launches), "path_launches", "integrated_points" (LOS
+   points integrated, all pencils), "trace_ms" (clear-air ray tracing),
+   "pencils" (primaries + secondary + solar pencils), "leaf_pencils",
+   "scattering_points", "scatter_ms" (whole scattering call),
+   "tuned_variant", "tuned_wpb", "tuned_ms" (autotuner choice),
+   "shared_grids" (1: the fast table layout is in use), "mie_spheres"
+   (spheres of the last get_opt_prop).  Unknown keys give NaN. */
 double jur_stat(const char *key);
 
 #ifdef __cplusplus

@@ -321,6 +321,37 @@ void read_shape_file(const char* filename, std::vector<double>& x, std::vector<d
   if (x.empty()) JUR_FAIL("Could not read any data!");
 }
 
+// read_shape (forwardmodel.c:577-606): two columns into x, y (NSHAPE max).
+extern "C" void read_shape(const char* filename, double* x, double* y, int* n) {
+  std::vector<double> a, b;
+  read_shape_file(filename, a, b);
+  for (size_t i = 0; i < a.size(); i++) {
+    x[i] = a[i];
+    y[i] = b[i];
+  }
+  *n = (int)a.size();
+}
+
+// copy_aero (scatter.c:237-271): copy; with init, clear the mode and layer
+// parameters and the optics of the ctl->nd channels (ss = 1).
+extern "C" void copy_aero(ctl_t* ctl, aero_t* dst, aero_t* src, int init) {
+  memmove(dst, src, sizeof(aero_t));
+  if (!init) return;
+  for (int im = 0; im < dst->nm; im++) {
+    dst->top_mod[im] = dst->bottom_mod[im] = dst->trans_mod[im] = 0;
+    dst->nn[im] = dst->rr[im] = 0;
+    dst->ss[im] = 1;
+  }
+  for (int il = 0; il < dst->nl; il++) {
+    dst->nmod[il] = 0;
+    dst->top[il] = dst->bottom[il] = dst->trans[il] = 0;
+    for (int id = 0; id < ctl->nd; id++) {
+      dst->beta_e[il][id] = dst->beta_s[il][id] = dst->beta_a[il][id] = 0;
+      for (int ia = 0; ia < NTHETA; ia++) dst->p[il][id][ia] = 0;
+    }
+  }
+}
+
 // Binary table: int np; double p[np]; int nt[np]; per p: double t[nt];
 // int nu[nt]; per T: float u[nu]; float eps[nu] (forwardmodel.c:636-651).
 bool read_table_bin(const char* filename, HostTable& tb) {

@@ -473,6 +473,10 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
   HIP_OK(hipEventElapsedTime(&t1, ev[0], ev[1]));
   e.stats["trace_ms"] = t1;
   e.stats["pencils"] = rays.nray;
+  // pencils the reference's queue would hold: the primaries are leaves only
+  // at depth 0 (forwardmodel.c:245-262)
+  e.stats["leaf_pencils"] = scattering == 0 ? rays.nray : 0;
+  e.stats["scattering_points"] = 0;
   end_call(s);
   for (auto& x : ev) HIP_OK(hipEventDestroy(x));
 }
@@ -575,7 +579,9 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
 }
 
 extern "C" int jur_set_option(const char* key, double value) {
-  E().opt[key] = value;
+  Engine& e = E();
+  e.opt[key] = value;
+  if (strcmp(key, "cap") == 0) e.cap = value < 16 ? 16 : (value > NLOS + 2 ? NLOS + 2 : (int)value);
   return 0;
 }
 extern "C" double jur_stat(const char* key) { return engine_stat(key); }

@@ -26,6 +26,8 @@ SIGNATURES = {
     "write_obs": (None, [C.c_char_p, C.c_char_p, P, P]),
     "read_aero": (None, [C.c_char_p, C.c_char_p, P, P]),
     "write_aero": (None, [C.c_char_p, C.c_char_p, P]),
+    "read_shape": (None, [C.c_char_p, D, D, C.POINTER(C.c_int)]),
+    "copy_aero": (None, [P, P, P, C.c_int]),
     "brightness": (C.c_double, [C.c_double, C.c_double]),
     "planck": (C.c_double, [C.c_double, C.c_double]),
     "jur_formod_batch": (C.c_int, [P, P, P, C.c_int]),
