@@ -129,6 +129,45 @@ int jur_set_option(const char *key, double value);
    (spheres of the last get_opt_prop).  Unknown keys give NaN. */
 double jur_stat(const char *key);
 
+/* ---- Scattering-source surrogate (no reference counterpart) ---------- */
+
+/* A perceptron 24 -> 4 x 256 (SiLU) -> 1 predicting the normalised
+   log of the thermal scattering source J at a scattering point from 24
+   local features (listed in the library's jur_surrogate.h); BF16 weights and
+   activations, FP32 accumulation, trained with Adam on the GPU.  With
+   jur_set_option("surrogate", 1) and a trained or loaded network, the
+   scattering forward model takes J at the primaries' scattering points
+   from the network; points whose features leave the training hull
+   (option "hull_margin", normalised units, default 0.25) fall back to the
+   plain secondary-pencil method, which also keeps the solar term.  The
+   parameter layout is jur_mlp_nparam() floats (W1 [32][256], b1, W2..W4
+   [256][256], b2..b4, W5 [256], b5). */
+int jur_mlp_init(unsigned seed);
+/* Training pairs: x [n][24] raw features, y [n] targets (log J).  normalize:
+   1 sets the feature / target normalisation and training hull from these
+   data and shuffles the rows (seed); 0 keeps the current normalisation. */
+int jur_mlp_set_data(const float *x, const float *y, long n, int normalize, unsigned seed);
+/* steps Adam steps (beta 0.9 / 0.999, eps 1e-8) on batches of `batch` rows
+   (multiple of 32), replaying one captured hipGraph per step; loss_hist
+   [steps] (optional) receives the batch mean squared error (normalised
+   units). */
+int jur_mlp_train(int steps, int batch, float lr, float *loss_hist);
+/* Gradients of the mean squared error on one batch (x raw features, y
+   targets) without updating: grads [jur_mlp_nparam()], loss (optional). */
+int jur_mlp_grads(const float *x, const float *y, int batch, float *grads, float *loss);
+/* Network output (log J) for n feature rows, fused inference kernel. */
+int jur_mlp_forward(const float *x, long n, float *y);
+int jur_mlp_get_params(float *p);
+int jur_mlp_set_params(const float *p);
+int jur_mlp_nparam(void);
+int jur_mlp_save(const char *path);
+int jur_mlp_load(const char *path);
+/* Training pairs of the last scattering call made with
+   jur_set_option("collect", 1): raw features feat [n][24] and the plain
+   method's thermal source jth [n] (no solar term) for every (primary
+   scattering point, channel); copies at most cap rows, returns n. */
+long jur_sca_collect(float *feat, float *jth, long cap);
+
 #ifdef __cplusplus
 }
 #endif

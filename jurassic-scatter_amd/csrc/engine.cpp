@@ -629,6 +629,21 @@ DevTables engine_tables() {
   return T;
 }
 const double* engine_nu_dev() { return E().nu_d.as<double>(); }
+int engine_find_emitter(const ctl_t* ctl, const char* name) { return find_emitter(ctl, name); }
+const float* engine_beta_e() { return E().beta_e.as<float>(); }
+const float* engine_beta_s() { return E().beta_s.as<float>(); }
+void engine_layer_edges(double* top, double* bottom) {
+  Engine& e = E();
+  for (int il = 0; il < NLMAX; il++) {
+    top[il] = e.top[il];
+    bottom[il] = e.bottom[il];
+  }
+}
+double engine_option_double(const char* key, double dflt) {
+  auto& m = E().opt;
+  auto it = m.find(key);
+  return it == m.end() ? dflt : it->second;
+}
 AeroGeom engine_aero_geom() {
   Engine& e = E();
   return {e.alti.as<double>(), e.nalti, e.altimax, e.altimin};

@@ -121,6 +121,11 @@ AeroGeom engine_aero_geom();
 int engine_cap();
 DevTables engine_tables();
 const double* engine_nu_dev();
+int engine_find_emitter(const ctl_t* ctl, const char* name);
+const float* engine_beta_e();  // [nl][nd] layer extinction (FP32)
+const float* engine_beta_s();
+void engine_layer_edges(double* top, double* bottom);  // [NLMAX]
+double engine_option_double(const char* key, double dflt);
 
 // Mie optics on the device (scatter.c:330-508).
 void engine_get_opt_prop(ctl_t* ctl, aero_t* aero);

@@ -15,6 +15,8 @@ struct ScaPt {
   double time;
   int il;
   int atm;
+  float af;  // aerosol factor of the point
+  int pad;
 };
 
 // Ray geometry arrays of a ray batch (device).
@@ -47,6 +49,13 @@ struct AngArgs {
   const double* theta_grid;  // [NTHETA] pi i / 180
   const double* bsun;        // [nd] Planck radiance of the Sun (TSUN)
   float* J;                  // [npt][nd]
+  // surrogate: sub_idx[k] >= 0 is the point's fan block in rad (plain
+  // angular sum); -1 takes the thermal source from the network output
+  // ymlp[k][nd] (normalised log J) instead.  Null: every point is plain.
+  const int* sub_idx;
+  const float* ymlp;
+  float ymu, ysd;
+  float* jth;                // [npt][nd] thermal part (no sun), or null
 };
 
 void launch_count_points(int nray, const int* np, const float* rec, int R, int cap, int* count, hipStream_t s);

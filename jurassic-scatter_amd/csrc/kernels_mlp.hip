@@ -1,0 +1,353 @@
+// kernels_mlp.hip -- matrix-core kernels of the scattering-source surrogate
+// (a 24 -> 4 x 256 SiLU -> 1 perceptron): an LDS-tiled BF16 GEMM on
+// v_mfma_f32_32x32x16_bf16 with split-K and a fused bias + SiLU epilogue,
+// the element-wise and reduction pieces of the backward pass, Adam, and a
+// fused inference kernel that keeps a 64-sample block's activations in LDS
+// through all five layers.
+//
+// Operand layout of v_mfma_f32_32x32x16_bf16 (one wave, C[32][32] +=
+// A[32][16] B[16][32]): lane l supplies A[l % 32][8 (l / 32) + 0..7] and
+// B[8 (l / 32) + 0..7][l % 32]; accumulator r of lane l holds
+// C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+#include "jur_mlp.h"
+
+namespace {
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline uint16_t f2bf(float f) { return mlp_f2bf(f); }
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ inline float silu(float z) { return z / (1.f + expf(-z)); }
+__device__ inline float dsilu(float z) {
+  const float s = 1.f / (1.f + expf(-z));
+  return s * (1.f + z * (1.f - s));
+}
+
+constexpr int TS = 64;        // block tile (M and N)
+constexpr int TK = 32;        // K per LDS stage
+constexpr int LDK = TK + 8;   // padded LDS row, BF16 elements (80 B)
+constexpr int LDH = MLP_H + 8;
+
+__device__ inline f32x16 zero16() {
+  f32x16 z;
+#pragma unroll
+  for (int i = 0; i < 16; i++) z[i] = 0.f;
+  return z;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[TS][LDK];
+  __shared__ __align__(16) uint16_t Bs[TS][LDK];  // [n][k]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m0 = blockIdx.y * TS, n0 = blockIdx.x * TS;
+  const int kper = g.K / g.ksplit;
+  const int kbeg = blockIdx.z * kper, kend = kbeg + kper;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+  const bool active = (m0 + wm < g.M) && (n0 + wn < g.N);
+  f32x16 acc = zero16();
+  for (int k0 = kbeg; k0 < kend; k0 += TK) {
+    // stage A[m0..+64][k0..+32] and B[k0..+32][n0..+64], walking the
+    // contiguous dimension with consecutive threads
+    if (g.sak == 1) {
+      for (int i = tid; i < TS * TK; i += 256) {
+        const int r = i >> 5, c = i & 31, m = m0 + r;
+        As[r][c] = m < g.M ? g.A[(long)m * g.sam + (k0 + c)] : (uint16_t)0;
+      }
+    } else {
+      for (int i = tid; i < TS * TK; i += 256) {
+        const int r = i & 63, c = i >> 6, m = m0 + r;
+        As[r][c] = m < g.M ? g.A[(long)m * g.sam + (long)(k0 + c) * g.sak] : (uint16_t)0;
+      }
+    }
+    if (g.sbn == 1) {
+      for (int i = tid; i < TS * TK; i += 256) {
+        const int c = i & 63, r = i >> 6, n = n0 + c;
+        Bs[c][r] = n < g.N ? g.B[(long)(k0 + r) * g.sbk + n] : (uint16_t)0;
+      }
+    } else {
+      for (int i = tid; i < TS * TK; i += 256) {
+        const int r = i & 31, c = i >> 5, n = n0 + c;
+        Bs[c][r] = n < g.N ? g.B[(long)(k0 + r) * g.sbk + (long)n * g.sbn] : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const int h = lane >> 5, r = lane & 31;
+#pragma unroll
+      for (int kk = 0; kk < TK; kk += 16) {
+        const bf16x8 a = *(const bf16x8*)&As[wm + r][kk + 8 * h];
+        const bf16x8 b = *(const bf16x8*)&Bs[wn + r][kk + 8 * h];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+  const int j = n0 + wn + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (g.epi == 0) {
+      g.C[(long)blockIdx.z * g.cslab + (long)i * g.ldc + j] = acc[r];
+    } else {
+      const float z = acc[r] + g.bias[j];
+      g.C[(long)i * g.ldc + j] = z;
+      g.act[(long)i * g.ldact + j] = f2bf(silu(z));
+    }
+  }
+}
+
+__global__ void k_splitk_sum(const float* part, int nslab, long slab, long n, float* out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < nslab; z++) s += part[(long)z * slab + i];
+  out[i] = s;
+}
+
+// Batch b of training step t takes row (t B + b) mod ntot (the rows are
+// shuffled once on upload).
+__global__ void k_gather_batch(const uint16_t* X, const float* Y, long ntot, int B, const int* step,
+                               uint16_t* xb, float* yb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const long row = ((long)(*step) * B + b) % ntot;
+  const uint4* src = (const uint4*)(X + row * MLP_INP);
+  uint4* dst = (uint4*)(xb + (long)b * MLP_INP);
+#pragma unroll
+  for (int i = 0; i < MLP_INP * 2 / 16; i++) dst[i] = src[i];
+  yb[b] = Y[row];
+}
+
+// y[b] = act[b][:] . W5 + b5, one wave per sample.
+__global__ void k_out_fwd(const uint16_t* act, const uint16_t* w5, const float* b5, int B, float* y) {
+  const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= B) return;
+  float s = 0.f;
+#pragma unroll
+  for (int q = 0; q < MLP_H / 64; q++) {
+    const int j = lane + 64 * q;
+    s += bf2f(act[(long)b * MLP_H + j]) * bf2f(w5[j]);
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) y[b] = s + b5[0];
+}
+
+// Mean squared error: dy = 2 (y - t) / B, loss_hist[step & mask] = mean (y - t)^2.
+__global__ void __launch_bounds__(1024) k_loss_grad(const float* y, const float* t, int B, float* dy,
+                                                    float* loss_hist, const int* step, int mask) {
+  __shared__ float red[16];
+  float s = 0.f;
+  for (int b = threadIdx.x; b < B; b += blockDim.x) {
+    const float d = y[b] - t[b];
+    dy[b] = 2.f * d / (float)B;
+    s += d * d;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float tot = 0.f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) tot += red[i];
+    if (loss_hist) loss_hist[(*step) & mask] = tot / (float)B;
+  }
+}
+
+// dz4 = dy W5 silu'(z4) (BF16).
+__global__ void k_out_dz(const float* z4, const float* dy, const uint16_t* w5, int B, uint16_t* dz4) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)B * MLP_H) return;
+  const int b = (int)(i / MLP_H), j = (int)(i % MLP_H);
+  dz4[i] = f2bf(dy[b] * bf2f(w5[j]) * dsilu(z4[i]));
+}
+
+// gW5[j] = sum_b act4[b][j] dy[b] (block j < 256), gb5 = sum_b dy[b] (block 256).
+__global__ void __launch_bounds__(256) k_out_wgrad(const uint16_t* act4, const float* dy, int B, float* gw5,
+                                                   float* gb5) {
+  __shared__ float red[4];
+  const int j = blockIdx.x;
+  float s = 0.f;
+  for (int b = threadIdx.x; b < B; b += blockDim.x)
+    s += (j < MLP_H ? bf2f(act4[(long)b * MLP_H + j]) : 1.f) * dy[b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float tot = red[0] + red[1] + red[2] + red[3];
+    if (j < MLP_H) gw5[j] = tot;
+    else gb5[0] = tot;
+  }
+}
+
+__global__ void k_silu_back(const float* da, const float* z, long n, uint16_t* dz) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = f2bf(da[i] * dsilu(z[i]));
+}
+
+// Column sums of dz [B][256] over 64-row chunks: part[chunk][j].
+__global__ void __launch_bounds__(256) k_colsum_part(const uint16_t* dz, int B, float* part) {
+  const int j = threadIdx.x, r0 = blockIdx.x * 64;
+  float s = 0.f;
+  for (int r = r0; r < r0 + 64 && r < B; r++) s += bf2f(dz[(long)r * MLP_H + j]);
+  part[(long)blockIdx.x * MLP_H + j] = s;
+}
+
+__global__ void k_adam(float* w, const float* g, float* m, float* v, uint16_t* wbf, long n, const int* step,
+                       float lr, float b1, float b2, float eps) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = (float)(*step + 1);
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / (1.f - powf(b1, t));
+  const float vh = vi / (1.f - powf(b2, t));
+  const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+  w[i] = wi;
+  wbf[i] = f2bf(wi);
+}
+
+__global__ void k_step_inc(int* step) { step[0] = step[0] + 1; }
+
+__global__ void k_f2bf(const float* a, uint16_t* b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = f2bf(a[i]);
+}
+
+__global__ void k_transpose_bf(const uint16_t* a, int rows, int cols, uint16_t* at) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)rows * cols) return;
+  const int r = (int)(i / cols), c = (int)(i % cols);
+  at[(long)c * rows + r] = a[i];
+}
+
+__global__ void k_fill_f32(float* p, long n, float v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: 64 samples per block, 4 waves; wave w computes rows
+// 32 (w & 1) .. +32 and columns 128 (w >> 1) .. +128 (four 32 x 32 tiles) of
+// every hidden layer.  Activations stay in LDS (BF16), the transposed
+// weights are read from L2 as 16-byte B fragments.
+// ---------------------------------------------------------------------------
+namespace {
+template <int K>
+__device__ inline void hidden_layer(const uint16_t (*in)[LDH], const uint16_t* wt, const float* bias,
+                                    uint16_t (*out)[LDH], int wm, int nb, int lane) {
+  const int h = lane >> 5, r = lane & 31;
+  f32x16 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = zero16();
+  for (int kk = 0; kk < K; kk += 16) {
+    const bf16x8 a = *(const bf16x8*)&in[wm + r][kk + 8 * h];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const bf16x8 b = *(const bf16x8*)&wt[(long)(nb + 32 * t + r) * K + kk + 8 * h];
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int j = nb + 32 * t + r;
+    const float bj = bias[j];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int i = wm + 8 * (q >> 2) + 4 * h + (q & 3);
+      out[i][j] = f2bf(silu(acc[t][q] + bj));
+    }
+  }
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
+  __shared__ __align__(16) uint16_t X0[64][LDH];
+  __shared__ __align__(16) uint16_t H0[64][LDH];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long s0 = (long)blockIdx.x * 64;
+  // x rows (32 BF16 each) into X0[:, 0..31]
+  for (int i = tid; i < 64 * (MLP_INP / 8); i += 256) {
+    const int r = i / (MLP_INP / 8), c = (i % (MLP_INP / 8)) * 8;
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = 0;
+    if (s0 + r < a.n) v = *(const bf16x8*)&a.x[(s0 + r) * MLP_INP + c];
+    *(bf16x8*)&X0[r][c] = v;
+  }
+  __syncthreads();
+  const int wm = (w & 1) * 32, nb = (w >> 1) * 128;
+  hidden_layer<MLP_INP>(X0, a.w1t, a.params + MLP_B1, H0, wm, nb, lane);
+  __syncthreads();
+  hidden_layer<MLP_H>(H0, a.w2t, a.params + MLP_B2, X0, wm, nb, lane);
+  __syncthreads();
+  hidden_layer<MLP_H>(X0, a.w3t, a.params + MLP_B3, H0, wm, nb, lane);
+  __syncthreads();
+  hidden_layer<MLP_H>(H0, a.w4t, a.params + MLP_B4, X0, wm, nb, lane);
+  __syncthreads();
+  // output: 4 threads per sample, 64 columns each
+  const int smp = tid >> 2, part = tid & 3;
+  float s = 0.f;
+  for (int j = part * 64; j < part * 64 + 64; j++) s += bf2f(X0[smp][j]) * bf2f(a.wbf[MLP_W5 + j]);
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  if (part == 0 && s0 + smp < a.n) a.y[s0 + smp] = s + a.params[MLP_B5];
+}
+
+// ---- launchers --------------------------------------------------------------
+static inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
+
+void launch_gemm(const GemmArgs& g, hipStream_t s) {
+  dim3 grid((g.N + TS - 1) / TS, (g.M + TS - 1) / TS, g.ksplit);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
+}
+void launch_splitk_sum(const float* part, int nslab, long slab, long n, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_splitk_sum, dim3(nb256(n)), dim3(256), 0, s, part, nslab, slab, n, out);
+}
+void launch_gather_batch(const uint16_t* X, const float* Y, long ntot, int B, const int* step, unsigned seed,
+                         uint16_t* xb, float* yb, hipStream_t s) {
+  (void)seed;
+  hipLaunchKernelGGL(k_gather_batch, dim3(nb256(B)), dim3(256), 0, s, X, Y, ntot, B, step, xb, yb);
+}
+void launch_out_fwd(const uint16_t* act, const uint16_t* w5, const float* b5, int B, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_out_fwd, dim3((B + 3) / 4), dim3(256), 0, s, act, w5, b5, B, y);
+}
+void launch_loss_grad(const float* y, const float* t, int B, float* dy, float* loss_hist, const int* step,
+                      int mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_loss_grad, dim3(1), dim3(1024), 0, s, y, t, B, dy, loss_hist, step, mask);
+}
+void launch_out_bwd(const uint16_t* act4, const float* z4, const float* dy, const uint16_t* w5, int B,
+                    float* gw5, float* gb5, uint16_t* dz4, hipStream_t s) {
+  hipLaunchKernelGGL(k_out_dz, dim3(nb256((long)B * MLP_H)), dim3(256), 0, s, z4, dy, w5, B, dz4);
+  hipLaunchKernelGGL(k_out_wgrad, dim3(MLP_H + 1), dim3(256), 0, s, act4, dy, B, gw5, gb5);
+}
+void launch_silu_back(const float* da, const float* z, long n, uint16_t* dz, hipStream_t s) {
+  hipLaunchKernelGGL(k_silu_back, dim3(nb256(n)), dim3(256), 0, s, da, z, n, dz);
+}
+void launch_colsum(const uint16_t* dz, int B, int N, float* part, hipStream_t s) {
+  (void)N;
+  hipLaunchKernelGGL(k_colsum_part, dim3((B + 63) / 64), dim3(MLP_H), 0, s, dz, B, part);
+}
+void launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wbf, long n, const int* step, float lr,
+                 float b1, float b2, float eps, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3(nb256(n)), dim3(256), 0, s, w, g, m, v, wbf, n, step, lr, b1, b2, eps);
+}
+void launch_step_inc(int* step, hipStream_t s) { hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(1), 0, s, step); }
+void launch_f2bf(const float* a, uint16_t* b, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2bf, dim3(nb256(n)), dim3(256), 0, s, a, b, n);
+}
+void launch_transpose_bf(const uint16_t* a, int rows, int cols, uint16_t* at, hipStream_t s) {
+  hipLaunchKernelGGL(k_transpose_bf, dim3(nb256((long)rows * cols)), dim3(256), 0, s, a, rows, cols, at);
+}
+void launch_fill_f32(float* p, long n, float v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_f32, dim3(nb256(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_mlp_infer(const InferArgs& a, hipStream_t s) {
+  if (a.n > 0) hipLaunchKernelGGL(k_mlp_infer, dim3((a.n + 63) / 64), dim3(256), 0, s, a);
+}

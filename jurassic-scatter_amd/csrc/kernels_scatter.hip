@@ -118,6 +118,8 @@ __global__ void k_emit_points(int nray, const int* np, const float* rec, int R, 
     P.il = (int)pr[(size_t)ip * R + REC_AI];
     P.atm = atm_idx ? atm_idx[r] : 0;
     P.time = time[r];
+    P.af = pr[(size_t)ip * R + REC_AF];
+    P.pad = 0;
     pts[k++] = P;
   }
 }
@@ -223,6 +225,25 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
   __shared__ double alpha[NALPHA];
   const int k = blockIdx.x;
   const ScaPt P = G.pts[k];
+  const int sk = G.sub_idx ? G.sub_idx[k] : k;
+  if (sk < 0) {
+    // surrogate point: thermal source from the network, solar term as below
+    const double* ph = G.phase + (size_t)P.il * G.nd * NTHETA;
+    for (int id = threadIdx.x; id < G.nd; id += blockDim.x) {
+      double src = exp((double)G.ymlp[(size_t)k * G.nd + id] * G.ysd + G.ymu);
+      if (G.jth) G.jth[(size_t)k * G.nd + id] = (float)src;
+      const int si = G.sun_idx[k];
+      if (si >= 0) {
+        const double* p = ph + (size_t)id * NTHETA;
+        const double t2 = G.sun_theta[k];
+        int idx = locate_mono<double>(G.theta_grid, NTHETA, t2);
+        double p2 = lin(G.theta_grid[idx], p[idx], G.theta_grid[idx + 1], p[idx + 1], t2);
+        src += 6.764e-5 * p2 * G.bsun[id] * (double)G.tau_sun[(size_t)id * G.nsun + si];
+      }
+      G.J[(size_t)k * G.nd + id] = (float)src;
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < 180 * NALPHA; i += blockDim.x) (&Aw[0][0])[i] = 0.0;
   if (threadIdx.x < NALPHA) alpha[threadIdx.x] = G.alpha[threadIdx.x];
   __syncthreads();
@@ -246,7 +267,7 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
     }
   }
   __syncthreads();
-  const size_t c0 = (size_t)k * NALPHA;
+  const size_t c0 = (size_t)sk * NALPHA;
   const double* ph = G.phase + (size_t)P.il * G.nd * NTHETA;
   for (int id = threadIdx.x; id < G.nd; id += blockDim.x) {
     double r[NALPHA];
@@ -264,6 +285,7 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
       den += 180.0 * w;
     }
     double src = num / den;
+    if (G.jth) G.jth[(size_t)k * G.nd + id] = (float)src;
     const int si = G.sun_idx[k];
     if (si >= 0) {
       const double t2 = G.sun_theta[k];

@@ -16,6 +16,7 @@
 
 #include "jur_scatter.h"
 #include "jur_scatter_dev.h"
+#include "jur_surrogate.h"
 
 namespace {
 
@@ -67,6 +68,13 @@ struct Level {
 struct State {
   DBuf geo, geo2, rec, npts, tsurf, gf, scan, alpha, theta_grid, bsun, phase, status, tpd;
   int nd_phase = 0;
+  // layer phase-function moments for the surrogate features ([nl][nd])
+  DBuf asym, pfwd, pback;
+  // surrogate: features, network output, hull flags, fallback compaction
+  DBuf fx, fy, fflag, fpos, fsub, fpts;
+  // collected training pairs of the last call (raw features, thermal J)
+  DBuf craw, cjth;
+  long ncollect = 0;
 };
 State& ST() {
   static State s;
@@ -172,6 +180,25 @@ void scatter_set_phase(const ctl_t* ctl, const aero_t* aero, hipStream_t s) {
       for (int id = 0; id < nd; id++)
         for (int it = 0; it < NTHETA; it++) ph[((size_t)il * nd + id) * NTHETA + it] = aero->p[il][id][it];
   upload_vec(st.phase, ph, s);
+  // asymmetry parameter g = int P cos sin / int P sin (trapezoid on the
+  // 1-degree grid), forward and backward phase values
+  std::vector<float> g((size_t)NLMAX * nd, 0.f), pf((size_t)NLMAX * nd, 0.f), pb((size_t)NLMAX * nd, 0.f);
+  for (int il = 0; il < NLMAX; il++)
+    for (int id = 0; id < nd; id++) {
+      const double* q = &ph[((size_t)il * nd + id) * NTHETA];
+      double num = 0, den = 0;
+      for (int it = 0; it < NTHETA - 1; it++) {
+        const double t0 = M_PI * it / (NTHETA - 1), t1 = M_PI * (it + 1) / (NTHETA - 1);
+        num += 0.5 * (q[it] * cos(t0) * sin(t0) + q[it + 1] * cos(t1) * sin(t1)) * (t1 - t0);
+        den += 0.5 * (q[it] * sin(t0) + q[it + 1] * sin(t1)) * (t1 - t0);
+      }
+      g[(size_t)il * nd + id] = den > 0 ? (float)(num / den) : 0.f;
+      pf[(size_t)il * nd + id] = (float)q[0];
+      pb[(size_t)il * nd + id] = (float)q[NTHETA - 1];
+    }
+  upload_vec(st.asym, g, s);
+  upload_vec(st.pfwd, pf, s);
+  upload_vec(st.pback, pb, s);
   std::vector<double> alpha(NALPHA), tg(NTHETA), bs(nd);
   alpha_grid(alpha.data());
   for (int i = 0; i < NTHETA; i++) tg[i] = M_PI * (double)i / (NTHETA - 1);
@@ -186,6 +213,14 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
                      hipStream_t s) {
   State& st = ST();
   const int nd = ctl->nd, Rl = rec_len(ctl->ng, ctl->nw);
+  // surrogate for the thermal source at the primaries' points (option
+  // "surrogate", once a network is trained or loaded); "collect" keeps the
+  // plain method's (features, thermal J) pairs of those points
+  const bool use_sur = engine_option_int("surrogate", 0) && mlp_trained();
+  const bool collect = engine_option_int("collect", 0) != 0;
+  st.ncollect = 0;
+  engine_set_stat("surrogate_points", 0);
+  engine_set_stat("fallback_points", 0);
   static std::vector<Level> lv;
   if ((int)lv.size() < S + 1) lv.resize(S + 1);
   lv[0].rays = rays;
@@ -236,12 +271,56 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
                          st.geo2.as<double4>(), st.gf.as<int>(), jbase + r0, C.time, C.atm_idx, pts, s);
       r0 += m;
     }
+    // surrogate / training-pair collection at the primaries' points
+    const ScaPt* fan_pts = pts;
+    int nfan = V.npt;
+    if (L == 0 && (use_sur || collect) && V.npt > 0) {
+      const long nrow = (long)V.npt * nd;
+      FeatArgs F;
+      memset(&F, 0, sizeof F);
+      F.npt = V.npt;
+      F.nd = nd;
+      F.pts = pts;
+      F.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw};
+      F.ig_h2o = engine_find_emitter(ctl, "H2O");
+      F.beta_e = engine_beta_e();
+      F.beta_s = engine_beta_s();
+      F.asym = st.asym.as<float>();
+      F.pfwd = st.pfwd.as<float>();
+      F.pback = st.pback.as<float>();
+      engine_layer_edges(F.top, F.bottom);
+      F.nu = engine_nu_dev();
+      F.margin = (float)engine_option_double("hull_margin", 0.25);
+      if (collect) F.raw = (float*)st.craw.need((size_t)nrow * MLP_IN * sizeof(float));
+      int* flag = (int*)st.fflag.need((size_t)(V.npt + 1) * sizeof(int));
+      if (use_sur) {
+        F.norm = (const MlpNorm*)mlp_norm_dev();
+        F.x = (uint16_t*)st.fx.need((size_t)nrow * MLP_INP * 2);
+        F.out_hull = flag;
+        launch_zero_int(flag, V.npt, s);
+      }
+      launch_features(F, s);
+      st.ncollect = collect ? nrow : 0;
+      if (use_sur) {
+        mlp_infer_dev(F.x, nrow, (float*)st.fy.need((size_t)nrow * sizeof(float)), s);
+        int* pos = (int*)st.fpos.need((size_t)(V.npt + 1) * sizeof(int));
+        size_t tb = scan_temp_bytes(V.npt);
+        launch_exclusive_scan(flag, pos, st.scan.need(tb), tb, V.npt, s);
+        nfan = read_int(pos + V.npt - 1, s) + read_int(flag + V.npt - 1, s);
+        ScaPt* sub_pts = (ScaPt*)st.fpts.need((size_t)(nfan + 1) * sizeof(ScaPt));
+        launch_fallback_compact(V.npt, flag, pos, pts, (int*)st.fsub.need((size_t)(V.npt + 1) * sizeof(int)),
+                                sub_pts, s);
+        fan_pts = sub_pts;
+        engine_set_stat("surrogate_points", (double)(V.npt - nfan));
+        engine_set_stat("fallback_points", (double)nfan);
+      }
+    }
     // secondary fans -> level L+1
     Level& W = lv[L + 1];
-    long nchild = (long)V.npt * NALPHA;
+    long nchild = (long)nfan * NALPHA;
     if (nchild > 0x7fffffffL) JUR_FAIL("Too many queue items!");
     W.own.alloc((int)nchild);
-    launch_children(V.npt, pts, st.alpha.as<double>(), W.own.arrays(), s);
+    launch_children(nfan, fan_pts, st.alpha.as<double>(), W.own.arrays(), s);
     W.rays = W.own.view();
     pencils += nchild;
     // solar rays of the points
@@ -306,6 +385,18 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     G.theta_grid = st.theta_grid.as<double>();
     G.bsun = st.bsun.as<double>();
     G.J = J;
+    G.sub_idx = nullptr;
+    G.ymlp = nullptr;
+    G.ymu = 0.f;
+    G.ysd = 1.f;
+    G.jth = nullptr;
+    if (L == 0 && use_sur && V.npt > 0) {
+      G.sub_idx = st.fsub.as<int>();
+      G.ymlp = st.fy.as<float>();
+      G.ymu = mlp_norm().ymu;
+      G.ysd = mlp_norm().ysd;
+    }
+    if (L == 0 && collect && V.npt > 0) G.jth = (float*)st.cjth.need((size_t)V.npt * nd * sizeof(float));
     launch_angular(G, s);
     if (L == 0) {
       integrate_set(ctl, atm, V.rays, S, J, V.jbase.as<int>(), out.rad, out.tau, out.tpz, out.tplon, out.tplat, s);
@@ -332,4 +423,13 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
   engine_set_stat("scattering_points", (double)npts0);
   HIP_OK(hipEventDestroy(e0));
   HIP_OK(hipEventDestroy(e1));
+}
+
+long scatter_collected(float* feat, float* jth, long cap) {
+  State& st = ST();
+  const long n = st.ncollect < cap ? st.ncollect : cap;
+  if (n <= 0) return st.ncollect;
+  HIP_OK(hipMemcpy(feat, st.craw.p, (size_t)n * MLP_IN * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(jth, st.cjth.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+  return st.ncollect;
 }

@@ -11,6 +11,7 @@ LIBPATH = PKG / "libjurassic_amd.so"
 # Every symbol include/jurassic_amd.h declares, with (restype, argtypes).
 P = C.c_void_p
 D = C.POINTER(C.c_double)
+F = C.POINTER(C.c_float)
 SIGNATURES = {
     "formod": (None, [P, P, P, P]),
     "formod_pencil": (None, [P, P, P, P, C.c_int, C.c_int]),
@@ -36,6 +37,17 @@ SIGNATURES = {
     "jur_init": (C.c_int, [C.c_int]),
     "jur_set_option": (C.c_int, [C.c_char_p, C.c_double]),
     "jur_stat": (C.c_double, [C.c_char_p]),
+    "jur_mlp_init": (C.c_int, [C.c_uint]),
+    "jur_mlp_set_data": (C.c_int, [F, F, C.c_long, C.c_int, C.c_uint]),
+    "jur_mlp_train": (C.c_int, [C.c_int, C.c_int, C.c_float, F]),
+    "jur_mlp_grads": (C.c_int, [F, F, C.c_int, F, F]),
+    "jur_mlp_forward": (C.c_int, [F, C.c_long, F]),
+    "jur_mlp_get_params": (C.c_int, [F]),
+    "jur_mlp_set_params": (C.c_int, [F]),
+    "jur_mlp_nparam": (C.c_int, []),
+    "jur_mlp_save": (C.c_int, [C.c_char_p]),
+    "jur_mlp_load": (C.c_int, [C.c_char_p]),
+    "jur_sca_collect": (C.c_long, [F, F, C.c_long]),
 }
 
 
