@@ -131,9 +131,10 @@ double jur_stat(const char *key);
 
 /* ---- Scattering-source surrogate (no reference counterpart) ---------- */
 
-/* A perceptron 24 -> 4 x 256 (SiLU) -> 1 predicting the normalised
-   log of the thermal scattering source J at a scattering point from 24
-   local features (listed in the library's jur_surrogate.h); BF16 weights and
+/* A perceptron 24 -> 4 x 256 (SiLU) -> 1 predicting ln(J / B(T)), J the
+   thermal scattering source at a scattering point and B(T) the Planck
+   radiance at its temperature (feature 12), from 24 local features (listed
+   in the library's jur_surrogate.h); BF16 weights and
    activations, FP32 accumulation, trained with Adam on the GPU.  With
    jur_set_option("surrogate", 1) and a trained or loaded network, the
    scattering forward model takes J at the primaries' scattering points
@@ -143,7 +144,8 @@ double jur_stat(const char *key);
    parameter layout is jur_mlp_nparam() floats (W1 [32][256], b1, W2..W4
    [256][256], b2..b4, W5 [256], b5). */
 int jur_mlp_init(unsigned seed);
-/* Training pairs: x [n][24] raw features, y [n] targets (log J).  normalize:
+/* Training pairs: x [n][24] raw features, y [n] targets (ln J - feature 12
+   for the scattering source).  normalize:
    1 sets the feature / target normalisation and training hull from these
    data and shuffles the rows (seed); 0 keeps the current normalisation. */
 int jur_mlp_set_data(const float *x, const float *y, long n, int normalize, unsigned seed);
@@ -155,7 +157,8 @@ int jur_mlp_train(int steps, int batch, float lr, float *loss_hist);
 /* Gradients of the mean squared error on one batch (x raw features, y
    targets) without updating: grads [jur_mlp_nparam()], loss (optional). */
 int jur_mlp_grads(const float *x, const float *y, int batch, float *grads, float *loss);
-/* Network output (log J) for n feature rows, fused inference kernel. */
+/* Network output (de-normalised target) for n feature rows, fused
+   inference kernel. */
 int jur_mlp_forward(const float *x, long n, float *y);
 int jur_mlp_get_params(float *p);
 int jur_mlp_set_params(const float *p);

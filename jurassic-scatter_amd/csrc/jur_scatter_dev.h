@@ -51,9 +51,11 @@ struct AngArgs {
   float* J;                  // [npt][nd]
   // surrogate: sub_idx[k] >= 0 is the point's fan block in rad (plain
   // angular sum); -1 takes the thermal source from the network output
-  // ymlp[k][nd] (normalised log J) instead.  Null: every point is plain.
+  // ymlp[k][nd] (normalised ln(J / B(T)), J = exp(y ysd + ymu + lnb))
+  // instead.  Null: every point is plain.
   const int* sub_idx;
   const float* ymlp;
+  const float* lnb;          // [npt][nd] ln B(T) at the point
   float ymu, ysd;
   float* jth;                // [npt][nd] thermal part (no sun), or null
 };

@@ -21,6 +21,9 @@ const float* mlp_norm_dev();  // device copy (MlpNorm layout)
 void mlp_infer_dev(const uint16_t* x, long n, float* y, hipStream_t s);
 bool mlp_trained();
 
+// The network predicts ln(J / B(T)) of the thermal source J, B(T) the
+// Planck radiance at the point's temperature (feature 12).
+//
 // Features of every (scattering point, channel) pair, row k * nd + id:
 //   0 ln p, 1 T, 2 z, 3 ln(af beta_e), 4 single-scattering albedo,
 //   5 asymmetry g, 6 ln P(0), 7 ln P(180), 8 cos of the LOS zenith angle,
@@ -28,7 +31,8 @@ bool mlp_trained();
 //   12..16 ln B(T) at the point, the ground, z + 3 km, z - 3 km and the
 //   coldest level above, 17 / 18 ln H2O column above / below,
 //   19 ln(vertical aerosol optical depth of the layer),
-//   20..23 sin / cos of 2 pi nu / 200 and 2 pi nu / 700.
+//   20 / 21 sin / cos of 2 pi nu / 200, 22 / 23 ln of the layer's vertical
+//   optical depth above / below the point (+1e-6).
 struct FeatArgs {
   int npt, nd;
   const ScaPt* pts;
@@ -46,6 +50,7 @@ struct FeatArgs {
   uint16_t* x;          // [npt nd][32] normalised BF16 or null
   int* out_hull;        // [npt]: set to 1 when a channel leaves the hull
   float margin;         // hull margin in normalised units
+  float* lnb;           // [npt nd] ln B(T) at the point (feature 12), or null
 };
 void launch_features(const FeatArgs& a, hipStream_t s);
 void launch_fallback_compact(int npt, const int* flag, const int* pos, const ScaPt* pts, int* sub,

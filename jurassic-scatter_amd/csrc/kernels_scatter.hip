@@ -230,7 +230,8 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
     // surrogate point: thermal source from the network, solar term as below
     const double* ph = G.phase + (size_t)P.il * G.nd * NTHETA;
     for (int id = threadIdx.x; id < G.nd; id += blockDim.x) {
-      double src = exp((double)G.ymlp[(size_t)k * G.nd + id] * G.ysd + G.ymu);
+      const size_t o = (size_t)k * G.nd + id;
+      double src = exp((double)G.ymlp[o] * G.ysd + G.ymu + (double)G.lnb[o]);
       if (G.jth) G.jth[(size_t)k * G.nd + id] = (float)src;
       const int si = G.sun_idx[k];
       if (si >= 0) {

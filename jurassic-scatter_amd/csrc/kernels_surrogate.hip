@@ -76,10 +76,13 @@ __global__ void k_features(FeatArgs F) {
   f[19] = (float)log(tau_l > 1e-20 ? tau_l : 1e-20);
   f[20] = (float)sin(2 * M_PI * nu / 200.0);
   f[21] = (float)cos(2 * M_PI * nu / 200.0);
-  f[22] = (float)sin(2 * M_PI * nu / 700.0);
-  f[23] = (float)cos(2 * M_PI * nu / 700.0);
+  // vertical optical depth of the layer above and below the point
+  const double ta = ext * fmax(F.top[il] - z, 0.0), tb = ext * fmax(z - F.bottom[il], 0.0);
+  f[22] = (float)log(ta + 1e-6);
+  f[23] = (float)log(tb + 1e-6);
   if (F.raw)
     for (int i = 0; i < MLP_IN; i++) F.raw[t * MLP_IN + i] = f[i];
+  if (F.lnb) F.lnb[t] = f[12];
   if (F.x) {
     const MlpNorm& N = *F.norm;
     bool out = false;

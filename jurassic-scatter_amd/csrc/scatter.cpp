@@ -71,7 +71,7 @@ struct State {
   // layer phase-function moments for the surrogate features ([nl][nd])
   DBuf asym, pfwd, pback;
   // surrogate: features, network output, hull flags, fallback compaction
-  DBuf fx, fy, fflag, fpos, fsub, fpts;
+  DBuf fx, fy, fflag, fpos, fsub, fpts, flnb;
   // collected training pairs of the last call (raw features, thermal J)
   DBuf craw, cjth;
   long ncollect = 0;
@@ -296,6 +296,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
       if (use_sur) {
         F.norm = (const MlpNorm*)mlp_norm_dev();
         F.x = (uint16_t*)st.fx.need((size_t)nrow * MLP_INP * 2);
+        F.lnb = (float*)st.flnb.need((size_t)nrow * sizeof(float));
         F.out_hull = flag;
         launch_zero_int(flag, V.npt, s);
       }
@@ -387,12 +388,14 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     G.J = J;
     G.sub_idx = nullptr;
     G.ymlp = nullptr;
+    G.lnb = nullptr;
     G.ymu = 0.f;
     G.ysd = 1.f;
     G.jth = nullptr;
     if (L == 0 && use_sur && V.npt > 0) {
       G.sub_idx = st.fsub.as<int>();
       G.ymlp = st.fy.as<float>();
+      G.lnb = st.flnb.as<float>();
       G.ymu = mlp_norm().ymu;
       G.ysd = mlp_norm().ysd;
     }
@@ -421,6 +424,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
   long npts0 = 0;
   for (int L = 0; L < S; L++) npts0 += lv[L].npt;
   engine_set_stat("scattering_points", (double)npts0);
+  engine_set_stat("scattering_points0", (double)lv[0].npt);
   HIP_OK(hipEventDestroy(e0));
   HIP_OK(hipEventDestroy(e1));
 }
