@@ -38,7 +38,7 @@ CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 4096, "aerosol_limb_8192
 # waves per block).  The profile summary named beside each figure holds the
 # commands and the raw counter values.
 MEASURED_TRAFFIC = {
-    ("clear_batch_65536", 65536, 2, 4): (2.1814e11, "profiles/k_path_fast_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 2, 4): (1.7182e11, "profiles/k_path_fast_clear_batch_65536.md"),
 }
 
 
@@ -55,7 +55,197 @@ def parse():
     ap.add_argument("--variant", type=int, default=None,
                     help="path-kernel variant (default: autotuned on the first call)")
     ap.add_argument("--wpb", type=int, default=None, help="waves per block (default: autotuned)")
+    ap.add_argument("--weights", default=None,
+                    help="trained surrogate for --workload cirrus_16384 (default: the packaged one)")
+    ap.add_argument("--plain-rays", type=int, default=64,
+                    help="rays of the plain-method comparison for --workload cirrus_16384")
     return ap.parse_args()
+
+
+def quiet(fn):
+    """Run fn with stdout silenced (the control parser echoes every key)."""
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    saved = os.dup(1)
+    os.dup2(devnull, 1)
+    try:
+        return fn()
+    finally:
+        os.dup2(saved, 1)
+        os.close(devnull)
+
+
+def setup(lib, workload, nray):
+    """Workload files, ctl and (for aerosol workloads) aero with the layer
+    optics from the library's own Mie (get_opt_prop), before any timing."""
+    from pyjurassic import jtypes as J, synth
+    tmp = Path(tempfile.mkdtemp(prefix="jur_bench_"))
+    w = synth.make(workload, tmp, nray=nray)
+    w.write_ctl(tmp / "bench.ctl")
+    ctl = J.ctl_t()
+    argc, argv = J.argv(tmp / "bench.ctl")
+    quiet(lambda: lib.read_ctl(argc, argv, C.byref(ctl)))
+    aero, scattering = None, 0
+    if w.aerofile:
+        aero = J.aero_t()
+        lib.read_aero(None, w.aerofile.encode(), C.byref(ctl), C.byref(aero))
+        lib.get_opt_prop(C.byref(ctl), C.byref(aero))
+        scattering = ctl.sca_mult
+    return w, ctl, aero, scattering, (argc, argv)
+
+
+def device_batch(w, nd, rays=None):
+    """jur_batch_t over device copies (torch) of w's rays (a subset with
+    `rays`); returns (batch, outputs, tensors kept alive)."""
+    import torch
+    from pyjurassic import lib as L
+    if rays is not None:
+        w = w.select(rays=rays)
+    dev = torch.device("cuda:0")
+    d = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt)
+    z, p, t, q = d(w.z), d(w.p), d(w.t), d(w.q)
+    geom = [d(w.geom[i]) for i in range(7)]
+    rad = torch.empty((nd, w.nray), dtype=torch.float32, device=dev)
+    tau = torch.empty((nd, w.nray), dtype=torch.float32, device=dev)
+    tpz = torch.empty(w.nray, dtype=torch.float64, device=dev)
+    npts = torch.empty(w.nray, dtype=torch.int32, device=dev)
+    b = L.jur_batch_t()
+    b.nray, b.nlev, b.natm = w.nray, w.nlev, w.natm
+    b.atm_of_ray = None
+    b.z, b.p, b.t, b.q, b.k = z.data_ptr(), p.data_ptr(), t.data_ptr(), q.data_ptr(), None
+    (b.time, b.obsz, b.obslon, b.obslat, b.vpz, b.vplon, b.vplat) = [g.data_ptr() for g in geom]
+    b.rad, b.tau, b.tpz, b.tplon, b.tplat, b.npts = (rad.data_ptr(), tau.data_ptr(), tpz.data_ptr(), None, None,
+                                                     npts.data_ptr())
+    torch.cuda.synchronize()
+    return b, (rad, tau, tpz, npts), (z, p, t, q, geom)
+
+
+def ncores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_oracle(w, argc, argv, sel, scattering):
+    """The FP64 oracle on rays `sel` of w with all host cores: (rad, tau
+    [ns][nd], seconds of the timed call, cores)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_ctypes
+    from pyjurassic import jtypes as J
+    olib = oracle_ctypes.load()
+    octl = J.ctl_t()
+    quiet(lambda: olib.oracle_read_ctl(argc, argv, C.byref(octl)))
+    oaero = None
+    if w.aerofile:
+        oaero = J.aero_t()
+        olib.oracle_read_aero(None, w.aerofile.encode(), C.byref(octl), C.byref(oaero))
+        olib.oracle_get_opt_prop(C.byref(octl), C.byref(oaero))
+    ns, nd = len(sel), octl.nd
+    arr = lambda a: np.ascontiguousarray(a)
+    zs, ps, ts, qs = arr(w.z[sel]), arr(w.p[sel]), arr(w.t[sel]), arr(w.q[sel])
+    gs = [arr(w.geom[i][sel]) for i in range(7)]
+    orad = np.zeros((ns, nd))
+    otau = np.zeros((ns, nd))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    nc = ncores()
+    call = lambda n: olib.oracle_formod_batch(
+        C.byref(octl), ns, w.nlev, dp(zs), dp(ps), dp(ts), dp(qs), None, *[dp(g) for g in gs], 0, n, nc,
+        dp(orad), dp(otau), None, None, None, None, C.byref(oaero) if oaero else None, scattering)
+    call(1)  # tables and optics loaded outside the timed call
+    c0 = time.perf_counter()
+    call(ns)
+    return orad, otau, time.perf_counter() - c0, nc
+
+
+def accuracy(grad, gtau, orad, otau, ctl):
+    """Relative radiance, brightness temperature and transmittance errors."""
+    grad, gtau = grad.astype(np.float64), gtau.astype(np.float64)
+    nd = orad.shape[1]
+    rel = np.abs(grad / orad - 1.0)
+    nu = np.array([ctl.nu[i] for i in range(nd)])
+    c1, c2 = 1.19104259e-8, 1.43877506
+    bt = lambda r: c2 * nu[None, :] / np.log1p(c1 * nu[None, :] ** 3 / r)
+    dbt = np.abs(bt(grad) - bt(orad))
+    return {"max_rel_rad": float(np.max(rel)), "mean_rel_rad": float(np.mean(rel)),
+            "max_bt_K": float(np.max(dbt)), "max_abs_tau": float(np.max(np.abs(gtau - otau)))}
+
+
+SURROGATE_WEIGHTS = ROOT / "jurassic-scatter_amd" / "data" / "surrogate_cirrus.bin"
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_surrogate(args, lib):
+    """Learned scattering source on the held-out cirrus set (16,384 rays x
+    32 channels, depth 2, other atmospheres than the training set): the
+    network (trained on the plain method over cirrus_4096 by
+    pyjurassic.surrogate) gives J at the primaries' scattering points, the
+    fused matrix-core kernel evaluates it; the plain method on the first
+    --plain-rays rays judges it (radiance difference, speedup of the call)."""
+    import torch
+    from pyjurassic import surrogate as S
+    weights = Path(args.weights) if args.weights else SURROGATE_WEIGHTS
+    if not weights.exists():
+        sys.exit("no trained surrogate at %s: python -m pyjurassic.surrogate --workload cirrus_4096 --out %s"
+                 % (weights, weights))
+    lib.jur_mlp_load(str(weights).encode())
+    info = weights.with_suffix(".json")
+    train = json.loads(info.read_text()) if info.exists() else None
+    w, ctl, aero, scattering, _ = setup(lib, "cirrus_16384", args.nray)
+    nd, nray = ctl.nd, w.nray
+    b, (rad, tau, _, _), keep = device_batch(w, nd)
+
+    def call(bb, sur):
+        lib.jur_set_option(b"surrogate", sur)
+        lib.jur_formod_batch_device(C.byref(ctl), C.byref(bb), C.byref(aero), scattering, None)
+        lib.jur_sync()
+
+    for _ in range(args.warmup):
+        call(b, 1)
+    torch.cuda.synchronize()
+    mlp_ms, samples = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call(b, 1)
+        mlp_ms.append(lib.jur_stat(b"mlp_ms"))
+        samples.append(lib.jur_stat(b"mlp_samples"))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nsur, nfb = lib.jur_stat(b"surrogate_points"), lib.jur_stat(b"fallback_points")
+    pencils = lib.jur_stat(b"pencils")
+    # plain method against the surrogate on a subset (second call of each timed)
+    sub = np.arange(min(args.plain_rays, nray))
+    bs, (rs, ts, _, _), keep2 = device_batch(w, nd, rays=sub)
+    res = {}
+    for sur in (0, 1):
+        call(bs, sur)
+        t1 = time.perf_counter()
+        call(bs, sur)
+        res[sur] = (time.perf_counter() - t1, rs.cpu().numpy().copy(), lib.jur_stat(b"pencils"))
+    rel = np.abs(res[1][1] / res[0][1] - 1.0)
+    mlp_avg = float(np.mean(mlp_ms))
+    achieved = float(np.mean(samples)) * S.INFER_FLOP_PER_SAMPLE / (mlp_avg * 1e-3) / 1e12
+    line = {
+        "metric": "ray x channel radiances/s, learned scattering source (depth 2)",
+        "value": nray * nd * args.steps / dt,
+        "unit": "ray*channel/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (network, fp32 accumulation), f32 (path integration), f64 (geometry)",
+        "data": "synthetic (seeded generator; held-out atmospheres and rays)",
+        "config": {"workload": "cirrus_16384", "rays": nray, "channels": nd, "scattering_depth": scattering,
+                   "pencils": pencils, "surrogate_points": nsur, "fallback_points": nfb,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_mlp_infer",
+                     "kernel_ms": mlp_avg, "samples": float(np.mean(samples)),
+                     "flop_per_sample": S.INFER_FLOP_PER_SAMPLE},
+        "surrogate": {"plain_rays": len(sub), "plain_s": res[0][0], "surrogate_s": res[1][0],
+                      "speedup": res[0][0] / res[1][0], "plain_pencils": res[0][2],
+                      "surrogate_pencils": res[1][2], "max_rel_rad": float(rel.max()),
+                      "mean_rel_rad": float(rel.mean()), "training": train},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -65,52 +255,19 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import torch
-    from pyjurassic import jtypes as J, lib as L, synth
+    from pyjurassic import jtypes as J, lib as L
 
     lib = L.load()
-    tmp = Path(tempfile.mkdtemp(prefix="jur_bench_"))
-    w = synth.make(args.workload, tmp, nray=args.nray)
-    w.write_ctl(tmp / "bench.ctl")
-    ctl = J.ctl_t()
-    argc, argv = J.argv(tmp / "bench.ctl")
-    devnull = os.open(os.devnull, os.O_WRONLY)
-    saved = os.dup(1)
-    os.dup2(devnull, 1)  # the control parser echoes every key
-    try:
-        lib.read_ctl(argc, argv, C.byref(ctl))
-    finally:
-        os.dup2(saved, 1)
-    aero, aero_p, scattering = None, None, 0
-    if w.aerofile:
-        # layer optics by the library's own Mie (get_opt_prop) before timing
-        aero = J.aero_t()
-        lib.read_aero(None, w.aerofile.encode(), C.byref(ctl), C.byref(aero))
-        lib.get_opt_prop(C.byref(ctl), C.byref(aero))
-        aero_p = C.byref(aero)
-        scattering = ctl.sca_mult
+    if args.workload == "cirrus_16384":
+        return bench_surrogate(args, lib)
+    w, ctl, aero, scattering, (argc, argv) = setup(lib, args.workload, args.nray)
+    aero_p = C.byref(aero) if aero is not None else None
     if args.variant is not None:
         lib.jur_set_option(b"variant", args.variant)
     if args.wpb is not None:
         lib.jur_set_option(b"wpb", args.wpb)
     nd, ng, nray = ctl.nd, ctl.ng, w.nray
-    dev = torch.device("cuda:0")
-
-    def d(a, dt=torch.float64):
-        return torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt)
-
-    z, p, t, q = d(w.z), d(w.p), d(w.t), d(w.q)
-    geom = [d(w.geom[i]) for i in range(7)]
-    rad = torch.empty((nd, nray), dtype=torch.float32, device=dev)
-    tau = torch.empty((nd, nray), dtype=torch.float32, device=dev)
-    tpz = torch.empty(nray, dtype=torch.float64, device=dev)
-    npts = torch.empty(nray, dtype=torch.int32, device=dev)
-    b = L.jur_batch_t()
-    b.nray, b.nlev, b.natm = nray, w.nlev, w.natm
-    b.atm_of_ray = None
-    b.z, b.p, b.t, b.q, b.k = z.data_ptr(), p.data_ptr(), t.data_ptr(), q.data_ptr(), None
-    (b.time, b.obsz, b.obslon, b.obslat, b.vpz, b.vplon, b.vplat) = [g.data_ptr() for g in geom]
-    b.rad, b.tau, b.tpz, b.tplon, b.tplat, b.npts = rad.data_ptr(), tau.data_ptr(), tpz.data_ptr(), None, None, npts.data_ptr()
-    torch.cuda.synchronize()
+    b, (rad, tau, tpz, npts), keep = device_batch(w, nd)
 
     def step():
         lib.jur_formod_batch_device(C.byref(ctl), C.byref(b), aero_p, scattering, None)
@@ -144,8 +301,9 @@ def main():
     alg_bytes = ipts_call * nd * (128 * ng + 8) + pencils * nd * 8
     path_avg = float(np.mean(path_ms))
     achieved = alg_bytes / (path_avg * 1e-3) / 1e9
-    variant, wpb = lib.jur_stat(b"tuned_variant"), lib.jur_stat(b"tuned_wpb")
-    traffic, traffic_src = MEASURED_TRAFFIC.get((args.workload, nray, int(variant), int(wpb)), (None, None))
+    variant = args.variant if args.variant is not None else int(lib.jur_stat(b"tuned_variant"))
+    wpb = args.wpb if args.wpb is not None else int(lib.jur_stat(b"tuned_wpb"))
+    traffic, traffic_src = MEASURED_TRAFFIC.get((args.workload, nray, variant, wpb), (None, None))
 
     line = {
         "metric": "ray x channel radiances/s (FP32 EGA + Kahan, FP64 geometry)",
@@ -172,59 +330,16 @@ def main():
     }
 
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle_ctypes
-        olib = oracle_ctypes.load()
-        octl = J.ctl_t()
-        os.dup2(devnull, 1)
-        try:
-            olib.oracle_read_ctl(argc, argv, C.byref(octl))
-        finally:
-            os.dup2(saved, 1)
-        oaero = None
-        if w.aerofile:
-            oaero = J.aero_t()
-            olib.oracle_read_aero(None, w.aerofile.encode(), C.byref(octl), C.byref(oaero))
-            olib.oracle_get_opt_prop(C.byref(octl), C.byref(oaero))
         ns = min(args.cpu_sample or CPU_SAMPLE[args.workload], nray)
         # every (nray/ns)-th ray: the sample spans limb and nadir geometries
         sel = np.linspace(0, nray - 1, ns).astype(np.int64)
-        arr = lambda a: np.ascontiguousarray(a)
-        zs, ps, ts, qs = arr(w.z[sel]), arr(w.p[sel]), arr(w.t[sel]), arr(w.q[sel])
-        gs = [arr(w.geom[i][sel]) for i in range(7)]
-        orad = np.zeros((ns, nd))
-        otau = np.zeros((ns, nd))
-        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-        ncores = os.cpu_count() or 1
-        try:
-            ncores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            pass
-        olib.oracle_formod_batch(C.byref(octl), ns, w.nlev, dp(zs), dp(ps), dp(ts), dp(qs), None,
-                                 dp(gs[0]), dp(gs[1]), dp(gs[2]), dp(gs[3]), dp(gs[4]), dp(gs[5]), dp(gs[6]),
-                                 0, 1, ncores, dp(orad), dp(otau), None, None, None, None,
-                                 C.byref(oaero) if oaero else None, scattering)
-        c0 = time.perf_counter()
-        olib.oracle_formod_batch(C.byref(octl), ns, w.nlev, dp(zs), dp(ps), dp(ts), dp(qs), None,
-                                 dp(gs[0]), dp(gs[1]), dp(gs[2]), dp(gs[3]), dp(gs[4]), dp(gs[5]), dp(gs[6]),
-                                 0, ns, ncores, dp(orad), dp(otau), None, None, None, None,
-                                 C.byref(oaero) if oaero else None, scattering)
-        cdt = time.perf_counter() - c0
-        line["cpu_baseline"] = {"value": ns * nd / cdt, "unit": "ray*channel/s", "cores": ncores,
+        orad, otau, cdt, nc = cpu_oracle(w, argc, argv, sel, scattering)
+        line["cpu_baseline"] = {"value": ns * nd / cdt, "unit": "ray*channel/s", "cores": nc,
                                 "kind": "port",
                                 "sample": "%d of the %d rays (evenly spaced), all %d channels, FP64 oracle, "
                                           "%.1f s wall" % (ns, nray, nd, cdt)}
-        grad = rad.cpu().numpy().T[sel].astype(np.float64)
-        gtau = tau.cpu().numpy().T[sel].astype(np.float64)
-        rel = np.abs(grad / orad - 1.0)
-        nu = np.array([ctl.nu[i] for i in range(nd)])
-        c1, c2 = 1.19104259e-8, 1.43877506
-        bt = lambda r: c2 * nu[None, :] / np.log1p(c1 * nu[None, :] ** 3 / r)
-        dbt = np.abs(bt(grad) - bt(orad))
-        line["accuracy"] = {"max_rel_rad": float(np.max(rel)), "mean_rel_rad": float(np.mean(rel)),
-                            "max_bt_K": float(np.max(dbt)), "max_abs_tau": float(np.max(np.abs(gtau - otau))),
-                            "sample_rays": ns}
-    os.close(devnull)
+        line["accuracy"] = accuracy(rad.cpu().numpy().T[sel], tau.cpu().numpy().T[sel], orad, otau, ctl)
+        line["accuracy"]["sample_rays"] = ns
     print(json.dumps(line), flush=True)
 
 

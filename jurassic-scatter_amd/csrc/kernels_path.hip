@@ -401,7 +401,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
   const DevTables& T = A.T;
 
   float ep[CPW][NGT], tp[CPW][NGT], radv[CPW], comp[CPW], tau[CPW];
-  int hint[CPW][NGT];
+  // column-search hints of the previous point live in LDS (one slot per lane
+  // and (channel, emitter); consecutive lanes, consecutive banks), which
+  // keeps them out of the register budget
+  __shared__ int hint_lds[4][CPW * NGT][64];
+  int (*hint)[64] = hint_lds[threadIdx.x >> 6];
   int hpt[NGT];  // packed (ipr, it0, it1) brackets of the previous point
 #pragma unroll
   for (int c = 0; c < CPW; c++) {
@@ -409,7 +413,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
     for (int g = 0; g < NGT; g++) {
       ep[c][g] = 0.f;
       tp[c][g] = 1.f;
-      hint[c][g] = 0;
+      hint[c * NGT + g][lane] = 0;
     }
     radv[c] = 0.f;
     comp[c] = 0.f;
@@ -476,10 +480,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
             const int2 m10 = T.meta[tb + gb1], m11 = T.meta[tb + gb1 + 1];
             if (m00.x >= 2 && m01.x >= 2 && m10.x >= 2 && m11.x >= 2) {
               const float e = ep[c][g];
-              const int h0 = hint[c][g];
+              const int h0 = hint[c * NGT + g][lane];
               int h = h0;
               EC e00 = corner_h(T.ue + m00.y, m00.x, e, du, h);
-              hint[c][g] = h;
+              hint[c * NGT + g][lane] = h;
               h = h0;
               EC e01 = corner_h(T.ue + m01.y, m01.x, e, du, h);
               h = h0;
@@ -581,13 +585,16 @@ void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
     case 1: launch_t<NGT, 1, 8>(A, wpb, s); break;
     case 2: launch_t<NGT, 2, 4>(A, wpb, s); break;
     case 3: launch_t<NGT, 4, 2>(A, wpb, s); break;
-    default: launch_t<NGT, 4, 4>(A, wpb, s); break;
+    case 4: launch_t<NGT, 4, 4>(A, wpb, s); break;
+    case 5: launch_t<NGT, 2, 3>(A, wpb, s); break;
+    case 6: launch_t<NGT, 1, 6>(A, wpb, s); break;
+    default: launch_t<NGT, 4, 3>(A, wpb, s); break;
   }
 }
 
 }  // namespace
 
-int path_variants() { return 5; }
+int path_variants() { return 8; }
 
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;

@@ -303,7 +303,19 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
       launch_features(F, s);
       st.ncollect = collect ? nrow : 0;
       if (use_sur) {
+        hipEvent_t m0, m1;
+        HIP_OK(hipEventCreate(&m0));
+        HIP_OK(hipEventCreate(&m1));
+        HIP_OK(hipEventRecord(m0, s));
         mlp_infer_dev(F.x, nrow, (float*)st.fy.need((size_t)nrow * sizeof(float)), s);
+        HIP_OK(hipEventRecord(m1, s));
+        HIP_OK(hipEventSynchronize(m1));
+        float mms = 0.f;
+        HIP_OK(hipEventElapsedTime(&mms, m0, m1));
+        HIP_OK(hipEventDestroy(m0));
+        HIP_OK(hipEventDestroy(m1));
+        engine_set_stat("mlp_ms", mms);
+        engine_set_stat("mlp_samples", (double)nrow);
         int* pos = (int*)st.fpos.need((size_t)(V.npt + 1) * sizeof(int));
         size_t tb = scan_temp_bytes(V.npt);
         launch_exclusive_scan(flag, pos, st.scan.need(tb), tb, V.npt, s);

@@ -26,6 +26,9 @@ Workloads (names used by bench.py and the tests):
 * ``cirrus_4096``: ice layer at 9-12 km (median radius U(5, 50) um, width
   1.6, number density U(0.01, 1) cm^-3, n = U(1.1, 1.5), k = U(0.01, 0.5)),
   4,096 limb and nadir rays, 32 channels, scattering depth 2.
+* ``cirrus_16384``: the same cloud and channels (same layer and refractive
+  indices), 16,384 other rays and atmospheres (own seed): the held-out set
+  of the scattering-source surrogate trained on ``cirrus_4096``.
 
 The scattering workloads observe at 2000-01-01 12:00 UTC (time 43200 s), so
 the solar source term is active; the layer optics use the reference's fixed
@@ -52,6 +55,7 @@ WORKLOADS = {
     "clear_batch_65536": 1,
     "aerosol_limb_8192": 2,
     "cirrus_4096": 3,
+    "cirrus_16384": 4,
 }
 
 
@@ -210,7 +214,8 @@ def make(name, outdir, nray=None, seed=None):
         nu = np.linspace(780.0, 1250.0, 32)
         z = np.arange(0.0, 121.0)
     tblbase = str(outdir / "syn")
-    trng = np.random.default_rng(SEED0 + 100 + cfg)
+    # the held-out cirrus set shares the training set's emissivity tables
+    trng = np.random.default_rng(SEED0 + 100 + (3 if cfg == 4 else cfg))
     for e in emitters:
         for v in nu:
             write_table("%s_%.4f_%s.bin" % (tblbase, v, e), trng)
@@ -231,7 +236,7 @@ def make(name, outdir, nray=None, seed=None):
         t = smooth3(tbase + rng.normal(0.0, 1.0, nlev))[None, :]
         q = qbase[None]
     else:
-        total = {1: 65536, 2: 8192, 3: 4096}[cfg]
+        total = {1: 65536, 2: 8192, 3: 4096, 4: 16384}[cfg]
         n = total if nray is None else min(nray, total)
         if cfg == 1:
             nlimb = int(round(0.6 * total))
@@ -244,7 +249,7 @@ def make(name, outdir, nray=None, seed=None):
             kind = np.zeros(total, int)
             th = rng.uniform(8.0, 30.0, total)
             zen = np.zeros(total)
-        else:
+        else:  # cirrus: half limb, half nadir
             kind = (rng.uniform(size=total) < 0.5).astype(int)
             th = rng.uniform(5.0, 30.0, total)
             zen = rng.uniform(0.0, 60.0, total)
@@ -258,8 +263,9 @@ def make(name, outdir, nray=None, seed=None):
         ih = emitters.index("H2O")
         q[:, ih, :] *= mult[:, None]
         total = n
-        if cfg in (2, 3):
-            arng = np.random.default_rng(SEED0 + 200 + cfg)
+        if cfg in (2, 3, 4):
+            # the cloud of the held-out cirrus set is the training set's
+            arng = np.random.default_rng(SEED0 + 200 + min(cfg, 3))
             if cfg == 2:
                 top, bot, rr, ss = 20.0, 15.0, arng.uniform(0.1, 1.0), arng.uniform(1.5, 2.0)
                 nn = arng.uniform(1.0, 100.0)
