@@ -116,7 +116,12 @@ int jur_init(int device);
    occupancy target, 0-4; default: autotuned), "wpb" waves per block (1, 2,
    4; default: autotuned), "autotune" (0/1, default 1), "fast" (0: always use
    the generic table search), "cap" initial LOS points per ray, "chunk_mb"
-   trace-buffer budget of the scattering levels in MiB (default 6144).
+   trace-buffer budget of the scattering levels in MiB (default 6144),
+   "graph" (1: a clear-air call is recorded once as a hipGraph -- trace,
+   sort, path integration -- and replayed while the inputs, outputs and
+   work buffers keep their addresses; a replay whose rays overflow the
+   recorded LOS capacity is detected and the call re-run), "surrogate",
+   "hull_margin", "collect" (learned scattering source, below).
    Returns 0. */
 int jur_set_option(const char *key, double value);
 /* Statistics of the last call: "path_ms" (path-integration kernel time,
@@ -125,6 +130,8 @@ int jur_set_option(const char *key, double value);
    "pencils" (primaries + secondary + solar pencils), "leaf_pencils",
    "scattering_points", "scatter_ms" (whole scattering call),
    "tuned_variant", "tuned_wpb", "tuned_ms" (autotuner choice),
+   "graph_ms" (replayed call), "graph_overflows", "surrogate_points",
+   "fallback_points", "mlp_ms", "mlp_samples", "train_ms",
    "shared_grids" (1: the fast table layout is in use), "mie_spheres"
    (spheres of the last get_opt_prop).  Unknown keys give NaN. */
 double jur_stat(const char *key);

@@ -40,6 +40,13 @@ struct Engine {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> path_ev;
   size_t path_ev_used = 0;
   DBuf pts_acc;
+  // captured clear-air forward call (option "graph"): the key holds every
+  // address and size the graph was recorded with
+  bool capturing = false;
+  DBuf hist;
+  std::string g_key;
+  hipGraph_t g_graph = nullptr;
+  hipGraphExec_t g_exec = nullptr;
 };
 
 Engine& E() {
@@ -352,15 +359,15 @@ void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
   HIP_OK(hipStreamSynchronize(s));
 }
 
-// Trace a set of rays into the given buffers; grows the per-ray capacity
-// until every ray fits.  Returns the capacity used.
-int engine_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on,
-                 TraceBufs& tb, double* tpz, double* tplon, double* tplat, hipStream_t s) {
+namespace {
+// Enqueue one ray trace of a ray set with per-ray capacity cap (status
+// flags cleared first); no host synchronisation.
+void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on, TraceBufs& tb,
+                   int cap, double* tpz, double* tplon, double* tplat, hipStream_t s) {
   Engine& e = E();
   const int R = rec_len(ctl->ng, ctl->nw);
-  int cap = tb.cap > 0 ? tb.cap : e.cap;
-  int ig_h2o = find_emitter(ctl, "H2O");
-  while (true) {
+  const int ig_h2o = find_emitter(ctl, "H2O");
+  {
     const size_t npt = (size_t)rays.nray * cap;
     TraceArgs A;
     memset(&A, 0, sizeof A);
@@ -405,8 +412,20 @@ int engine_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, 
     A.status = (int*)e.status.need(sizeof(int) * 4);
     launch_zero_int(A.status, 1, s);
     launch_raytrace(A, s);
+  }
+}
+}  // namespace
+
+// Trace a set of rays into the given buffers; grows the per-ray capacity
+// until every ray fits.  Returns the capacity used.
+int engine_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on,
+                 TraceBufs& tb, double* tpz, double* tplon, double* tplat, hipStream_t s) {
+  Engine& e = E();
+  int cap = tb.cap > 0 ? tb.cap : e.cap;
+  while (true) {
+    enqueue_trace(ctl, atm, rays, sca_on, tb, cap, tpz, tplon, tplat, s);
     int st = 0;
-    HIP_OK(hipMemcpyAsync(&st, A.status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&st, e.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     if (st & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
     if (st & JUR_ST_NLOS) JUR_FAIL("Too many LOS points!");
@@ -446,6 +465,90 @@ void end_call(hipStream_t s) {
 }
 }  // namespace
 
+namespace {
+// Every address and size a captured clear-air call depends on.
+std::string graph_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out,
+                      int scattering, hipStream_t s) {
+  Engine& e = E();
+  char k[1024];
+  snprintf(k, sizeof k, "%s|%s|%d %d %d %d|%p %p %p %p %p|%p %p %p %p %p %p %p %p|%p %p %p %p %p %p|"
+           "%p %p %p %p %p %p %p %p|%p|%d %g %g",
+           e.tbl_key.c_str(), e.chan_key.c_str(), rays.nray, atm.natm, atm.nlev, scattering,
+           (const void*)atm.z, (const void*)atm.p, (const void*)atm.t, (const void*)atm.q, (const void*)atm.k,
+           (const void*)rays.time, (const void*)rays.obsz, (const void*)rays.obslon, (const void*)rays.obslat,
+           (const void*)rays.vpz, (const void*)rays.vplon, (const void*)rays.vplat, (const void*)rays.atm_idx,
+           (void*)out.rad, (void*)out.tau, (void*)out.tpz, (void*)out.tplon, (void*)out.tplat, (void*)out.npts,
+           e.geo.p, e.rec.p, e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, (void*)s,
+           ctl->sca_n, ctl->rayds, ctl->raydz);
+  return k;
+}
+
+void drop_forward_graph() {
+  Engine& e = E();
+  if (e.g_exec) HIP_OK(hipGraphExecDestroy(e.g_exec));
+  if (e.g_graph) HIP_OK(hipGraphDestroy(e.g_graph));
+  e.g_exec = nullptr;
+  e.g_graph = nullptr;
+  e.g_key.clear();
+}
+
+// Record trace -> sort -> integrate of the call just made eagerly (so the
+// LOS capacity and the kernel variant are settled) as one hipGraph.
+void capture_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out,
+                     int scattering, int cap, hipStream_t s) {
+  Engine& e = E();
+  drop_forward_graph();
+  e.hist.need((size_t)(cap + 1) * sizeof(int));  // sized before the key is taken
+  TraceBufs tb{&e.geo, &e.geo2, &e.rec, &e.npts, &e.tsurf, nullptr, nullptr, cap};
+  e.capturing = true;
+  HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  launch_zero_u64((unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
+  enqueue_trace(ctl, atm, rays, ctl->sca_n > 0, tb, cap, out.tpz, out.tplon, out.tplat, s);
+  engine_integrate(ctl, rays.nray, cap, e.rec.as<float>(), e.npts.as<int>(), e.tsurf.as<float>(), 0, nullptr,
+                   nullptr, out.rad, out.tau, rays.nray, 0, s);
+  if (out.npts) HIP_OK(hipMemcpyAsync(out.npts, e.npts.p, rays.nray * sizeof(int), hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipStreamEndCapture(s, &e.g_graph));
+  e.capturing = false;
+  HIP_OK(hipGraphInstantiate(&e.g_exec, e.g_graph, nullptr, nullptr, 0));
+  e.g_key = graph_key(ctl, atm, rays, out, scattering, s);
+}
+
+// Replay the captured call; false (graph dropped) when a ray overflowed the
+// captured LOS capacity, and the caller then runs the call eagerly.
+bool replay_forward(const RaySetDev& rays, hipStream_t s) {
+  Engine& e = E();
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  HIP_OK(hipEventRecord(a, s));
+  HIP_OK(hipGraphLaunch(e.g_exec, s));
+  HIP_OK(hipEventRecord(b, s));
+  int st = 0;
+  unsigned long long pts = 0;
+  HIP_OK(hipMemcpyAsync(&st, e.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(&pts, e.pts_acc.p, sizeof pts, hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, a, b));
+  HIP_OK(hipEventDestroy(a));
+  HIP_OK(hipEventDestroy(b));
+  if (st & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
+  if (st & JUR_ST_NLOS) JUR_FAIL("Too many LOS points!");
+  if (st & JUR_ST_CAP) {
+    drop_forward_graph();
+    e.stats["graph_overflows"] += 1;
+    return false;
+  }
+  e.stats["graph_ms"] = ms;
+  e.stats["path_ms"] = NAN;
+  e.stats["trace_ms"] = NAN;
+  e.stats["path_launches"] = 1;
+  e.stats["integrated_points"] = (double)pts;
+  e.stats["pencils"] = rays.nray;
+  return true;
+}
+}  // namespace
+
 void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int scattering,
                     const OutDev& out, hipStream_t s) {
   Engine& e = E();
@@ -458,10 +561,14 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
     end_call(s);
     return;
   }
+  TraceBufs tb{&e.geo, &e.geo2, &e.rec, &e.npts, &e.tsurf, nullptr, nullptr, 0};
+  const bool graph = engine_option_int("graph", 0) != 0;
+  if (graph && e.g_exec && e.g_key == graph_key(ctl, atm, rays, out, scattering, s)) {
+    if (replay_forward(rays, s)) return;
+  }
   hipEvent_t ev[3];
   for (auto& x : ev) HIP_OK(hipEventCreate(&x));
   HIP_OK(hipEventRecord(ev[0], s));
-  TraceBufs tb{&e.geo, &e.geo2, &e.rec, &e.npts, &e.tsurf, nullptr, nullptr, 0};
   int cap = engine_trace(ctl, atm, rays, sca_on, tb, out.tpz, out.tplon, out.tplat, s);
   HIP_OK(hipEventRecord(ev[1], s));
   engine_integrate(ctl, rays.nray, cap, e.rec.as<float>(), e.npts.as<int>(), e.tsurf.as<float>(), 0, nullptr,
@@ -479,6 +586,7 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
   e.stats["scattering_points"] = 0;
   end_call(s);
   for (auto& x : ev) HIP_OK(hipEventDestroy(x));
+  if (graph) capture_forward(ctl, atm, rays, out, scattering, cap, s);
 }
 
 // Sort rays by point count and run the path integration.
@@ -486,13 +594,18 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
                       const float* tsurf, int scattering, const float* J, const int* jbase, float* rad,
                       float* tau, int out_ld, int out_off, hipStream_t s) {
   Engine& e = E();
-  int* ids = (int*)e.ids.need(nray * sizeof(int));
   int* order = (int*)e.order.need(nray * sizeof(int));
-  int* keys = (int*)e.keys.need(nray * sizeof(int));
-  size_t tb = sort_temp_bytes(nray);
-  void* tmp = e.sorttmp.need(tb);
-  launch_iota(ids, nray, s);
-  launch_sort_rays(npts, nullptr, keys, ids, order, tmp, tb, nray, s);
+  if (e.capturing) {
+    // counting sort: kernels only (no library temporaries or memsets)
+    launch_count_sort(npts, nray, cap + 1, (int*)e.hist.need((size_t)(cap + 1) * sizeof(int)), order, s);
+  } else {
+    int* ids = (int*)e.ids.need(nray * sizeof(int));
+    int* keys = (int*)e.keys.need(nray * sizeof(int));
+    size_t tb = sort_temp_bytes(nray);
+    void* tmp = e.sorttmp.need(tb);
+    launch_iota(ids, nray, s);
+    launch_sort_rays(npts, nullptr, keys, ids, order, tmp, tb, nray, s);
+  }
   PathArgs P;
   memset(&P, 0, sizeof P);
   P.nray = nray;
@@ -534,7 +647,7 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     int best = 2 * 8 + 4;  // variant 2, 4 waves per block
     if (it != e.opt.end()) {
       best = (int)it->second;
-    } else if (cls >= 2 && engine_option_int("autotune", 1)) {
+    } else if (cls >= 2 && engine_option_int("autotune", 1) && !e.capturing) {
       hipEvent_t a, b;
       HIP_OK(hipEventCreate(&a));
       HIP_OK(hipEventCreate(&b));
@@ -565,6 +678,11 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     if (wpb < 0) wpb = best % 8;
   }
   P.cpw = variant;
+  launch_sum_points(npts, nray, (unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
+  if (e.capturing) {
+    launch_path(P, wpb, s);
+    return;
+  }
   if (e.path_ev.size() <= e.path_ev_used) {
     hipEvent_t a, b;
     HIP_OK(hipEventCreate(&a));
@@ -572,7 +690,6 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     e.path_ev.push_back({a, b});
   }
   auto& ev = e.path_ev[e.path_ev_used++];
-  launch_sum_points(npts, nray, (unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
   HIP_OK(hipEventRecord(ev.first, s));
   launch_path(P, wpb, s);
   HIP_OK(hipEventRecord(ev.second, s));

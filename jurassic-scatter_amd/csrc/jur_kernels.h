@@ -75,5 +75,7 @@ void launch_sort_rays(const int* np, int* keys_tmp, int* keys_out, int* ids_in, 
                       void* temp, size_t temp_bytes, int n, hipStream_t s);
 void launch_iota(int* ids, int n, hipStream_t s);
 void launch_zero_int(int* p, int n, hipStream_t s);
+// Rays by descending point count into order (hist: nbins ints of scratch).
+void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, hipStream_t s);
 void launch_sum_points(const int* np, int n, unsigned long long* acc, hipStream_t s);
 void launch_zero_u64(unsigned long long* p, hipStream_t s);

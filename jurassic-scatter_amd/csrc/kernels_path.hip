@@ -586,15 +586,13 @@ void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
     case 2: launch_t<NGT, 2, 4>(A, wpb, s); break;
     case 3: launch_t<NGT, 4, 2>(A, wpb, s); break;
     case 4: launch_t<NGT, 4, 4>(A, wpb, s); break;
-    case 5: launch_t<NGT, 2, 3>(A, wpb, s); break;
-    case 6: launch_t<NGT, 1, 6>(A, wpb, s); break;
-    default: launch_t<NGT, 4, 3>(A, wpb, s); break;
+    default: launch_t<NGT, 2, 3>(A, wpb, s); break;
   }
 }
 
 }  // namespace
 
-int path_variants() { return 8; }
+int path_variants() { return 6; }
 
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;
@@ -648,6 +646,54 @@ void launch_sum_points(const int* np, int n, unsigned long long* acc, hipStream_
 void launch_zero_u64(unsigned long long* p, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, p);
 }
+// Counting sort of rays by descending LOS point count (keys 0..nbins-1), for
+// captured graphs: three kernels, no library temporaries or memsets.  The
+// order within one count is arbitrary (rays are independent).
+__global__ void k_count_hist(const int* np, int n, int nbins, int* hist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&hist[min(max(np[i], 0), nbins - 1)], 1);
+}
+// In place: hist[b] <- number of rays with a larger count (one block).
+__global__ void __launch_bounds__(1024) k_hist_offsets(int* hist, int nbins) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (nbins + 1023) / 1024;
+  // thread t owns bins hi..lo in descending order: b = nbins - 1 - (t per + j)
+  int sum = 0;
+  for (int j = 0; j < per; j++) {
+    const int b = nbins - 1 - (t * per + j);
+    if (b >= 0) sum += hist[b];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - sum;  // exclusive prefix of thread t
+  for (int j = 0; j < per; j++) {
+    const int b = nbins - 1 - (t * per + j);
+    if (b >= 0) {
+      const int c = hist[b];
+      hist[b] = run;
+      run += c;
+    }
+  }
+}
+__global__ void k_scatter_order(const int* np, int n, int nbins, int* cursor, int* order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) order[atomicAdd(&cursor[min(max(np[i], 0), nbins - 1)], 1)] = i;
+}
+
+void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_int, dim3((nbins + 255) / 256), dim3(256), 0, s, hist, nbins);
+  hipLaunchKernelGGL(k_count_hist, dim3((n + 255) / 256), dim3(256), 0, s, np, n, nbins, hist);
+  hipLaunchKernelGGL(k_hist_offsets, dim3(1), dim3(1024), 0, s, hist, nbins);
+  hipLaunchKernelGGL(k_scatter_order, dim3((n + 255) / 256), dim3(256), 0, s, np, n, nbins, hist, order);
+}
+
 void launch_zero_int(int* p, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
 }
