@@ -194,13 +194,15 @@ def _rays_nadir(zen_deg, obsz=800.0):
     return g
 
 
-def make(name, outdir, nray=None, seed=None):
+def make(name, outdir, nray=None, seed=None, nu=None):
     """Generate workload `name` into outdir (tables, filters, aerosol files).
-    nray overrides the ray count (the same seeded sequence, truncated)."""
+    nray overrides the ray count (the same seeded sequence, truncated); nu
+    replaces the channel wavenumbers (tables are generated for them)."""
     cfg = WORKLOADS[name]
     outdir = Path(outdir)
     outdir.mkdir(parents=True, exist_ok=True)
     rng = np.random.default_rng(SEED0 + cfg if seed is None else seed)
+    nu_arg = nu
     if cfg == 0:
         emitters = ["CO2", "H2O"]
         nu = np.linspace(792.0, 832.0, 4)
@@ -213,6 +215,8 @@ def make(name, outdir, nray=None, seed=None):
         emitters = ["CO2", "H2O", "O3", "HNO3", "F11"]
         nu = np.linspace(780.0, 1250.0, 32)
         z = np.arange(0.0, 121.0)
+    if nu_arg is not None:
+        nu = np.asarray(nu_arg, dtype=float)
     tblbase = str(outdir / "syn")
     # the held-out cirrus set shares the training set's emissivity tables
     trng = np.random.default_rng(SEED0 + 100 + (3 if cfg == 4 else cfg))
