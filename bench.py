@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--variant", type=int, default=None,
                     help="path-kernel variant (default: autotuned on the first call)")
     ap.add_argument("--wpb", type=int, default=None, help="waves per block (default: autotuned)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="1: replay the clear-air call from a captured hipGraph, 0: eager calls (default)")
     ap.add_argument("--weights", default=None,
                     help="trained surrogate for --workload cirrus_16384 (default: the packaged one)")
     ap.add_argument("--plain-rays", type=int, default=64,
@@ -273,18 +275,31 @@ def main():
         lib.jur_formod_batch_device(C.byref(ctl), C.byref(b), aero_p, scattering, None)
         lib.jur_sync()
 
-    for _ in range(args.warmup):
+    # kernel times from eager calls (a replayed graph has no per-kernel events)
+    for _ in range(max(args.warmup, 1)):
         step()
     torch.cuda.synchronize()
     path_ms, trace_ms, ipts = [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(2 if args.graph else 0):
         step()
         path_ms.append(lib.jur_stat(b"path_ms"))
         trace_ms.append(lib.jur_stat(b"trace_ms"))
         ipts.append(lib.jur_stat(b"integrated_points"))
+    graph = bool(args.graph) and not scattering
+    if graph:
+        lib.jur_set_option(b"graph", 1)
+        step()  # eager call + capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        if not graph:
+            path_ms.append(lib.jur_stat(b"path_ms"))
+            trace_ms.append(lib.jur_stat(b"trace_ms"))
+        ipts.append(lib.jur_stat(b"integrated_points"))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    lib.jur_set_option(b"graph", 0)
     ms_per_step = 1e3 * dt / args.steps
     value = nray * nd * args.steps / dt
 
@@ -320,7 +335,8 @@ def main():
         "data": "synthetic (seeded generator, reference file formats)",
         "config": {"workload": args.workload, "rays": nray, "channels": nd, "emitters": ng,
                    "levels": w.nlev, "los_points": pts, "scattering_depth": scattering,
-                   "pencils": pencils, "integrated_points": ipts_call, "parallelism": "single GPU"},
+                   "pencils": pencils, "integrated_points": ipts_call, "hip_graph": graph,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_path_fast", "kernel_ms": path_avg,

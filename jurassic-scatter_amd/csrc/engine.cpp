@@ -44,6 +44,9 @@ struct Engine {
   // address and size the graph was recorded with
   bool capturing = false;
   DBuf hist;
+  // inputs of the last engine_set_aero
+  bool aero_ready = false;
+  std::vector<char> aero_snap;
   std::string g_key;
   hipGraph_t g_graph = nullptr;
   hipGraphExec_t g_exec = nullptr;
@@ -306,6 +309,33 @@ void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
   Engine& e = E();
   hipStream_t s = e.stream;
   const int nd = ctl->nd;
+  // Layer optics and phase functions are uploaded only when they change:
+  // the snapshot holds every input this function reads.
+  std::vector<char> snap;
+  auto add = [&snap](const void* p, size_t n) {
+    snap.insert(snap.end(), (const char*)p, (const char*)p + n);
+  };
+  const int on = (aero && ctl->sca_n > 0) ? 1 : 0;
+  add(&on, sizeof on);
+  add(&nd, sizeof nd);
+  add(ctl->nu, sizeof(double) * nd);
+  if (on) {
+    const int nl = aero->nl < NLMAX ? aero->nl : NLMAX;
+    add(&nl, sizeof nl);
+    add(&ctl->transs, sizeof ctl->transs);
+    add(aero->top, sizeof(double) * nl);
+    add(aero->bottom, sizeof(double) * nl);
+    add(aero->trans, sizeof(double) * nl);
+    for (int il = 0; il < nl; il++) {
+      add(aero->beta_e[il], sizeof(double) * nd);
+      add(aero->beta_s[il], sizeof(double) * nd);
+      add(aero->beta_a[il], sizeof(double) * nd);
+      for (int id = 0; id < nd; id++) add(aero->p[il][id], sizeof(double) * NTHETA);
+    }
+  }
+  if (e.aero_ready && snap == e.aero_snap) return;
+  e.aero_snap.swap(snap);
+  e.aero_ready = true;
   std::vector<float> be((size_t)NLMAX * nd, 0.f), bs((size_t)NLMAX * nd, 0.f), ba((size_t)NLMAX * nd, 0.f);
   std::vector<double> alti;
   e.nl = 0;
@@ -711,6 +741,7 @@ extern "C" int jur_init(int device) {
     HIP_OK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
     e.tbl_key.clear();
     e.chan_key.clear();
+    e.aero_ready = false;
   }
   return 0;
 }

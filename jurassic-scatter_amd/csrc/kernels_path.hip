@@ -371,14 +371,23 @@ __device__ inline int near_col(const float2* c, int n, float x, int h, float2& l
 }
 
 // Corner lookup (inverse then forward) starting from bracket hint h; returns
-// the inverse bracket in h.
+// the inverse bracket in h.  The forward search starts in the inverse
+// bracket with its two nodes already loaded: u + du usually stays there
+// (du >= 0), and then no further load is on the dependent chain.
 __device__ inline EC corner_h(const float2* col, int n, float ep, float du, int& h) {
   float2 a, b, c, d;
   const int i = near_col<true>(col, n, ep, h, a, b);
   h = i;
   const float u = lin(a.y, a.x, b.y, b.x, ep);
   const float ux = u + du;
-  near_col<false>(col, n, ux, i, c, d);
+  // same bracket as near_col<false> would return from hint i: x in [lo, hi)
+  // (or beyond an end bracket), NaN excluded
+  if (ux == ux && (i == 0 || a.x <= ux) && (i == n - 2 || ux < b.x)) {
+    c = a;
+    d = b;
+  } else {
+    near_col<false>(col, n, ux, i, c, d);
+  }
   const float s = (d.y - c.y) / (d.x - c.x), dx = ux - c.x;
   return clamp_ec(c.y + s * dx, (1.f - c.y) - s * dx);
 }
