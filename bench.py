@@ -30,7 +30,7 @@ sys.path.insert(0, str(ROOT / "jurassic-scatter_amd"))
 HBM_PEAK_GBS = 8000.0
 
 # Rays of the CPU oracle sample per workload (about 10-30 s on 16 cores).
-CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 4096, "aerosol_limb_8192": 192, "cirrus_4096": 2}
+CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 32768, "aerosol_limb_8192": 192, "cirrus_4096": 2}
 
 # HBM bytes per launch of the path kernel measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a plain copy
@@ -38,7 +38,7 @@ CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 4096, "aerosol_limb_8192
 # waves per block).  The profile summary named beside each figure holds the
 # commands and the raw counter values.
 MEASURED_TRAFFIC = {
-    ("clear_batch_65536", 65536, 2, 4): (1.7182e11, "profiles/k_path_fast_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 6, 2): (2.2868e11, "profiles/k_path_fast_clear_batch_65536.md"),
 }
 
 

@@ -119,7 +119,7 @@ __device__ inline void kahan_add(float& sum, float& comp, float term) {
 // (forwardmodel.c:120-152; continua.c ctmco2 :854-869, ctmh2o :1898-1928,
 // ctmn2 :1980-1992, ctmo2 :2037-2048).
 __device__ inline float continua(const ChanConst& c, float p, float t, float ds, float qh2o,
-                                 float uco2, float uh2o, float kw) {
+                                 float uco2, float uh2o, float kw, float rt, float rds) {
   float beta = kw;
   // A zero-length segment (an aerosol edge point inserted onto the
   // observer; u = 0) adds no CO2/H2O continuum: the reference's u/ds is
@@ -128,19 +128,19 @@ __device__ inline float continua(const ChanConst& c, float p, float t, float ds,
   if (c.co2_on && seg) {
     float d230 = t - 230.f, d260 = t - 260.f, d296 = t - 296.f;
     float ctw = d260 * d296 * c.co2_k230 - d230 * d296 * c.co2_k260 + d230 * d260 * c.co2_k296;
-    beta += uco2 * (float)(1.0 / (JUR_AVOGADRO * 1000.0 * P0)) * p * ctw / ds;
+    beta += uco2 * (float)(1.0 / (JUR_AVOGADRO * 1000.0 * P0)) * p * ctw * rds;
   }
   if (c.h2o_on && seg) {
-    float ctwslf = c.h2o_s296 * expf(c.h2o_lnratio * ((296.f - t) / 36.f));
-    float a1 = c.nu * uh2o * tanhf(c.h2o_tanhk / t);
-    float a2 = 296.f / t;
+    float ctwslf = c.h2o_s296 * expf(c.h2o_lnratio * ((296.f - t) * (1.f / 36.f)));
+    float a1 = c.nu * uh2o * tanhf(c.h2o_tanhk * rt);
+    float a2 = 296.f * rt;
     float a3 = p * (float)(1.0 / P0) * (qh2o * ctwslf + (1.f - qh2o) * c.h2o_frn) * 1e-20f;
-    beta += a1 * a2 * a3 / ds;
+    beta += a1 * a2 * a3 * rds;
   }
   if (c.n2_on || c.o2_on) {
-    float pp = p * (float)(1.0 / P0), tt = 273.f / t;
+    float pp = p * (float)(1.0 / P0), tt = 273.f * rt;
     float base = 0.1f * pp * pp * tt * tt;
-    float ex = (1.f / 296.f - 1.f / t);
+    float ex = (1.f / 296.f - rt);
     if (c.n2_on)
       beta += base * 0.79f * c.n2_b * expf(c.n2_beta * ex) *
               (0.79f + 0.21f * (1.294f - 0.4545f * t / 296.f));
@@ -190,12 +190,12 @@ __device__ inline bool map_work(const PathArgs& A, int cpw, int& chunk, int& gro
 // and its complement et = 1 - tau_gas.
 __device__ inline void point_radiance(const PathArgs& A, const ChanConst& cc, int id, const float* pr, int uo,
                                       float p, float t, float ds, float qh, float af, int ai, bool scpt, int kj,
-                                      int itp, float dtp, float tg, float et, float& radv, float& comp,
-                                      float& tau) {
+                                      int itp, float dtp, float tg, float et, float rt, float rds, float& radv,
+                                      float& comp, float& tau) {
   const float uco2 = cc.co2_on ? pr[uo + cc.co2_ig] : 0.f;
   const float uh2o = cc.h2o_on ? pr[uo + cc.h2o_ig] : 0.f;
   const float kw = A.nw > 0 ? pr[REC_K + cc.window] : 0.f;
-  const float beta = continua(cc, p, t, ds, qh, uco2, uh2o, kw);
+  const float beta = continua(cc, p, t, ds, qh, uco2, uh2o, kw, rt, rds);
   const float B = planck_at(A.planck + (size_t)id * JUR_NPLANCK, itp, dtp);
   if (!(tg > 0.f)) return;
   if (scpt) {
@@ -300,7 +300,8 @@ __global__ void __launch_bounds__(256) k_path(PathArgs A) {
               tg *= f;
             }
           }
-          point_radiance(A, cc, id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg, et, radv[c], comp[c],
+          point_radiance(A, cc, id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg, et, 1.f / t, 1.f / ds,
+                         radv[c], comp[c],
                          tau[c]);
         }
       }
@@ -370,6 +371,15 @@ __device__ inline int near_col(const float2* c, int n, float x, int h, float2& l
   return h;
 }
 
+// Quotients of the fast kernel as a multiply by the hardware reciprocal
+// (v_rcp_f32, 1 ulp): a full IEEE division costs about ten instructions and
+// the kernel issues nine per (point, channel, emitter).  The resulting
+// radiance differences stay orders of magnitude below the 1e-4 budget.
+__device__ inline float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline float lin_r(float x0, float y0, float x1, float y1, float x) {
+  return y0 + fdiv(y1 - y0, x1 - x0) * (x - x0);
+}
+
 // Corner lookup (inverse then forward) starting from bracket hint h; returns
 // the inverse bracket in h.  The forward search starts in the inverse
 // bracket with its two nodes already loaded: u + du usually stays there
@@ -378,7 +388,7 @@ __device__ inline EC corner_h(const float2* col, int n, float ep, float du, int&
   float2 a, b, c, d;
   const int i = near_col<true>(col, n, ep, h, a, b);
   h = i;
-  const float u = lin(a.y, a.x, b.y, b.x, ep);
+  const float u = lin_r(a.y, a.x, b.y, b.x, ep);
   const float ux = u + du;
   // same bracket as near_col<false> would return from hint i: x in [lo, hi)
   // (or beyond an end bracket), NaN excluded
@@ -388,7 +398,7 @@ __device__ inline EC corner_h(const float2* col, int n, float ep, float du, int&
   } else {
     near_col<false>(col, n, ux, i, c, d);
   }
-  const float s = (d.y - c.y) / (d.x - c.x), dx = ux - c.x;
+  const float s = fdiv(d.y - c.y, d.x - c.x), dx = ux - c.x;
   return clamp_ec(c.y + s * dx, (1.f - c.y) - s * dx);
 }
 
@@ -445,6 +455,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
       planck_index(t, itp, dtp);
       const bool scpt = A.scattering > 0 && af > 0.f;
       const int ai = scpt || (A.sca_n > 0 && af != 0.f) ? (int)pr[REC_AI] : 0;
+      const float rtp = __builtin_amdgcn_rcpf(t), rdsp = __builtin_amdgcn_rcpf(ds);
       // Emitter-outer loop: the (p, T) brackets and weights of an emitter are
       // shared by all channels of the wave; only the per-channel segment
       // transmittance products (tg, et) stay live across emitters.
@@ -469,9 +480,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
         const int it0 = near_grid(t0g, nt0, t, (hpt[g] >> 8) & 0xff);
         const int it1 = near_grid(t1g, nt1, t, (hpt[g] >> 16) & 0xff);
         hpt[g] = ipr | (it0 << 8) | (it1 << 16);
-        const float wt0 = (t - t0g[it0]) / (t0g[it0 + 1] - t0g[it0]);
-        const float wt1 = (t - t1g[it1]) / (t1g[it1 + 1] - t1g[it1]);
-        const float wp = (p - pg[ipr]) / (pg[ipr + 1] - pg[ipr]);
+        const float wt0 = fdiv(t - t0g[it0], t0g[it0 + 1] - t0g[it0]);
+        const float wt1 = fdiv(t - t1g[it1], t1g[it1 + 1] - t1g[it1]);
+        const float wp = fdiv(p - pg[ipr], pg[ipr + 1] - pg[ipr]);
         const int gb0 = ipr * TBLNTMAX + it0, gb1 = (ipr + 1) * TBLNTMAX + it1;
 #pragma unroll
         for (int c = 0; c < CPW; c++) {
@@ -502,11 +513,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
               e00 = mix_ec(e00, e01, wt0);
               e11 = mix_ec(e10, e11, wt1);
               e00 = mix_ec(e00, e11, wp);
+              const float rt = __builtin_amdgcn_rcpf(tp[c][g]);
               if (e00.e < 0.5f) {
-                s = (e00.e - e) / tp[c][g];
+                s = (e00.e - e) * rt;
                 f = 1.f - s;
               } else {
-                f = e00.c / tp[c][g];
+                f = e00.c * rt;
                 s = 1.f - f;
               }
               ep[c][g] = e00.e;
@@ -521,7 +533,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
       for (int c = 0; c < CPW; c++) {
         const int id = c0 + c;
         if (id < A.nd)
-          point_radiance(A, A.cc[id], id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg[c], et[c], radv[c],
+          point_radiance(A, A.cc[id], id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg[c], et[c], rtp, rdsp,
+                         radv[c],
                          comp[c], tau[c]);
       }
       if (scpt) kj++;
@@ -595,13 +608,14 @@ void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
     case 2: launch_t<NGT, 2, 4>(A, wpb, s); break;
     case 3: launch_t<NGT, 4, 2>(A, wpb, s); break;
     case 4: launch_t<NGT, 4, 4>(A, wpb, s); break;
-    default: launch_t<NGT, 2, 3>(A, wpb, s); break;
+    case 5: launch_t<NGT, 2, 3>(A, wpb, s); break;
+    default: launch_t<NGT, 2, 5>(A, wpb, s); break;
   }
 }
 
 }  // namespace
 
-int path_variants() { return 6; }
+int path_variants() { return 7; }
 
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;
