@@ -35,10 +35,10 @@ struct EC {
 
 // GSL_MAX(GSL_MIN(eps, 1), 0) on the pair; NaN becomes 1 like the reference.
 __device__ inline EC clamp_ec(float e, float c) {
-  if (e != e) return {1.f, 0.f};
-  if (e <= 0.f) return {0.f, 1.f};
-  if (c <= 0.f) return {1.f, 0.f};
-  return {e < 1.f ? e : 1.f, c};
+  const bool nan = !(e == e), lo = e <= 0.f, hi = c <= 0.f;
+  const float re = nan ? 1.f : (lo ? 0.f : (hi ? 1.f : (e < 1.f ? e : 1.f)));
+  const float rc = nan ? 0.f : (lo ? 1.f : (hi ? 0.f : c));
+  return {re, rc};
 }
 
 // LIN between two (eps, 1 - eps) nodes.
@@ -321,11 +321,12 @@ __global__ void __launch_bounds__(256) k_path(PathArgs A) {
 // a few steps falls back to the bisection.
 // ---------------------------------------------------------------------------
 __device__ inline int near_grid(const float* xx, int n, float x, int h) {
-  if (!(x == x)) return n - 2;
   h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
   const float a = xx[h], b = xx[h + 1];
   const bool okl = (h == 0) || (a <= x), okh = (h == n - 2) || (x < b);
+  // a NaN passes only when n == 2, where its bracket (n - 2) is h anyway
   if (okl && okh) return h;
+  if (!(x == x)) return n - 2;
   if (okl) {
     const int g = h + 1;
     if (g == n - 2 || x < xx[g + 1]) return g;
@@ -341,17 +342,18 @@ __device__ inline float keyof(float2 v) { return KEY_EPS ? v.y : v.x; }
 
 template <bool KEY_EPS>
 __device__ inline int near_col(const float2* c, int n, float x, int h, float2& lo, float2& hi) {
-  if (!(x == x)) {
-    lo = c[n - 2];
-    hi = c[n - 1];
-    return n - 2;
-  }
   h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
   lo = c[h];
   hi = c[h + 1];
   const bool okl = (h == 0) || (keyof<KEY_EPS>(lo) <= x);
   const bool okh = (h == n - 2) || (x < keyof<KEY_EPS>(hi));
+  // a NaN passes only when n == 2, where its bracket (n - 2) is h anyway
   if (okl && okh) return h;
+  if (!(x == x)) {
+    lo = c[n - 2];
+    hi = c[n - 1];
+    return n - 2;
+  }
   if (okl) {
     for (int s = 0; s < 3; s++) {
       h++;
@@ -513,14 +515,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
               e00 = mix_ec(e00, e01, wt0);
               e11 = mix_ec(e10, e11, wt1);
               e00 = mix_ec(e00, e11, wp);
+              // transparent: segment emissivity first; opaque: transmittance
               const float rt = __builtin_amdgcn_rcpf(tp[c][g]);
-              if (e00.e < 0.5f) {
-                s = (e00.e - e) * rt;
-                f = 1.f - s;
-              } else {
-                f = e00.c * rt;
-                s = 1.f - f;
-              }
+              const bool tr = e00.e < 0.5f;
+              const float s1 = (e00.e - e) * rt, f1 = e00.c * rt;
+              s = tr ? s1 : 1.f - f1;
+              f = tr ? 1.f - s1 : f1;
               ep[c][g] = e00.e;
               tp[c][g] = e00.c;
             }
