@@ -25,6 +25,7 @@ struct Engine {
   DBuf t_np, t_p, t_nt, t_t, t_ncol, t_col, t_ue, planck, chan, nu_d;
   DBuf g_np, g_p, g_nt, g_t, t_meta;
   int shared = 0;
+  unsigned ue_bytes = 0, meta_bytes = 0;
   // aerosol layers
   DBuf beta_e, beta_s, beta_a, alti;
   int nalti = 0;
@@ -245,6 +246,7 @@ void engine_prepare(const ctl_t* ctl) {
     upload(e.g_nt, gnt, s);
     upload(e.g_t, gt, s);
     upload(e.t_meta, meta, s);
+    e.meta_bytes = (unsigned)(meta.size() * sizeof(int2));
     upload(e.t_np, np, s);
     upload(e.t_p, p, s);
     upload(e.t_nt, nt, s);
@@ -252,6 +254,7 @@ void engine_prepare(const ctl_t* ctl) {
     upload(e.t_ncol, ncol, s);
     upload(e.t_col, col, s);
     upload(e.t_ue, ue, s);
+    e.ue_bytes = (unsigned)(ue.size() * sizeof(float2));
     // Filter functions -> Planck table on the device.
     std::vector<double> fnu, fw;
     std::vector<int> foff, fnn;
@@ -768,12 +771,14 @@ DevTables engine_tables() {
   T.ncol = e.t_ncol.as<int>();
   T.col = e.t_col.as<int>();
   T.ue = e.t_ue.as<float2>();
+  T.ue_bytes = e.ue_bytes;
   T.shared = e.shared;
   T.gnp = e.g_np.as<int>();
   T.gp = e.g_p.as<float>();
   T.gnt = e.g_nt.as<int>();
   T.gt = e.g_t.as<float>();
   T.meta = e.t_meta.as<int2>();
+  T.meta_bytes = e.meta_bytes;
   return T;
 }
 const double* engine_nu_dev() { return E().nu_d.as<double>(); }

@@ -27,6 +27,7 @@ struct DevTables {
   const int* ncol;     // [ntbl][TBLNPMAX][TBLNTMAX]
   const int* col;      // [ntbl][TBLNPMAX][TBLNTMAX]
   const float2* ue;    // (u, eps) pairs
+  unsigned ue_bytes;   // size of ue in bytes
   // When every channel of an emitter has the same increasing (p, T) grids
   // and every column is strictly increasing (shared != 0): the grids once
   // per emitter, and (ncol, col) packed per column.
@@ -36,6 +37,7 @@ struct DevTables {
   const int* gnt;      // [ng][TBLNPMAX]
   const float* gt;     // [ng][TBLNPMAX][TBLNTMAX]
   const int2* meta;    // [ntbl][TBLNPMAX][TBLNTMAX] = (ncol, col)
+  unsigned meta_bytes;
 };
 
 // Per-channel constants of the continua and the channel's window (filled on
@@ -97,8 +99,8 @@ __host__ __device__ inline int locate_mono(const T* xx, int n, T x) {
 
 // Bisection over an increasing float2 column, key .x or .y
 // (locate_tbl, forwardmodel.c:546-565).
-template <bool KEY_EPS>
-__device__ inline int locate_col(const float2* c, int n, float x) {
+template <bool KEY_EPS, class C>
+__device__ inline int locate_col(const C& c, int n, float x) {
   int ilo = 0, ihi = n - 1;
   while (ihi > ilo + 1) {
     int i = (ihi + ilo) >> 1;

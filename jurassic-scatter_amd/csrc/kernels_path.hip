@@ -340,8 +340,27 @@ __device__ inline int near_grid(const float* xx, int n, float x, int h) {
 template <bool KEY_EPS>
 __device__ inline float keyof(float2 v) { return KEY_EPS ? v.y : v.x; }
 
-template <bool KEY_EPS>
-__device__ inline int near_col(const float2* c, int n, float x, int h, float2& lo, float2& hi) {
+// A table column read through a buffer resource: one wave-uniform
+// descriptor for the whole (u, eps) array and 32-bit per-lane offsets
+// (buffer_load_dwordx2 offen), so no 64-bit address arithmetic per gather;
+// the hardware range check returns zeros beyond the array.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+struct ColB {
+  __amdgpu_buffer_rsrc_t r;
+  int base;  // element offset of the column in the array
+  __device__ float2 operator[](int i) const {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (base + i) * 8, 0, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+  }
+};
+
+__device__ inline int2 ldmeta(__amdgpu_buffer_rsrc_t r, int i) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0);
+  return make_int2((int)v.x, (int)v.y);
+}
+
+template <bool KEY_EPS, class C>
+__device__ inline int near_col(const C& c, int n, float x, int h, float2& lo, float2& hi) {
   h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
   lo = c[h];
   hi = c[h + 1];
@@ -386,7 +405,7 @@ __device__ inline float lin_r(float x0, float y0, float x1, float y1, float x) {
 // the inverse bracket in h.  The forward search starts in the inverse
 // bracket with its two nodes already loaded: u + du usually stays there
 // (du >= 0), and then no further load is on the dependent chain.
-__device__ inline EC corner_h(const float2* col, int n, float ep, float du, int& h) {
+__device__ inline EC corner_h(const ColB& col, int n, float ep, float du, int& h) {
   float2 a, b, c, d;
   const int i = near_col<true>(col, n, ep, h, a, b);
   h = i;
@@ -420,6 +439,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
   const int c0 = chunk * CPW;
   const int uo = rec_u(A.nw);
   const DevTables& T = A.T;
+  const __amdgpu_buffer_rsrc_t ur = __builtin_amdgcn_make_buffer_rsrc((void*)T.ue, (short)0, (int)T.ue_bytes,
+                                                                      0x00020000);
+  const __amdgpu_buffer_rsrc_t mr = __builtin_amdgcn_make_buffer_rsrc((void*)T.meta, (short)0,
+                                                                      (int)T.meta_bytes, 0x00020000);
 
   float ep[CPW][NGT], tp[CPW][NGT], radv[CPW], comp[CPW], tau[CPW];
   // column-search hints of the previous point live in LDS (one slot per lane
@@ -497,21 +520,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
             ep[c][g] = 1.f;
             tp[c][g] = 0.f;
           } else {
-            const size_t tb = (size_t)(id * A.ng + g) * TBLNPMAX * TBLNTMAX;
-            const int2 m00 = T.meta[tb + gb0], m01 = T.meta[tb + gb0 + 1];
-            const int2 m10 = T.meta[tb + gb1], m11 = T.meta[tb + gb1 + 1];
+            const int tb = (id * A.ng + g) * TBLNPMAX * TBLNTMAX;
+            const int2 m00 = ldmeta(mr, tb + gb0), m01 = ldmeta(mr, tb + gb0 + 1);
+            const int2 m10 = ldmeta(mr, tb + gb1), m11 = ldmeta(mr, tb + gb1 + 1);
             if (m00.x >= 2 && m01.x >= 2 && m10.x >= 2 && m11.x >= 2) {
               const float e = ep[c][g];
               const int h0 = hint[c * NGT + g][lane];
               int h = h0;
-              EC e00 = corner_h(T.ue + m00.y, m00.x, e, du, h);
+              EC e00 = corner_h(ColB{ur, m00.y}, m00.x, e, du, h);
               hint[c * NGT + g][lane] = h;
               h = h0;
-              EC e01 = corner_h(T.ue + m01.y, m01.x, e, du, h);
+              EC e01 = corner_h(ColB{ur, m01.y}, m01.x, e, du, h);
               h = h0;
-              EC e10 = corner_h(T.ue + m10.y, m10.x, e, du, h);
+              EC e10 = corner_h(ColB{ur, m10.y}, m10.x, e, du, h);
               h = h0;
-              EC e11 = corner_h(T.ue + m11.y, m11.x, e, du, h);
+              EC e11 = corner_h(ColB{ur, m11.y}, m11.x, e, du, h);
               e00 = mix_ec(e00, e01, wt0);
               e11 = mix_ec(e10, e11, wt1);
               e00 = mix_ec(e00, e11, wp);
