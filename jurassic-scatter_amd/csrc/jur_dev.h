@@ -130,6 +130,8 @@ __host__ __device__ inline void cart2geo(const double* x, double* z, double* lon
   *lon = atan2(x[1], x[0]) * 180 / M_PI;
   *z = radius - RE;
 }
+// The altitude part of cart2geo alone.
+__host__ __device__ inline double cart2z(const double* x) { return norm3(x) - RE; }
 __host__ __device__ inline double dist3(const double* a, const double* b) {
   double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
   return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
@@ -158,6 +160,24 @@ struct DevAtm {
 struct AtmPoint {
   double p, t;
 };
+// locate on an increasing altitude grid starting from bracket h (the
+// previous evaluation of the same line of sight): the bracket
+// z[h] <= z0 < z[h + 1] is unique, so this returns locate_mono's index;
+// a miss after one step up or down falls back to the bisection.
+__device__ inline int atm_locate_h(const DevAtm& A, int a, double z0, int h) {
+  const double* z = A.z + (size_t)a * A.nlev;
+  const int n = A.nlev;
+  if (!(z[0] < z[1]) || !(z0 == z0)) return locate_mono<double>(z, n, z0);
+  h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
+  const bool okl = (h == 0) || (z[h] <= z0), okh = (h == n - 2) || (z0 < z[h + 1]);
+  if (okl && okh) return h;
+  if (okl) {
+    if (h + 1 == n - 2 || z0 < z[h + 2]) return h + 1;
+  } else {
+    if (h - 1 == 0 || z[h - 1] <= z0) return h - 1;
+  }
+  return locate_mono<double>(z, n, z0);
+}
 __device__ inline int atm_locate(const DevAtm& A, int a, double z0) {
   return locate_mono<double>(A.z + (size_t)a * A.nlev, A.nlev, z0);
 }

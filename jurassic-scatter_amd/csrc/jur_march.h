@@ -80,6 +80,7 @@ __device__ MarchOut march_ray(const DevAtm& M, int a, double obsz, double obslon
   }
   // (pl, tl): last atmosphere evaluation, as the reference's locals p, t.
   double pl = 0, tl = 0;
+  int ih = 0;  // altitude bracket of the previous evaluation (search hint)
   G4 prev = {0, 0, 0, 0};
   int stop = 0, np = 0;
   while (1) {
@@ -99,7 +100,8 @@ __device__ MarchOut march_ray(const DevAtm& M, int a, double obsz, double obslon
       if (np > 0) cur = chord_point(prev, cur, (stop == 2 ? zmin : zmax));
       cur.ds = 0.;
     } else {
-      AtmPoint ap = atm_pt(M, a, z, atm_locate(M, a, z));
+      ih = atm_locate_h(M, a, z, ih);
+      AtmPoint ap = atm_pt(M, a, z, ih);
       pl = ap.p;
       tl = ap.t;
     }
@@ -120,15 +122,16 @@ __device__ MarchOut march_ray(const DevAtm& M, int a, double obsz, double obslon
     double n = (refrac && z <= 60) ? 1 + refractivity({pl, tl}) : 1;
     for (int i = 0; i < 3; i++) ex1[i] = ex0[i] * n;
     if (refrac && z <= 60) {
+      // the refractivity gradient needs only the altitudes of the points
       for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
-      double zz, lo, la;
-      cart2geo(xh, &zz, &lo, &la);
-      AtmPoint ap = atm_pt(M, a, zz, atm_locate(M, a, zz));
+      double zz = cart2z(xh);
+      int jh = atm_locate_h(M, a, zz, ih);
+      AtmPoint ap = atm_pt(M, a, zz, jh);
       double n0 = refractivity(ap);
       for (int i = 0; i < 3; i++) {
         xh[i] += 0.02;
-        cart2geo(xh, &zz, &lo, &la);
-        ap = atm_pt(M, a, zz, atm_locate(M, a, zz));
+        zz = cart2z(xh);
+        ap = atm_pt(M, a, zz, atm_locate_h(M, a, zz, jh));
         ng[i] = (refractivity(ap) - n0) / 0.02;
         xh[i] -= 0.02;
       }

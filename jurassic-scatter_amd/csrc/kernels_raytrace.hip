@@ -92,10 +92,11 @@ __global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
     }
   }
 
-  // Aerosol-layer point insertion (lineofsight.c:193-372) into geo2.
+  // Aerosol-layer point insertion (lineofsight.c:193-372) into geo2; the
+  // segment lengths, layer membership and point records of the final
+  // points follow in k_records (one thread per point).
   const double4* F = G;
   int nf = np;
-  float* rec = A.rec + (size_t)r * A.cap * A.R;
   if (A.sca_n > 0 && np > 0 && !(status & fatal)) {
     double4* G2 = A.geo2 + (size_t)r * A.cap;
     const int jl = A.nalti;
@@ -111,78 +112,12 @@ __global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
       G2[ins.n++] = G[ip];
     }
     status |= ins.status;
-    const int n2 = ins.n;
     if (!(status & fatal)) {
-      // Segment lengths from Cartesian half-chords (lineofsight.c:310-324).
-      double xa[3], xb[3], xc[3];
-      if (n2 >= 2) {
-        G4 g0 = load_g(G2, 0), g1 = load_g(G2, 1);
-        geo2cart(g0.z, g0.lon, g0.lat, xa);
-        geo2cart(g1.z, g1.lon, g1.lat, xb);
-        G2[0].w = 0.5 * dist3(xa, xb);
-        for (int ip = 1; ip < n2 - 1; ip++) {
-          G4 p0 = load_g(G2, ip - 1), p1 = load_g(G2, ip), p2 = load_g(G2, ip + 1);
-          geo2cart(p0.z, p0.lon, p0.lat, xa);
-          geo2cart(p1.z, p1.lon, p1.lat, xb);
-          geo2cart(p2.z, p2.lon, p2.lat, xc);
-          G2[ip].w = 0.5 * (dist3(xa, xb) + dist3(xb, xc));
-        }
-        G4 gl = load_g(G2, n2 - 1), gm = load_g(G2, n2 - 2);
-        geo2cart(gl.z, gl.lon, gl.lat, xa);
-        geo2cart(gm.z, gm.lon, gm.lat, xb);
-        G2[n2 - 1].w = 0.5 * dist3(xa, xb);
-      }
-      // Layer membership and transition factor (lineofsight.c:334-363).
-      for (int ip = 0; ip < n2; ip++) {
-        double zi = G2[ip].x;
-        int ai = -999;
-        double af = 0;
-        bool pre = (ip == 0) || ((G2[ip - 1].x < altimax || zi < altimax) &&
-                                 (G2[ip - 1].x > altimin || zi > altimin));
-        if (pre) {
-          for (int il = 0; il < A.nl; il++) {
-            double top = A.top[il], bot = A.bottom[il], tr = A.trans[il];
-            if (zi <= top && zi >= bot) { ai = il; af = 1.; }
-            if (tr > A.transs && zi <= top + tr && zi > top) { ai = il; af = (top + tr - zi) / tr; }
-            if (tr > A.transs && zi < bot && zi >= bot - tr) { ai = il; af = fabs(bot - tr - zi) / tr; }
-          }
-        }
-        rec[(size_t)ip * A.R + REC_AF] = (float)af;
-        rec[(size_t)ip * A.R + REC_AI] = (float)ai;
-      }
       F = G2;
-      nf = n2;
-    }
-  } else {
-    for (int ip = 0; ip < np; ip++) {
-      rec[(size_t)ip * A.R + REC_AF] = 0.f;
-      rec[(size_t)ip * A.R + REC_AI] = -999.f;
+      nf = ins.n;
     }
   }
-
-  // FP32 point records: atmosphere at each final point and column densities.
-  if (!(status & fatal)) {
-    const int uo = rec_u(A.nw);
-    for (int ip = 0; ip < nf; ip++) {
-      double4 g = F[ip];
-      int il = atm_locate(M, a, g.x);
-      AtmPoint ap = atm_pt(M, a, g.x, il);
-      float* pr = rec + (size_t)ip * A.R;
-      pr[REC_P] = (float)ap.p;
-      pr[REC_T] = (float)ap.t;
-      pr[REC_DS] = (float)g.w;
-      double qh = 0;
-      for (int ig = 0; ig < M.ng; ig++) {
-        double q = atm_q(M, a, ig, g.x, il);
-        if (ig == A.ig_h2o) qh = q;
-        pr[uo + ig] = (float)(10 * q * ap.p / (JUR_KBOLTZ * ap.t) * g.w);
-      }
-      pr[REC_QH2O] = (float)qh;
-      for (int iw = 0; iw < M.nw; iw++) pr[REC_K + iw] = (float)atm_k(M, a, iw, g.x, il);
-    }
-  } else {
-    nf = 0;
-  }
+  if (status & fatal) nf = 0;
 
   A.np_out[r] = nf;
   A.tsurf_out[r] = (float)mo.tsurf;
@@ -194,8 +129,82 @@ __global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
   if (status) atomicOr(A.status, status);
 }
 
+// One thread per (ray, final point): with aerosol layers the segment length
+// from Cartesian half-chords (lineofsight.c:310-324) and the layer
+// membership / transition factor (:334-363); then the atmosphere at the
+// point's altitude and the FP32 record with the column densities
+// u = 10 q p / (k_B T) ds (:182-185, :330-332).
+__global__ void k_records(TraceArgs A) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)A.nray * A.cap) return;
+  const int r = (int)(t / A.cap), ip = (int)(t % A.cap);
+  const int nf = A.np_out[r];
+  if (ip >= nf) return;
+  const int a = A.atm_idx ? A.atm_idx[r] : 0;
+  const DevAtm& M = A.atm;
+  // k_raytrace inserted layer-edge points (into geo2) for every ray with
+  // points when aerosol layers are on
+  const bool ins = A.sca_n > 0 && A.geo2 != nullptr;
+  double4* F = (ins ? A.geo2 : A.geo) + (size_t)r * A.cap;
+  float* pr = A.rec + ((size_t)r * A.cap + ip) * A.R;
+  const double4 g = F[ip];
+  double ds = g.w;
+  float af = 0.f, ai = -999.f;
+  if (ins) {
+    double xa[3], xb[3], xc[3];
+    if (nf >= 2) {
+      if (ip == 0 || ip == nf - 1) {
+        const double4 o = F[ip == 0 ? 1 : nf - 2];
+        geo2cart(g.x, g.y, g.z, xa);
+        geo2cart(o.x, o.y, o.z, xb);
+        ds = 0.5 * dist3(xa, xb);
+      } else {
+        const double4 p0 = F[ip - 1], p2 = F[ip + 1];
+        geo2cart(p0.x, p0.y, p0.z, xa);
+        geo2cart(g.x, g.y, g.z, xb);
+        geo2cart(p2.x, p2.y, p2.z, xc);
+        ds = 0.5 * (dist3(xa, xb) + dist3(xb, xc));
+      }
+      F[ip].w = ds;
+    }
+    const double zi = g.x;
+    const double altimax = A.nalti > 0 ? A.altimax : NAN, altimin = A.nalti > 0 ? A.altimin : NAN;
+    const bool pre = (ip == 0) || ((F[ip - 1].x < altimax || zi < altimax) && (F[ip - 1].x > altimin || zi > altimin));
+    double afd = 0;
+    int aii = -999;
+    if (pre) {
+      for (int il = 0; il < A.nl; il++) {
+        double top = A.top[il], bot = A.bottom[il], tr = A.trans[il];
+        if (zi <= top && zi >= bot) { aii = il; afd = 1.; }
+        if (tr > A.transs && zi <= top + tr && zi > top) { aii = il; afd = (top + tr - zi) / tr; }
+        if (tr > A.transs && zi < bot && zi >= bot - tr) { aii = il; afd = fabs(bot - tr - zi) / tr; }
+      }
+    }
+    af = (float)afd;
+    ai = (float)aii;
+  }
+  pr[REC_AF] = af;
+  pr[REC_AI] = ai;
+  const int uo = rec_u(A.nw);
+  const int il = atm_locate(M, a, g.x);
+  const AtmPoint ap = atm_pt(M, a, g.x, il);
+  pr[REC_P] = (float)ap.p;
+  pr[REC_T] = (float)ap.t;
+  pr[REC_DS] = (float)ds;
+  double qh = 0;
+  for (int ig = 0; ig < M.ng; ig++) {
+    const double q = atm_q(M, a, ig, g.x, il);
+    if (ig == A.ig_h2o) qh = q;
+    pr[uo + ig] = (float)(10 * q * ap.p / (JUR_KBOLTZ * ap.t) * ds);
+  }
+  pr[REC_QH2O] = (float)qh;
+  for (int iw = 0; iw < M.nw; iw++) pr[REC_K + iw] = (float)atm_k(M, a, iw, g.x, il);
+}
+
 void launch_raytrace(const TraceArgs& A, hipStream_t s) {
   if (A.nray <= 0) return;
   int bs = 128;
   hipLaunchKernelGGL(k_raytrace, dim3((A.nray + bs - 1) / bs), dim3(bs), 0, s, A);
+  const long npt = (long)A.nray * A.cap;
+  hipLaunchKernelGGL(k_records, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, s, A);
 }
