@@ -75,6 +75,33 @@ void copy_aero(ctl_t *ctl, aero_t *aero_dest, aero_t *aero_src, int init);
 double brightness(double rad, double nu);
 double planck(double t, double nu);
 
+/* ---- Retrieval: finite-difference Jacobian ------------------------------ */
+
+/* Layout of GSL's gsl_matrix (gsl 1.15, matrix/gsl_matrix_double.h): a
+   caller's gsl_matrix * may be passed where this type is expected. */
+typedef struct {
+  size_t size1, size2, tda;
+  double *data;
+  void *block;
+  int owner;
+} jur_gsl_matrix;
+
+/* kernel (reference src/retrievalmodel.h, src/retrievalmodel.c:201-298):
+   k[i][j] = dy_i/dx_j by forward differences.  Runs formod on the
+   unperturbed state first (filling obs, applying hydrostatic balance to atm
+   and refreshing retrieved aerosol optics), then all perturbed states as
+   batched forward calls.  k->size1 / size2 must be the measurement / state
+   vector sizes (jur_meas_size after formod, jur_state_size). */
+void kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, jur_gsl_matrix *k);
+/* The same with a plain row-major matrix k [m][ldk]. Returns 0. */
+int jur_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m, size_t n,
+               size_t ldk);
+/* Sizes of atm2x's state vector (reference src/atmosphere.c:5-68; aero may
+   be NULL) and of obs2y's measurement vector (finite radiances, reference
+   src/retrievalmodel.c:379-404). */
+size_t jur_state_size(ctl_t *ctl, atm_t *atm, aero_t *aero);
+size_t jur_meas_size(ctl_t *ctl, obs_t *obs);
+
 /* ---- Batched extension (no reference counterpart) --------------------- */
 
 /* Rays with their own atmospheres.  Profiles: z, p, t [natm][nlev];
@@ -121,7 +148,15 @@ int jur_init(int device);
    sort, path integration -- and replayed while the inputs, outputs and
    work buffers keep their addresses; a replay whose rays overflow the
    recorded LOS capacity is detected and the call re-run), "surrogate",
-   "hull_margin", "collect" (learned scattering source, below).
+   "hull_margin", "collect" (learned scattering source, below),
+   "kernel_rays" (rays per batched call of kernel/jur_kernel, default
+   131072: the perturbed atmospheres are grouped into calls of about this
+   many rays), "fp64" (1: path integration in FP64 arithmetic, the
+   reference's operation order, with FP64 radiances for formod; default 0:
+   FP32 with a compensated radiance sum), "kernel_fp64" (default 1: kernel
+   and jur_kernel integrate the perturbed paths in FP64, since a 1 %
+   perturbation of one level can change a radiance by less than FP32
+   resolves).
    Returns 0. */
 int jur_set_option(const char *key, double value);
 /* Statistics of the last call: "path_ms" (path-integration kernel time,
@@ -133,7 +168,8 @@ int jur_set_option(const char *key, double value);
    "graph_ms" (replayed call), "graph_overflows", "surrogate_points",
    "fallback_points", "mlp_ms", "mlp_samples", "train_ms",
    "shared_grids" (1: the fast table layout is in use), "mie_spheres"
-   (spheres of the last get_opt_prop).  Unknown keys give NaN. */
+   (spheres of the last get_opt_prop), "kernel_ms" (wall time of the last
+   kernel call), "kernel_columns".  Unknown keys give NaN. */
 double jur_stat(const char *key);
 
 /* ---- Scattering-source surrogate (no reference counterpart) ---------- */

@@ -36,6 +36,19 @@
 #define NRAD 170
 #define REFMAX 5000
 
+/* Retrieval vector sizes and state-vector quantity indices (reference
+   src/jurassic.h:139-221).  IDXK/IDXNN/IDXRR/IDXSS need `ctl` in scope. */
+#define NQMAX (2 + NGMAX + NWMAX)
+#define MMAX (NRMAX * NDMAX)
+#define NMAX (NQMAX * NPMAX)
+#define IDXP 0
+#define IDXT 1
+#define IDXQ(ig) (2 + (ig))
+#define IDXK(iw) (2 + ctl->ng + (iw))
+#define IDXNN (2 + ctl->ng + ctl->nw + 1)
+#define IDXRR (2 + ctl->ng + ctl->nw + 2)
+#define IDXSS (2 + ctl->ng + ctl->nw + 3)
+
 /* ------------------------------------------------------------------------
    Physical constants (reference src/jurassic.h:113-129 and the GSL 1.15
    constants the reference takes from gsl_const_mksa.h / gsl_const_num.h).

@@ -35,6 +35,12 @@ struct Engine {
   // work buffers of the primary pass
   DBuf geo, geo2, rec, npts, tsurf, status, ids, order, keys, sorttmp;
   int cap = 512;
+  // FP64 point records and surface temperatures of the last trace (option
+  // "fp64"); the next path integration reads them
+  DBuf rec64, tsurf64;
+  const double* last_rec64 = nullptr;
+  const double* last_tsurf64 = nullptr;
+  int last_nray = 0, last_cap = 0;
   std::map<std::string, double> opt, stats;
   // per-call accounting of the path-integration launches: event pairs
   // around each launch and the LOS points integrated (device counter)
@@ -442,6 +448,15 @@ void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays
     A.tpz = tpz;
     A.tplon = tplon;
     A.tplat = tplat;
+    A.rec64 = nullptr;
+    if (engine_option_int("fp64", 0)) {
+      A.rec64 = (double*)e.rec64.need(npt * R * sizeof(double));
+      if (!A.tsurf64) A.tsurf64 = (double*)e.tsurf64.need(rays.nray * sizeof(double));
+    }
+    e.last_rec64 = A.rec64;
+    e.last_tsurf64 = A.tsurf64;
+    e.last_nray = rays.nray;
+    e.last_cap = cap;
     A.status = (int*)e.status.need(sizeof(int) * 4);
     launch_zero_int(A.status, 1, s);
     launch_raytrace(A, s);
@@ -595,7 +610,8 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
     return;
   }
   TraceBufs tb{&e.geo, &e.geo2, &e.rec, &e.npts, &e.tsurf, nullptr, nullptr, 0};
-  const bool graph = engine_option_int("graph", 0) != 0;
+  // FP64 calls are not recorded (the graph key does not hold the precision)
+  const bool graph = engine_option_int("graph", 0) != 0 && engine_option_int("fp64", 0) == 0;
   if (graph && e.g_exec && e.g_key == graph_key(ctl, atm, rays, out, scattering, s)) {
     if (replay_forward(rays, s)) return;
   }
@@ -605,7 +621,7 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
   int cap = engine_trace(ctl, atm, rays, sca_on, tb, out.tpz, out.tplon, out.tplat, s);
   HIP_OK(hipEventRecord(ev[1], s));
   engine_integrate(ctl, rays.nray, cap, e.rec.as<float>(), e.npts.as<int>(), e.tsurf.as<float>(), 0, nullptr,
-                   nullptr, out.rad, out.tau, rays.nray, 0, s);
+                   nullptr, out.rad, out.tau, rays.nray, 0, s, out.rad64, out.tau64);
   HIP_OK(hipEventRecord(ev[2], s));
   if (out.npts) HIP_OK(hipMemcpyAsync(out.npts, e.npts.p, rays.nray * sizeof(int), hipMemcpyDeviceToDevice, s));
   HIP_OK(hipEventSynchronize(ev[2]));
@@ -625,7 +641,7 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
 // Sort rays by point count and run the path integration.
 void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, const int* npts,
                       const float* tsurf, int scattering, const float* J, const int* jbase, float* rad,
-                      float* tau, int out_ld, int out_off, hipStream_t s) {
+                      float* tau, int out_ld, int out_off, hipStream_t s, double* rad64, double* tau64) {
   Engine& e = E();
   int* order = (int*)e.order.need(nray * sizeof(int));
   if (e.capturing) {
@@ -667,6 +683,12 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   P.tau = tau;
   P.out_ld = out_ld;
   P.out_off = out_off;
+  P.fp64 = engine_option_int("fp64", 0) != 0;
+  P.rad64 = P.fp64 ? rad64 : nullptr;
+  P.tau64 = P.fp64 ? tau64 : nullptr;
+  // the FP64 records belong to the trace just made of these rays
+  P.rec64 = (P.fp64 && e.last_nray == nray && e.last_cap == cap) ? e.last_rec64 : nullptr;
+  P.tsurf64 = P.rec64 ? e.last_tsurf64 : nullptr;
   int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
   if (variant < 0 || wpb < 0) {
     // Autotuner: the first large batch of a (emitters, channels, size class)

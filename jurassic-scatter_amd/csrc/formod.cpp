@@ -15,7 +15,7 @@ struct Staging {
   DBuf az, ap, at, aq, ak;             // atmosphere [nlev], q [ng][nlev], k [nw][nlev]
   DBuf time, obsz, obslon, obslat, vpz, vplon, vplat;
   DBuf rad, tau, tpz, tplon, tplat, npts;
-  DBuf radd, taud, radd2, taud2, nu, fdz, fw, st;
+  DBuf radd, taud, radd2, taud2, nu, fdz, fw, st, rad64, tau64;
   DBuf prof0, prof1, lat;
   DBuf zll, pt, q, k, u, ds, af, ai;   // FP64 LOS dump
   DBuf bz, bp, bt, bq, bk, bidx, bo[7];  // batched host path
@@ -73,6 +73,10 @@ OutDev stage_out(const ctl_t* ctl, int nray) {
   O.tplon = (double*)g.tplon.need(nray * sizeof(double));
   O.tplat = (double*)g.tplat.need(nray * sizeof(double));
   O.npts = (int*)g.npts.need(nray * sizeof(int));
+  if (engine_option_int("fp64", 0)) {
+    O.rad64 = (double*)g.rad64.need((size_t)ctl->nd * nray * sizeof(double) + 8);
+    O.tau64 = (double*)g.tau64.need((size_t)ctl->nd * nray * sizeof(double) + 8);
+  }
   return O;
 }
 
@@ -124,11 +128,52 @@ void engine_hydrostatic(ctl_t* ctl, atm_t* atm) {
   HIP_OK(hipStreamSynchronize(s));
 }
 
+// FP64 epilogue of formod (forwardmodel.c:101-109) on the device: FOV
+// convolution over rays of the same time within each run of seg consecutive
+// rays (one obs_t per run), then the brightness temperature.  Starts from
+// the FP64 outputs O.rad64/tau64 when present (modified in place), else
+// from the FP32 ones.
+void formod_epilogue(const ctl_t* ctl, int nray, int seg, const double* time, const OutDev& O, double** rad,
+                     double** tau, hipStream_t s) {
+  Staging& g = S();
+  const int nd = ctl->nd;
+  size_t n = (size_t)nd * nray;
+  double *radd = O.rad64, *taud = O.tau64;
+  if (!radd || !taud) {
+    radd = (double*)g.radd.need(n * sizeof(double) + 8);
+    taud = (double*)g.taud.need(n * sizeof(double) + 8);
+    launch_f2d(O.rad, radd, n, s);
+    launch_f2d(O.tau, taud, n, s);
+  }
+  if (ctl->fov[0] != '-') {
+    std::vector<double> dz, w;
+    read_shape_file(ctl->fov, dz, w);
+    double* d_dz = put(g.fdz, dz.data(), dz.size(), s);
+    double* d_w = put(g.fw, w.data(), w.size(), s);
+    double* r2 = (double*)g.radd2.need(n * sizeof(double) + 8);
+    double* t2 = (double*)g.taud2.need(n * sizeof(double) + 8);
+    int* st = (int*)g.st.need(sizeof(int));
+    launch_zero_int(st, 1, s);
+    launch_fov(nray, seg, nd, time, O.tpz, radd, taud, d_dz, d_w, (int)dz.size(), r2, t2, st, s);
+    int sth = 0;
+    HIP_OK(hipMemcpyAsync(&sth, st, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (sth & JUR_ST_FOV) JUR_FAIL("Cannot apply FOV convolution!");
+    radd = r2;
+    taud = t2;
+  }
+  if (ctl->write_bbt) {
+    double* nu = put(g.nu, ctl->nu, nd, s);
+    launch_brightness(nray, nd, nu, radd, s);
+  }
+  *rad = radd;
+  *tau = taud;
+}
+
 // formod (forwardmodel.c:15-116)
 extern "C" void formod(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero) {
   static int mask[NDMAX][NRMAX];
   hipStream_t s = engine_stream();
-  Staging& g = S();
   for (int id = 0; id < ctl->nd; id++)
     for (int ir = 0; ir < obs->nr; ir++) mask[id][ir] = !std::isfinite(obs->rad[id][ir]);
   hydrostatic(ctl, atm);
@@ -148,33 +193,8 @@ extern "C" void formod(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero) {
     ctl->queue.items = nullptr;
   }
   ctl->queue.state = Queue_Inactive;
-  // Epilogue in FP64 on the device: FOV convolution, brightness temperature.
-  size_t n = (size_t)nd * nr;
-  double* radd = (double*)g.radd.need(n * sizeof(double) + 8);
-  double* taud = (double*)g.taud.need(n * sizeof(double) + 8);
-  launch_f2d(O.rad, radd, n, s);
-  launch_f2d(O.tau, taud, n, s);
-  if (ctl->fov[0] != '-') {
-    std::vector<double> dz, w;
-    read_shape_file(ctl->fov, dz, w);
-    double* d_dz = put(g.fdz, dz.data(), dz.size(), s);
-    double* d_w = put(g.fw, w.data(), w.size(), s);
-    double* r2 = (double*)g.radd2.need(n * sizeof(double) + 8);
-    double* t2 = (double*)g.taud2.need(n * sizeof(double) + 8);
-    int* st = (int*)g.st.need(sizeof(int));
-    launch_zero_int(st, 1, s);
-    launch_fov(nr, nd, R.time, O.tpz, radd, taud, d_dz, d_w, (int)dz.size(), r2, t2, st, s);
-    int sth = 0;
-    HIP_OK(hipMemcpyAsync(&sth, st, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    if (sth & JUR_ST_FOV) JUR_FAIL("Cannot apply FOV convolution!");
-    radd = r2;
-    taud = t2;
-  }
-  if (ctl->write_bbt) {
-    double* nu = put(g.nu, ctl->nu, nd, s);
-    launch_brightness(nr, nd, nu, radd, s);
-  }
+  double *radd, *taud;
+  formod_epilogue(ctl, nr, nr, R.time, O, &radd, &taud, s);
   for (int id = 0; id < nd; id++) {
     HIP_OK(hipMemcpyAsync(obs->rad[id], radd + (size_t)id * nr, nr * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(obs->tau[id], taud + (size_t)id * nr, nr * sizeof(double), hipMemcpyDeviceToHost, s));

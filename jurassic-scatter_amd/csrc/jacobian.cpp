@@ -1,0 +1,306 @@
+// jacobian.cpp -- kernel() (reference src/retrievalmodel.c:201-298): the
+// forward-difference Jacobian of formod.  The reference runs one formod per
+// state-vector element on a perturbed copy of the atmosphere; here the
+// perturbed copies are the profiles of one ray batch (each copy's rays select
+// their profile), traced and integrated together, with hydrostatic balance
+// applied to every copy in one launch.  Aerosol elements (N, r, sigma of the
+// first mode) change the layer optics and run as one batch each.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <strings.h>
+
+#include "jur_host.h"
+#include "jur_kernels_misc.h"
+
+namespace {
+
+// Rays per batched forward call (a group of atmosphere copies); option
+// "kernel_rays".
+constexpr int kRaysPerCall = 1 << 17;
+
+struct StateVec {
+  std::vector<double> x;
+  std::vector<int> iqa, ipa;
+  size_t n = 0;
+  void add(double v, int iq, int ip) {
+    x.push_back(v);
+    iqa.push_back(iq);
+    ipa.push_back(ip);
+    n++;
+  }
+};
+
+// atm2x / atm2x_help (atmosphere.c:5-95): p, T, each q, each k on the levels
+// zmin <= z <= zmax of their quantity, then the first mode's N, r, sigma.
+void state_vector(const ctl_t* ctl, const atm_t* atm, const aero_t* aero, StateVec& sv) {
+  auto levels = [&](double zmin, double zmax, const double* v, int iq) {
+    for (int ip = 0; ip < atm->np; ip++)
+      if (atm->z[ip] >= zmin && atm->z[ip] <= zmax) sv.add(v[ip], iq, ip);
+  };
+  levels(ctl->retp_zmin, ctl->retp_zmax, atm->p, IDXP);
+  levels(ctl->rett_zmin, ctl->rett_zmax, atm->t, IDXT);
+  for (int ig = 0; ig < ctl->ng; ig++) levels(ctl->retq_zmin[ig], ctl->retq_zmax[ig], atm->q[ig], IDXQ(ig));
+  for (int iw = 0; iw < ctl->nw; iw++) levels(ctl->retk_zmin[iw], ctl->retk_zmax[iw], atm->k[iw], IDXK(iw));
+  if (ctl->retnn) sv.add(aero->nn[0], IDXNN, -1);
+  if (ctl->retrr) sv.add(aero->rr[0], IDXRR, -1);
+  if (ctl->retss) sv.add(aero->ss[0], IDXSS, -1);
+}
+
+// Perturbation size (retrievalmodel.c:242-268); <= 0 when none is defined.
+double perturbation(const ctl_t* ctl, int iq, double x) {
+  double h = -999;
+  if (iq == IDXP) h = std::fmax(std::fabs(0.01 * x), 1e-7);
+  if (iq == IDXT) h = 1;
+  if (iq >= IDXQ(0) && iq < IDXQ(ctl->ng)) h = std::fmax(std::fabs(0.01 * x), 1e-15);
+  if (iq >= IDXK(0) && iq < IDXK(ctl->nw)) h = 1e-4;
+  if (iq == IDXNN) h = std::fmax(std::fabs(0.5 * x), 0.1);
+  if (iq == IDXRR) h = std::fmax(std::fabs(0.25 * x), 0.1);
+  if (iq == IDXSS) h = std::fmax(std::fabs(0.1 * x), 0.01);
+  return h;
+}
+
+// obs2y (retrievalmodel.c:379-404): finite radiances, ray-major.
+size_t meas_count(const ctl_t* ctl, const obs_t* obs) {
+  size_t m = 0;
+  for (int ir = 0; ir < obs->nr; ir++)
+    for (int id = 0; id < ctl->nd; id++) m += std::isfinite(obs->rad[id][ir]) ? 1 : 0;
+  return m;
+}
+
+struct JacBufs {
+  DBuf z, p, t, q, k, h2o, lat, p0, p1, geo[7], idx, rad, tau, rad64, tau64, tpz, tplon, tplat;
+};
+JacBufs& B() {
+  static JacBufs b;
+  return b;
+}
+
+template <typename T>
+T* upload(DBuf& b, const std::vector<T>& h, hipStream_t s) {
+  T* d = (T*)b.need(h.size() * sizeof(T) + sizeof(T));
+  if (!h.empty()) HIP_OK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return d;
+}
+
+// One batched formod over copies of atm: copy c has state element elem[c]
+// increased by step[c] (elem < 0 or an aerosol element: the atmosphere
+// unchanged).  Returns the FP64 radiances [nd][C * nr] after formod's
+// epilogue, read back to y.
+void run_copies(ctl_t* ctl, const atm_t* atm, const obs_t* obs, const aero_t* aero, const StateVec& sv,
+                const std::vector<long>& elem, const std::vector<double>& step, std::vector<double>& y,
+                hipStream_t s) {
+  JacBufs& b = B();
+  const int C = (int)elem.size(), nl = atm->np, nr = obs->nr, ng = ctl->ng, nw = ctl->nw, nd = ctl->nd;
+  const size_t N = (size_t)C * nr;
+  std::vector<double> z((size_t)C * nl), p((size_t)C * nl), t((size_t)C * nl), lat((size_t)C * nl);
+  std::vector<double> q((size_t)C * (ng > 0 ? ng : 1) * nl), k((size_t)C * (nw > 0 ? nw : 1) * nl);
+  int ih2o = -1;
+  for (int ig = 0; ig < ng; ig++)
+    if (strcasecmp(ctl->emitter[ig], "H2O") == 0) {
+      ih2o = ig;
+      break;
+    }
+  std::vector<double> h2o(ih2o >= 0 ? (size_t)C * nl : 0);
+  for (int c = 0; c < C; c++) {
+    double* zc = &z[(size_t)c * nl];
+    double* pc = &p[(size_t)c * nl];
+    double* tc = &t[(size_t)c * nl];
+    double* qc = &q[(size_t)c * ng * nl];
+    double* kc = &k[(size_t)c * nw * nl];
+    memcpy(zc, atm->z, nl * sizeof(double));
+    memcpy(pc, atm->p, nl * sizeof(double));
+    memcpy(tc, atm->t, nl * sizeof(double));
+    memcpy(&lat[(size_t)c * nl], atm->lat, nl * sizeof(double));
+    for (int ig = 0; ig < ng; ig++) memcpy(qc + (size_t)ig * nl, atm->q[ig], nl * sizeof(double));
+    for (int iw = 0; iw < nw; iw++) memcpy(kc + (size_t)iw * nl, atm->k[iw], nl * sizeof(double));
+    const long j = elem[c];
+    if (j >= 0 && sv.ipa[j] >= 0) {
+      const int iq = sv.iqa[j], ip = sv.ipa[j];
+      double* v = iq == IDXP ? pc : iq == IDXT ? tc : iq < IDXK(0) ? qc + (size_t)(iq - IDXQ(0)) * nl
+                                                                   : kc + (size_t)(iq - IDXK(0)) * nl;
+      v[ip] = sv.x[j] + step[c];
+    }
+    if (ih2o >= 0) memcpy(&h2o[(size_t)c * nl], qc + (size_t)ih2o * nl, nl * sizeof(double));
+  }
+  AtmSetDev A;
+  A.natm = C;
+  A.nlev = nl;
+  double* dz = upload(b.z, z, s);
+  double* dp = upload(b.p, p, s);
+  double* dt = upload(b.t, t, s);
+  A.z = dz;
+  A.p = dp;
+  A.t = dt;
+  A.q = upload(b.q, q, s);
+  A.k = upload(b.k, k, s);
+  if (ctl->hydz >= 0 && nl > 0) {
+    // hydrostatic (atmosphere.c:163-184) on every copy: profiles are runs of
+    // equal (lon, lat)
+    std::vector<int> p0, p1;
+    for (int c = 0; c < C; c++) {
+      int ip0 = 0;
+      for (int ip = 1; ip <= nl; ip++)
+        if (ip == nl || atm->lon[ip] != atm->lon[ip0] || atm->lat[ip] != atm->lat[ip0]) {
+          p0.push_back(c * nl + ip0);
+          p1.push_back(c * nl + ip);
+          ip0 = ip;
+        }
+    }
+    launch_hydrostatic(dz, dp, dt, ih2o >= 0 ? upload(b.h2o, h2o, s) : nullptr, upload(b.lat, lat, s),
+                       upload(b.p0, p0, s), upload(b.p1, p1, s), (int)p0.size(), ctl->hydz, s);
+  }
+  const double* src[7] = {obs->time, obs->obsz, obs->obslon, obs->obslat, obs->vpz, obs->vplon, obs->vplat};
+  double* geo[7];
+  std::vector<double> rep(N);
+  for (int i = 0; i < 7; i++) {
+    for (int c = 0; c < C; c++) memcpy(&rep[(size_t)c * nr], src[i], nr * sizeof(double));
+    geo[i] = upload(b.geo[i], rep, s);
+  }
+  std::vector<int> idx(N);
+  for (size_t i = 0; i < N; i++) idx[i] = (int)(i / nr);
+  RaySetDev R;
+  R.nray = (int)N;
+  R.time = geo[0];
+  R.obsz = geo[1];
+  R.obslon = geo[2];
+  R.obslat = geo[3];
+  R.vpz = geo[4];
+  R.vplon = geo[5];
+  R.vplat = geo[6];
+  R.atm_idx = C > 1 ? upload(b.idx, idx, s) : nullptr;
+  OutDev O;
+  O.rad = (float*)b.rad.need(nd * N * sizeof(float) + 4);
+  O.tau = (float*)b.tau.need(nd * N * sizeof(float) + 4);
+  if (engine_option_int("fp64", 0)) {
+    O.rad64 = (double*)b.rad64.need(nd * N * sizeof(double) + 8);
+    O.tau64 = (double*)b.tau64.need(nd * N * sizeof(double) + 8);
+  }
+  O.tpz = (double*)b.tpz.need(N * sizeof(double) + 8);
+  O.tplon = (double*)b.tplon.need(N * sizeof(double) + 8);
+  O.tplat = (double*)b.tplat.need(N * sizeof(double) + 8);
+  HIP_OK(hipStreamSynchronize(s));
+  engine_prepare(ctl);
+  engine_set_aero(ctl, ctl->sca_n > 0 ? aero : nullptr);
+  engine_forward(ctl, A, R, ctl->sca_mult, O, s);
+  double *rad, *tau;
+  formod_epilogue(ctl, (int)N, nr, R.time, O, &rad, &tau, s);
+  y.resize(nd * N);
+  HIP_OK(hipMemcpyAsync(y.data(), rad, nd * N * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" size_t jur_state_size(ctl_t* ctl, atm_t* atm, aero_t* aero) {
+  static aero_t none;
+  StateVec sv;
+  state_vector(ctl, atm, aero ? aero : &none, sv);
+  return sv.n;
+}
+
+extern "C" size_t jur_meas_size(ctl_t* ctl, obs_t* obs) { return meas_count(ctl, obs); }
+
+// kernel (retrievalmodel.c:201-298)
+extern "C" int jur_kernel(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, double* k, size_t m, size_t n,
+                          size_t ldk) {
+  static aero_t none, aero1;
+  if (!aero) aero = &none;
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = engine_stream();
+  // Path integration in FP64 for the perturbed calls (option "kernel_fp64",
+  // default 1): a 1 % perturbation of one level can change a radiance by
+  // less than FP32 resolves.
+  const double fp64_saved = engine_option_double("fp64", 0);
+  const bool fp64 = engine_option_int("kernel_fp64", 1) != 0;
+  // the unperturbed forward model: fills obs, applies hydrostatic balance to
+  // atm and refreshes retrieved aerosol optics, as the reference's first call
+  formod(ctl, atm, obs, aero);
+  StateVec sv;
+  state_vector(ctl, atm, aero, sv);
+  if (sv.n != n || meas_count(ctl, obs) != m) JUR_FAIL("Matrix size does not match the state and measurement vectors!");
+  if (n > 0 && ldk < n) JUR_FAIL("Leading dimension of the kernel matrix is too small!");
+  std::vector<double> h(n);
+  for (size_t j = 0; j < n; j++) {
+    h[j] = perturbation(ctl, sv.iqa[j], sv.x[j]);
+    if (h[j] <= 0) JUR_FAIL("Cannot set perturbation size!");
+  }
+  for (size_t i = 0; i < m; i++)
+    for (size_t j = 0; j < n; j++) k[i * ldk + j] = 0;
+  const int nr = obs->nr, nd = ctl->nd;
+  if (n == 0 || m == 0) return 0;
+  // measurement slots of obs2y (ray-major, channels whose unperturbed
+  // radiance is finite; a perturbed formod masks the others)
+  std::vector<int> slot_id, slot_ir;
+  for (int ir = 0; ir < nr; ir++)
+    for (int id = 0; id < nd; id++)
+      if (std::isfinite(obs->rad[id][ir])) {
+        slot_id.push_back(id);
+        slot_ir.push_back(ir);
+      }
+  // obs2y of copy c of a batched call with N rays: the finite radiances among
+  // the slots (a perturbed radiance that turns non-finite shifts the rest up
+  // and leaves the last slots as the previous column set them, as in the
+  // reference)
+  std::vector<double> yy1(m, NAN), y0(m), y;
+  auto compact = [&](size_t N, size_t c, std::vector<double>& out) {
+    size_t i = 0;
+    for (size_t sl = 0; sl < m; sl++) {
+      double v = y[(size_t)slot_id[sl] * N + c * nr + slot_ir[sl]];
+      if (std::isfinite(v)) out[i++] = v;
+    }
+  };
+  auto column = [&](size_t j, const std::vector<double>& base) {
+    for (size_t r = 0; r < m; r++) k[r * ldk + j] = (yy1[r] - base[r]) / h[j];
+  };
+  // Atmospheric elements: groups of perturbed copies, each group with its own
+  // unperturbed copy 0, so every difference is taken between rays of the
+  // same batched call.
+  jur_set_option("fp64", fp64 ? 1 : fp64_saved);
+  size_t natm = 0;
+  while (natm < n && sv.ipa[natm] >= 0) natm++;
+  const int rays_per_call = engine_option_int("kernel_rays", kRaysPerCall);
+  const size_t per = (size_t)std::max(2, rays_per_call / std::max(nr, 1)) - 1;
+  for (size_t j0 = 0; j0 < natm; j0 += per) {
+    std::vector<long> elem{-1};
+    std::vector<double> step{0};
+    for (size_t j = j0; j < std::min(natm, j0 + per); j++) {
+      elem.push_back((long)j);
+      step.push_back(h[j]);
+    }
+    run_copies(ctl, atm, obs, aero, sv, elem, step, y, s);
+    const size_t N = elem.size() * (size_t)nr;
+    for (size_t i = 0; i < m; i++) y0[i] = y[(size_t)slot_id[i] * N + slot_ir[i]];
+    for (size_t c = 1; c < elem.size(); c++) {
+      compact(N, c, yy1);
+      column((size_t)elem[c], y0);
+    }
+  }
+  // Aerosol elements (first mode's N, r, sigma): fresh layer optics for the
+  // perturbed parameter, one call each beside an unperturbed call.
+  if (natm < n) {
+    run_copies(ctl, atm, obs, aero, sv, {-1}, {0}, y, s);
+    for (size_t i = 0; i < m; i++) y0[i] = y[(size_t)slot_id[i] * nr + slot_ir[i]];
+  }
+  for (size_t j = natm; j < n; j++) {
+    aero1 = *aero;
+    const double v = sv.x[j] + h[j];
+    if (sv.iqa[j] == IDXNN) aero1.nn[0] = v;
+    if (sv.iqa[j] == IDXRR) aero1.rr[0] = v;
+    if (sv.iqa[j] == IDXSS) aero1.ss[0] = v;
+    get_opt_prop(ctl, &aero1);
+    run_copies(ctl, atm, obs, &aero1, sv, {-1}, {0}, y, s);
+    compact((size_t)nr, 0, yy1);
+    column(j, y0);
+  }
+  jur_set_option("fp64", fp64_saved);
+  engine_set_stat("kernel_ms",
+                  std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  engine_set_stat("kernel_columns", (double)n);
+  return 0;
+}
+
+extern "C" void kernel(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, jur_gsl_matrix* k) {
+  if (!k) JUR_FAIL("Missing kernel matrix!");
+  jur_kernel(ctl, atm, obs, aero, k->data, k->size1, k->size2, k->tda);
+}

@@ -82,6 +82,9 @@ struct OutDev {
   float* tau = nullptr;   // [nd][nray]
   double *tpz = nullptr, *tplon = nullptr, *tplat = nullptr;  // [nray]
   int* npts = nullptr;    // [nray] (optional)
+  // FP64 radiance / transmittance [nd][nray] (optional): with option "fp64"
+  // the path integration runs in FP64 and also writes these
+  double *rad64 = nullptr, *tau64 = nullptr;
 };
 
 // Make sure tables / Planck table / channel constants for ctl are on the
@@ -110,7 +113,8 @@ int engine_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, 
 // Sort by point count and integrate along the traced rays.
 void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, const int* npts,
                       const float* tsurf, int scattering, const float* J, const int* jbase,
-                      float* rad, float* tau, int out_ld, int out_off, hipStream_t s);
+                      float* rad, float* tau, int out_ld, int out_off, hipStream_t s,
+                      double* rad64 = nullptr, double* tau64 = nullptr);
 // Layer-edge altitudes of the aerosol layers (for the solar-ray search).
 struct AeroGeom {
   const double* alti;
@@ -131,3 +135,7 @@ double engine_option_double(const char* key, double dflt);
 void engine_get_opt_prop(ctl_t* ctl, aero_t* aero);
 // Hydrostatic equilibrium for a host atmosphere (atmosphere.c:163-243), on the device.
 void engine_hydrostatic(ctl_t* ctl, atm_t* atm);
+// formod's FP64 epilogue (FOV within runs of seg rays, brightness temperature)
+// on outputs O of nray rays; returns device [nd][nray] radiance/transmittance.
+void formod_epilogue(const ctl_t* ctl, int nray, int seg, const double* time, const OutDev& O, double** rad,
+                     double** tau, hipStream_t s);

@@ -30,6 +30,7 @@ struct TraceArgs {
   double4* geo;        // [nray][cap] march points (z, lon, lat, ds)
   double4* geo2;       // [nray][cap] points after layer insertion
   float* rec;          // [nray][cap][R] FP32 point records
+  double* rec64;       // optional FP64 copy of the records
   int* np_out;
   float* tsurf_out;
   double* tsurf64;     // optional
@@ -58,6 +59,10 @@ struct PathArgs {
   float* tau;
   int out_ld, out_off;
   int cpw;                 // kernel variant (channels per wave x occupancy), see launch_path
+  int fp64;                // 1: FP64 arithmetic (k_path64); also writes rad64/tau64 if set
+  double *rad64, *tau64;   // [nd][out_ld] (optional)
+  const double* rec64;     // FP64 records / surface temperatures (optional, fp64)
+  const double* tsurf64;
 };
 int path_variants();
 // Launch the path integration with the given waves per block.

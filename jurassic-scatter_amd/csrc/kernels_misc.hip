@@ -57,8 +57,9 @@ void launch_hydrostatic(double* z, double* p, const double* t, const double* qh2
 
 // formod_fov (reference forwardmodel.c:156-220): convolution of each ray's
 // radiance and transmittance with the field-of-view shape over neighbouring
-// rays (+-NFOV) of the same time.  One thread per ray.
-__global__ void k_fov(int nr, int nd, const double* time, const double* tpz, const double* rad_in,
+// rays (+-NFOV) of the same time.  One thread per ray; rays are [nr] with
+// independent runs of seg rays (one observation set each).
+__global__ void k_fov(int nr, int seg, int nd, const double* time, const double* tpz, const double* rad_in,
                       const double* tau_in, const double* dz, const double* w, int n,
                       double* rad_out, double* tau_out, int* status) {
   int ir = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,7 +67,8 @@ __global__ void k_fov(int nr, int nd, const double* time, const double* tpz, con
   int idx[2 * NFOV + 1];
   double z[2 * NFOV + 1];
   int nz = 0;
-  int lo = ir - NFOV > 0 ? ir - NFOV : 0, hi = ir + 1 + NFOV < nr ? ir + 1 + NFOV : nr;
+  const int s0 = ir / seg * seg, s1 = s0 + seg < nr ? s0 + seg : nr;
+  int lo = ir - NFOV > s0 ? ir - NFOV : s0, hi = ir + 1 + NFOV < s1 ? ir + 1 + NFOV : s1;
   for (int ir2 = lo; ir2 < hi; ir2++)
     if (time[ir2] == time[ir]) {
       z[nz] = tpz[ir2];
@@ -92,10 +94,11 @@ __global__ void k_fov(int nr, int nd, const double* time, const double* tpz, con
   }
 }
 
-void launch_fov(int nr, int nd, const double* time, const double* tpz, const double* rad_in,
+void launch_fov(int nr, int seg, int nd, const double* time, const double* tpz, const double* rad_in,
                 const double* tau_in, const double* dz, const double* w, int n, double* rad_out,
                 double* tau_out, int* status, hipStream_t s) {
-  hipLaunchKernelGGL(k_fov, dim3((nr + 63) / 64), dim3(64), 0, s, nr, nd, time, tpz, rad_in, tau_in,
+  if (nr <= 0 || seg <= 0) return;
+  hipLaunchKernelGGL(k_fov, dim3((nr + 63) / 64), dim3(64), 0, s, nr, seg, nd, time, tpz, rad_in, tau_in,
                      dz, w, n, rad_out, tau_out, status);
 }
 

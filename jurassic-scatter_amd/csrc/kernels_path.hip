@@ -610,6 +610,216 @@ __global__ void k_zero_int(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 path integration: the reference's arithmetic and operation order
+// (forwardmodel.c:303-399 with intpol_tbl :409-502 and formod_continua
+// :120-152) in double precision on the same FP32 tables, records and
+// per-channel constants.  Used where radiance DIFFERENCES matter: the
+// finite-difference Jacobian perturbs one level by 1 % and its radiance
+// change can be far below FP32 resolution.  One channel per wave.
+// ---------------------------------------------------------------------------
+
+// LIN on float table nodes as the reference's macro evaluates it: under C's
+// usual arithmetic conversions the differences and their quotient are
+// single precision, the offset and the sum double (jurassic.h:73-74).
+__device__ inline double lin_f(float x0, float y0, float x1, float y1, double x) {
+  const float sl = (y1 - y0) / (x1 - x0);
+  return (double)y0 + (double)sl * (x - (double)x0);
+}
+// GSL_MAX(GSL_MIN(x, 1), 0): a NaN becomes 1.
+__device__ inline double clamp01d(double x) {
+  const double a = x < 1.0 ? x : 1.0;
+  return a > 0.0 ? a : 0.0;
+}
+// locate_tbl (forwardmodel.c:546-565) on an increasing column, double key.
+template <bool KEY_EPS>
+__device__ inline int locate_col64(const float2* c, int n, double x) {
+  int ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    const int i = (ihi + ilo) >> 1;
+    const double v = KEY_EPS ? c[i].y : c[i].x;
+    if (v > x) ihi = i; else ilo = i;
+  }
+  return ilo;
+}
+// locate (misc.c:237-265) on a float grid, double key.
+__device__ inline int locate_grid64(const float* xx, int n, double x) {
+  int ilo = 0, ihi = n - 1, i = (ihi + ilo) >> 1;
+  if (xx[i] < xx[i + 1]) {
+    while (ihi > ilo + 1) {
+      i = (ihi + ilo) >> 1;
+      if ((double)xx[i] > x) ihi = i; else ilo = i;
+    }
+  } else {
+    while (ihi > ilo + 1) {
+      i = (ihi + ilo) >> 1;
+      if ((double)xx[i] <= x) ihi = i; else ilo = i;
+    }
+  }
+  return ilo;
+}
+// intpol_tbl_u then intpol_tbl_eps of one (p, T) corner (forwardmodel.c:460-474).
+__device__ inline double corner64(const float2* col, int n, double e_path, double du) {
+  const int i = locate_col64<true>(col, n, e_path);
+  const double uu = lin_f(col[i].y, col[i].x, col[i + 1].y, col[i + 1].x, e_path) + du;
+  const int j = locate_col64<false>(col, n, uu);
+  return clamp01d(lin_f(col[j].x, col[j].y, col[j + 1].x, col[j + 1].y, uu));
+}
+// Segment emissivity of one emitter; updates the path transmittance
+// (forwardmodel.c:433-500).
+__device__ inline double ega64(const DevTables& T, int tb, double p, double t, double du, double& tau_path) {
+  double eps = 0.0;
+  const int ntp = T.np[tb];
+  if (ntp < 2) {
+    eps = 0.0;
+  } else if (tau_path < 1e-9) {
+    eps = 1.0;
+  } else {
+    const float* pg = T.p + (size_t)tb * TBLNPMAX;
+    const int ipr = locate_grid64(pg, ntp, p);
+    const size_t b0 = (size_t)tb * TBLNPMAX + ipr;
+    const int nt0 = T.nt[b0], nt1 = T.nt[b0 + 1];
+    if (nt0 >= 2 && nt1 >= 2) {
+      const float* t0g = T.t + b0 * TBLNTMAX;
+      const float* t1g = t0g + TBLNTMAX;
+      const int it0 = locate_grid64(t0g, nt0, t), it1 = locate_grid64(t1g, nt1, t);
+      const size_t k00 = b0 * TBLNTMAX + it0, k10 = (b0 + 1) * TBLNTMAX + it1;
+      const int n00 = T.ncol[k00], n01 = T.ncol[k00 + 1], n10 = T.ncol[k10], n11 = T.ncol[k10 + 1];
+      if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
+        const double ep = 1.0 - tau_path;
+        double e00 = corner64(T.ue + T.col[k00], n00, ep, du);
+        double e01 = corner64(T.ue + T.col[k00 + 1], n01, ep, du);
+        double e10 = corner64(T.ue + T.col[k10], n10, ep, du);
+        double e11 = corner64(T.ue + T.col[k10 + 1], n11, ep, du);
+        e00 = clamp01d(lin<double>(t0g[it0], e00, t0g[it0 + 1], e01, t));
+        e11 = clamp01d(lin<double>(t1g[it1], e10, t1g[it1 + 1], e11, t));
+        e00 = clamp01d(lin<double>(pg[ipr], e00, pg[ipr + 1], e11, p));
+        eps = 1.0 - (1.0 - e00) / tau_path;
+      }
+    }
+  }
+  tau_path *= 1.0 - eps;
+  return eps;
+}
+// Continuum extinction [1/km] (forwardmodel.c:120-152) in FP64.
+__device__ inline double continua64(const ChanConst& c, double p, double t, double ds, double qh2o,
+                                    double uco2, double uh2o, double kw) {
+  double beta = kw;
+  if (c.co2_on && ds > 0.0) {
+    const double d230 = t - 230.0, d260 = t - 260.0, d296 = t - 296.0;
+    const double ctw = d260 * d296 * c.co2_k230 - d230 * d296 * c.co2_k260 + d230 * d260 * c.co2_k296;
+    beta += uco2 / (JUR_AVOGADRO * 1000.0 * P0) * p * ctw / ds;
+  }
+  if (c.h2o_on && ds > 0.0) {
+    const double ctwslf = c.h2o_s296 * exp(c.h2o_lnratio * ((296.0 - t) / 36.0));
+    const double a1 = (double)c.nu * uh2o * tanh(c.h2o_tanhk / t);
+    const double a3 = p / P0 * (qh2o * ctwslf + (1.0 - qh2o) * c.h2o_frn) * 1e-20;
+    beta += a1 * (296.0 / t) * a3 / ds;
+  }
+  if (c.n2_on || c.o2_on) {
+    const double pp = p / P0, tt = 273.0 / t;
+    const double base = 0.1 * pp * pp * tt * tt, ex = 1.0 / 296.0 - 1.0 / t;
+    if (c.n2_on)
+      beta += base * 0.79 * c.n2_b * exp(c.n2_beta * ex) * (0.79 + 0.21 * (1.294 - 0.4545 * t / 296.0));
+    if (c.o2_on) beta += base * 0.21 * c.o2_b * exp(c.o2_beta * ex);
+  }
+  return beta;
+}
+// Filter-weighted Planck function from the table (forwardmodel.c:786-793).
+__device__ inline double planck64(const float* row, double t) {
+  double x = (t - 100.0) * 4.0;
+  int it;
+  if (!(x == x)) {
+    it = JUR_NPLANCK - 2;
+  } else {
+    x = floor(x);
+    it = x < 0.0 ? 0 : (x > (double)(JUR_NPLANCK - 2) ? JUR_NPLANCK - 2 : (int)x);
+  }
+  const double t0 = 100.0 + 0.25 * it;
+  return lin<double>(t0, row[it], t0 + 0.25, row[it + 1], t);
+}
+
+template <int NGT>
+__global__ void __launch_bounds__(256) k_path64(PathArgs A) {
+  const int lane = threadIdx.x & 63;
+  int chunk, group;
+  if (!map_work(A, 1, chunk, group)) return;
+  const int slot = group * 64 + lane;
+  const bool valid = slot < A.nray;
+  const int ray = valid ? A.order[slot] : 0;
+  const int npts = valid ? A.np[ray] : 0;
+  const int id = chunk;
+  const int uo = rec_u(A.nw);
+  const ChanConst& cc = A.cc[id];
+  const float* prow = A.planck + (size_t)id * JUR_NPLANCK;
+  double tau_path[NGT];
+#pragma unroll
+  for (int g = 0; g < NGT; g++) tau_path[g] = 1.0;
+  double rad = 0.0, tau = 1.0;
+  int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
+  // point records: the FP64 copy when the trace wrote one
+  const size_t rbase = (size_t)ray * A.cap * A.R;
+  auto rec = [&](int ip, int f) -> double {
+    const size_t i = rbase + (size_t)ip * A.R + f;
+    return A.rec64 ? A.rec64[i] : (double)A.rec[i];
+  };
+  for (int ip = 0; ip < npts; ip++) {
+    const double p = rec(ip, REC_P), t = rec(ip, REC_T), ds = rec(ip, REC_DS), af = rec(ip, REC_AF);
+    const double qh = rec(ip, REC_QH2O);
+    double tau_gas = 1.0;
+#pragma unroll
+    for (int g = 0; g < NGT; g++)
+      if (g < A.ng) tau_gas *= 1.0 - ega64(A.T, id * A.ng + g, p, t, rec(ip, uo + g), tau_path[g]);
+    const double uco2 = cc.co2_on ? rec(ip, uo + cc.co2_ig) : 0.0;
+    const double uh2o = cc.h2o_on ? rec(ip, uo + cc.h2o_ig) : 0.0;
+    const double kw = A.nw > 0 ? rec(ip, REC_K + cc.window) : 0.0;
+    const double beta = continua64(cc, p, t, ds, qh, uco2, uh2o, kw);
+    const double B = planck64(prow, t);
+    const bool scpt = A.scattering > 0 && af > 0.0;
+    const int ai = scpt || (A.sca_n > 0 && af != 0.0) ? (int)rec(ip, REC_AI) : 0;
+    if (scpt) {
+      // scattering source (forwardmodel.c:317-360); a zero-length segment
+      // adds nothing (the reference forms -log(1)/0 there)
+      if (tau_gas > 0.0 && ds > 0.0) {
+        const double be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
+        const double J = A.J[(size_t)kj * A.nd + id];
+        const double bt = -log(tau_gas) / ds + beta + af * be;
+        const double eps = 1.0 - exp(-bt * ds);
+        const double src = ((bt - af * bs) * B + af * bs * J) / bt;
+        rad += tau * eps * src;
+        tau *= exp(-bt * ds);
+      }
+      kj++;
+    } else if (tau_gas > 0.0) {
+      // gas, continua and aerosol extinction (forwardmodel.c:363-390)
+      double bx = beta;
+      if (A.sca_n > 0 && af != 0.0) bx += af * (A.ext_beta_e ? A.beta_e[ai * A.nd + id] : A.beta_a[ai * A.nd + id]);
+      const double eps = 1.0 - tau_gas * exp(-bx * ds);
+      rad += B * eps * tau;
+      tau *= 1.0 - eps;
+    }
+  }
+  if (!valid) return;
+  // surface (forwardmodel.c:393-399)
+  const double ts = A.tsurf64 ? A.tsurf64[ray] : (double)A.tsurf[ray];
+  if (ts > 0.0) rad += planck64(prow, ts) * tau;
+  const size_t o = (size_t)id * A.out_ld + A.out_off + ray;
+  if (A.rad64) {
+    A.rad64[o] = rad;
+    A.tau64[o] = tau;
+  }
+  A.rad[o] = (float)rad;
+  A.tau[o] = (float)tau;
+}
+
+template <int NGT>
+void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
+  const int ngroups = (A.nray + 63) / 64;
+  long nb = (long)A.nd * ((ngroups + wpb - 1) / wpb);
+  nb = (nb + 7) / 8 * 8;
+  hipLaunchKernelGGL((k_path64<NGT>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+}
+
 template <int NGT, int CPW, int OCC>
 void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   int nchunk = (A.nd + CPW - 1) / CPW;
@@ -642,6 +852,12 @@ int path_variants() { return 7; }
 
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;
+  if (A.fp64) {
+    if (A.ng <= 2) launch_64<2>(A, 4, s);
+    else if (A.ng <= 5) launch_64<5>(A, 4, s);
+    else launch_64<NGMAX>(A, 4, s);
+    return;
+  }
   const int v = A.cpw;  // variant index
   if (A.ng <= 2) launch_ng<2>(A, wpb, v, s);
   else if (A.ng <= 3) launch_ng<3>(A, wpb, v, s);

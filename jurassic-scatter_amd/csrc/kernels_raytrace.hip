@@ -150,6 +150,7 @@ __global__ void k_records(TraceArgs A) {
   const double4 g = F[ip];
   double ds = g.w;
   float af = 0.f, ai = -999.f;
+  double af64 = 0.0;
   if (ins) {
     double xa[3], xb[3], xc[3];
     if (nf >= 2) {
@@ -181,6 +182,7 @@ __global__ void k_records(TraceArgs A) {
       }
     }
     af = (float)afd;
+    af64 = afd;
     ai = (float)aii;
   }
   pr[REC_AF] = af;
@@ -191,14 +193,29 @@ __global__ void k_records(TraceArgs A) {
   pr[REC_P] = (float)ap.p;
   pr[REC_T] = (float)ap.t;
   pr[REC_DS] = (float)ds;
+  double* pd = A.rec64 ? A.rec64 + ((size_t)r * A.cap + ip) * A.R : nullptr;
+  if (pd) {
+    pd[REC_AF] = af64;
+    pd[REC_AI] = ai;
+    pd[REC_P] = ap.p;
+    pd[REC_T] = ap.t;
+    pd[REC_DS] = ds;
+  }
   double qh = 0;
   for (int ig = 0; ig < M.ng; ig++) {
     const double q = atm_q(M, a, ig, g.x, il);
     if (ig == A.ig_h2o) qh = q;
-    pr[uo + ig] = (float)(10 * q * ap.p / (JUR_KBOLTZ * ap.t) * ds);
+    const double u = 10 * q * ap.p / (JUR_KBOLTZ * ap.t) * ds;
+    pr[uo + ig] = (float)u;
+    if (pd) pd[uo + ig] = u;
   }
   pr[REC_QH2O] = (float)qh;
-  for (int iw = 0; iw < M.nw; iw++) pr[REC_K + iw] = (float)atm_k(M, a, iw, g.x, il);
+  if (pd) pd[REC_QH2O] = qh;
+  for (int iw = 0; iw < M.nw; iw++) {
+    const double k = atm_k(M, a, iw, g.x, il);
+    pr[REC_K + iw] = (float)k;
+    if (pd) pd[REC_K + iw] = k;
+  }
 }
 
 void launch_raytrace(const TraceArgs& A, hipStream_t s) {

@@ -112,7 +112,7 @@ TraceBufs trace_bufs(State& st, int cap) {
 // Trace + integrate all rays of a set in chunks; outputs [nd][n].
 void integrate_set(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& R, int scattering,
                    const float* J, const int* jbase, float* rad, float* tau, double* tpz, double* tplon,
-                   double* tplat, hipStream_t s) {
+                   double* tplat, hipStream_t s, double* rad64 = nullptr, double* tau64 = nullptr) {
   State& st = ST();
   const int n = R.nray;
   if (n <= 0) return;
@@ -124,7 +124,7 @@ void integrate_set(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& R, i
     int cap = engine_trace(ctl, atm, C, 1, tb, tpz ? tpz + r0 : tpd, tplon ? tplon + r0 : tpd + n,
                            tplat ? tplat + r0 : tpd + 2 * n, s);
     engine_integrate(ctl, m, cap, st.rec.as<float>(), st.npts.as<int>(), st.tsurf.as<float>(), scattering, J,
-                     jbase ? jbase + r0 : nullptr, rad, tau, n, r0, s);
+                     jbase ? jbase + r0 : nullptr, rad, tau, n, r0, s, rad64, tau64);
     r0 += m;
   }
 }
@@ -414,7 +414,8 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     if (L == 0 && collect && V.npt > 0) G.jth = (float*)st.cjth.need((size_t)V.npt * nd * sizeof(float));
     launch_angular(G, s);
     if (L == 0) {
-      integrate_set(ctl, atm, V.rays, S, J, V.jbase.as<int>(), out.rad, out.tau, out.tpz, out.tplon, out.tplat, s);
+      integrate_set(ctl, atm, V.rays, S, J, V.jbase.as<int>(), out.rad, out.tau, out.tpz, out.tplon, out.tplat, s,
+                    out.rad64, out.tau64);
       if (out.npts) {
         // LOS points of the primaries (of the last chunk's trace when chunked)
         if (V.rays.nray <= chunk_rays(ctl, V.rays.nray))

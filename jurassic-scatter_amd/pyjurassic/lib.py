@@ -31,6 +31,10 @@ SIGNATURES = {
     "copy_aero": (None, [P, P, P, C.c_int]),
     "brightness": (C.c_double, [C.c_double, C.c_double]),
     "planck": (C.c_double, [C.c_double, C.c_double]),
+    "kernel": (None, [P, P, P, P, P]),
+    "jur_kernel": (C.c_int, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t]),
+    "jur_state_size": (C.c_size_t, [P, P, P]),
+    "jur_meas_size": (C.c_size_t, [P, P]),
     "jur_formod_batch": (C.c_int, [P, P, P, C.c_int]),
     "jur_formod_batch_device": (C.c_int, [P, P, P, C.c_int, P]),
     "jur_sync": (C.c_int, []),
@@ -62,6 +66,12 @@ class jur_batch_t(C.Structure):
                 ("rad", C.c_void_p), ("tau", C.c_void_p),
                 ("tpz", C.c_void_p), ("tplon", C.c_void_p), ("tplat", C.c_void_p),
                 ("npts", C.c_void_p)]
+
+
+class jur_gsl_matrix(C.Structure):
+    """GSL 1.15's gsl_matrix layout."""
+    _fields_ = [("size1", C.c_size_t), ("size2", C.c_size_t), ("tda", C.c_size_t),
+                ("data", C.c_void_p), ("block", C.c_void_p), ("owner", C.c_int)]
 
 
 _lib = None

@@ -1748,6 +1748,140 @@ void oracle_formod(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero) {
       if (mask[id][ir]) obs->rad[id][ir] = NAN;
 }
 
+/* ------------------------------------------------------------------------
+   Finite-difference Jacobian (reference src/retrievalmodel.c:201-298) with
+   the state- and measurement-vector mappings it uses.
+   ------------------------------------------------------------------------ */
+
+/* Levels zmin <= z <= zmax of one quantity (reference src/atmosphere.c:72-95). */
+static void o_atm2x_help(const atm_t *atm, double zmin, double zmax, const double *value,
+                         int iq, double *x, int *iqa, int *ipa, size_t *n) {
+  for (int ip = 0; ip < atm->np; ip++)
+    if (atm->z[ip] >= zmin && atm->z[ip] <= zmax) {
+      if (x) x[*n] = value[ip];
+      if (iqa) iqa[*n] = iq;
+      if (ipa) ipa[*n] = ip;
+      (*n)++;
+    }
+}
+
+/* State vector: p, T, each q, each k over their altitude ranges, then the
+   first mode's N, r, sigma (reference src/atmosphere.c:5-68). */
+size_t oracle_atm2x(const ctl_t *ctl, const atm_t *atm, const aero_t *aero, double *x, int *iqa,
+                    int *ipa) {
+  size_t n = 0;
+  o_atm2x_help(atm, ctl->retp_zmin, ctl->retp_zmax, atm->p, IDXP, x, iqa, ipa, &n);
+  o_atm2x_help(atm, ctl->rett_zmin, ctl->rett_zmax, atm->t, IDXT, x, iqa, ipa, &n);
+  for (int ig = 0; ig < ctl->ng; ig++)
+    o_atm2x_help(atm, ctl->retq_zmin[ig], ctl->retq_zmax[ig], atm->q[ig], IDXQ(ig), x, iqa, ipa, &n);
+  for (int iw = 0; iw < ctl->nw; iw++)
+    o_atm2x_help(atm, ctl->retk_zmin[iw], ctl->retk_zmax[iw], atm->k[iw], IDXK(iw), x, iqa, ipa, &n);
+  const int on[3] = {ctl->retnn, ctl->retrr, ctl->retss};
+  const int idx[3] = {IDXNN, IDXRR, IDXSS};
+  const double val[3] = {aero->nn[0], aero->rr[0], aero->ss[0]};
+  for (int i = 0; i < 3; i++)
+    if (on[i]) {
+      if (x) x[n] = val[i];
+      if (iqa) iqa[n] = idx[i];
+      if (ipa) ipa[n] = -1;
+      n++;
+    }
+  return n;
+}
+
+/* Inverse of oracle_atm2x (reference src/retrievalmodel.c:1103-1162). */
+static void o_x2atm_help(atm_t *atm, double zmin, double zmax, double *value, const double *x,
+                         size_t *n) {
+  for (int ip = 0; ip < atm->np; ip++)
+    if (atm->z[ip] >= zmin && atm->z[ip] <= zmax) value[ip] = x[(*n)++];
+}
+
+void oracle_x2atm(const ctl_t *ctl, const double *x, atm_t *atm, aero_t *aero) {
+  size_t n = 0;
+  o_x2atm_help(atm, ctl->retp_zmin, ctl->retp_zmax, atm->p, x, &n);
+  o_x2atm_help(atm, ctl->rett_zmin, ctl->rett_zmax, atm->t, x, &n);
+  for (int ig = 0; ig < ctl->ng; ig++)
+    o_x2atm_help(atm, ctl->retq_zmin[ig], ctl->retq_zmax[ig], atm->q[ig], x, &n);
+  for (int iw = 0; iw < ctl->nw; iw++)
+    o_x2atm_help(atm, ctl->retk_zmin[iw], ctl->retk_zmax[iw], atm->k[iw], x, &n);
+  if (ctl->retnn) aero->nn[0] = x[n++];
+  if (ctl->retrr) aero->rr[0] = x[n++];
+  if (ctl->retss) aero->ss[0] = x[n++];
+}
+
+/* Measurement vector: finite radiances, ray-major then channel
+   (reference src/retrievalmodel.c:379-404). */
+size_t oracle_obs2y(const ctl_t *ctl, const obs_t *obs, double *y, int *ida, int *ira) {
+  size_t m = 0;
+  for (int ir = 0; ir < obs->nr; ir++)
+    for (int id = 0; id < ctl->nd; id++)
+      if (isfinite(obs->rad[id][ir])) {
+        if (y) y[m] = obs->rad[id][ir];
+        if (ida) ida[m] = id;
+        if (ira) ira[m] = ir;
+        m++;
+      }
+  return m;
+}
+
+/* Perturbation size of a state-vector element of quantity iq and value x
+   (reference src/retrievalmodel.c:242-268); <= 0: none defined. */
+double oracle_perturbation(const ctl_t *ctl, int iq, double x) {
+  double h = -999;
+  if (iq == IDXP) h = o_max(fabs(0.01 * x), 1e-7);
+  if (iq == IDXT) h = 1;
+  if (iq >= IDXQ(0) && iq < IDXQ(ctl->ng)) h = o_max(fabs(0.01 * x), 1e-15);
+  if (iq >= IDXK(0) && iq < IDXK(ctl->nw)) h = 1e-4;
+  if (iq == IDXNN) h = o_max(fabs(0.5 * x), 0.1);
+  if (iq == IDXRR) h = o_max(fabs(0.25 * x), 0.1);
+  if (iq == IDXSS) h = o_max(fabs(0.1 * x), 0.01);
+  return h;
+}
+
+/* k[i * ldk + j] = dy_i / dx_j by forward differences: formod on the
+   unperturbed state (which also fills obs and applies hydrostatic to atm),
+   then one formod per state-vector element on a perturbed copy.  m and n must
+   be the sizes of oracle_obs2y / oracle_atm2x after that first formod.
+   DEVIATION: measurement slots a perturbed formod leaves unset (its radiance
+   turned non-finite) hold NaN; the reference leaves them uninitialised
+   (gsl_vector_alloc) or stale from the previous column. */
+void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m, size_t n,
+                   size_t ldk) {
+  static atm_t atm1;
+  static obs_t obs1;
+  static aero_t aero1, aero_none;
+  if (!aero) aero = &aero_none;
+  double *x0 = malloc((n + 1) * sizeof(double)), *x1 = malloc((n + 1) * sizeof(double));
+  double *yy0 = malloc((m + 1) * sizeof(double)), *yy1 = malloc((m + 1) * sizeof(double));
+  int *iqa = malloc((n + 1) * sizeof(int)), *ipa = malloc((n + 1) * sizeof(int));
+  oracle_formod(ctl, atm, obs, aero);
+  if (oracle_atm2x(ctl, atm, aero, x0, iqa, ipa) != n || oracle_obs2y(ctl, obs, yy0, NULL, NULL) != m)
+    OFAIL("Matrix size does not match the state and measurement vectors!");
+  for (size_t i = 0; i < m; i++) {
+    yy1[i] = NAN;
+    for (size_t j = 0; j < n; j++) k[i * ldk + j] = 0;
+  }
+  for (size_t j = 0; j < n; j++) {
+    double h = oracle_perturbation(ctl, iqa[j], x0[j]);
+    if (h <= 0) OFAIL("Cannot set perturbation size!");
+    memcpy(x1, x0, n * sizeof(double));
+    x1[j] += h;
+    atm1 = *atm;
+    aero1 = *aero;
+    obs1 = *obs;
+    oracle_x2atm(ctl, x1, &atm1, &aero1);
+    oracle_formod(ctl, &atm1, &obs1, &aero1);
+    oracle_obs2y(ctl, &obs1, yy1, NULL, NULL);
+    for (size_t i = 0; i < m; i++) k[i * ldk + j] = (yy1[i] - yy0[i]) / h;
+  }
+  free(x0);
+  free(x1);
+  free(yy0);
+  free(yy1);
+  free(iqa);
+  free(ipa);
+}
+
 /* Batched clear-air forward model, one atmosphere per ray. */
 void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z, const double *p,
                          const double *t, const double *q, const double *k, const double *time,

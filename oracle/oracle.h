@@ -29,6 +29,10 @@
  *       field of view reproducing the unconvolved radiance.
  *   Scattering depth >= 2: parity unpinned by fixtures (no reference test
  *       uses SCA_MULT >= 2); checked for the isotropic-field closure only.
+ *   kernel (finite-difference Jacobian): parity unpinned by fixtures (the
+ *       reference ships no kernel output); its columns are checked against
+ *       formod on independently perturbed atmospheres, first-order
+ *       linearity, and the zero pattern of levels a ray never reaches.
  */
 #ifndef JURASSIC_ORACLE_H
 #define JURASSIC_ORACLE_H
@@ -95,6 +99,18 @@ void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z,
                          int nthreads, double *rad, double *tau, double *tpz,
                          double *tplon, double *tplat, int *npts, const aero_t *aero,
                          int scattering);
+
+/* Retrieval: state vector (reference src/atmosphere.c:5-95), its inverse
+   (src/retrievalmodel.c:1103-1162), measurement vector (:379-404), the
+   perturbation sizes (:242-268) and the finite-difference Jacobian
+   k [m][ldk] (:201-298).  x/iqa/ipa/y/ida/ira may be NULL (size query). */
+size_t oracle_atm2x(const ctl_t *ctl, const atm_t *atm, const aero_t *aero, double *x, int *iqa,
+                    int *ipa);
+void oracle_x2atm(const ctl_t *ctl, const double *x, atm_t *atm, aero_t *aero);
+size_t oracle_obs2y(const ctl_t *ctl, const obs_t *obs, double *y, int *ida, int *ira);
+double oracle_perturbation(const ctl_t *ctl, int iq, double x);
+void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m, size_t n,
+                   size_t ldk);
 
 /* Drop the cached tables, filter functions and Planck tables. */
 void oracle_reset(void);

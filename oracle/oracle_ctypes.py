@@ -43,6 +43,11 @@ def load():
         "oracle_srcfunc_planck": (None, [P, C.c_double, D]),
         "oracle_formod_batch": (None, [P, C.c_int, C.c_int] + [D] * 12 + [C.c_int] * 3 +
                                 [D] * 5 + [C.POINTER(C.c_int), P, C.c_int]),
+        "oracle_atm2x": (C.c_size_t, [P, P, P, D, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "oracle_x2atm": (None, [P, D, P, P]),
+        "oracle_obs2y": (C.c_size_t, [P, P, D, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "oracle_perturbation": (C.c_double, [P, C.c_int, C.c_double]),
+        "oracle_kernel": (None, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t]),
         "oracle_reset": (None, []),
     }
     for name, (res, args) in sig.items():

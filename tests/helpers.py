@@ -13,12 +13,13 @@ EXAMPLE_CASES = {
 }
 
 
-def read_example(lib, prefix, case):
+def read_example(lib, prefix, case, keys=()):
     """Read ctl/obs/atm/aero of an example with `lib`'s readers (prefix
-    'oracle_' for the oracle, '' for the GPU library)."""
+    'oracle_' for the oracle, '' for the GPU library); keys are extra
+    KEY VALUE control arguments."""
     ctlf, obsf, atmf, aerof, _ = EXAMPLE_CASES[case]
     ctl, obs, atm, aero = J.ctl_t(), J.obs_t(), J.atm_t(), J.aero_t()
-    extra = ["AEROFILE", aerof] if aerof else []
+    extra = (["AEROFILE", aerof] if aerof else []) + list(keys)
     argc, argv = J.argv(ctlf, *extra)
     getattr(lib, prefix + "read_ctl")(argc, argv, C.byref(ctl))
     getattr(lib, prefix + "read_obs")(None, obsf.encode(), C.byref(ctl), C.byref(obs))

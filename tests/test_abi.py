@@ -43,3 +43,5 @@ def test_struct_layouts():
     J.check_layout()
     import ctypes as C
     assert C.sizeof(L.jur_batch_t) == 3 * 4 + 4 + 19 * 8  # 3 ints, pad, 19 pointers
+    # gsl_matrix: three size_t, two pointers, int (+ padding)
+    assert C.sizeof(L.jur_gsl_matrix) == 48 and L.jur_gsl_matrix.data.offset == 24

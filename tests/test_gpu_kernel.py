@@ -1,0 +1,168 @@
+"""kernel(): the finite-difference Jacobian (reference src/retrievalmodel.c:201-298)
+on the GPU against the oracle's restatement, on the reference examples with
+retrieval keys: temperature, pressure, all three emitters, extinction-layer
+aerosol parameters (N, r, sigma: fresh Mie optics per column), single
+scattering, hydrostatic balance with brightness temperatures, and a field of
+view (convolution within each perturbed copy's rays).
+
+The perturbed paths are integrated in FP64 (option kernel_fp64, default):
+a 1 % perturbation of one level can change a radiance by less than FP32
+resolves.  Bar: every entry within 2e-5 of its row's largest |K| (a Jacobian
+row mixes columns of very different size), and the rows' dominant entries
+(>= 10 % of the row maximum) within 1e-4 relative; what remains is the FP32
+storage of the tables and the Planck table.
+"""
+import ctypes as C
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ctypes
+from helpers import read_example
+from pyjurassic import lib as L
+
+pytestmark = pytest.mark.gpu
+
+TQ = ("RETT_ZMIN", 0, "RETT_ZMAX", 80, "RETQ_ZMIN[0]", 0, "RETQ_ZMAX[0]", 80,
+      "RETQ_ZMIN[1]", 0, "RETQ_ZMAX[1]", 30, "RETQ_ZMIN[2]", 10, "RETQ_ZMAX[2]", 50,
+      "RETP_ZMIN", 10, "RETP_ZMAX", 20)
+CASES = {
+    "clear_t_p_q": ("clear", TQ),
+    "clear_hydz_bbt": ("clear", ("HYDZ", 15, "WRITE_BBT", 1, "RETP_ZMIN", 10, "RETP_ZMAX", 20,
+                                 "RETT_ZMIN", 0, "RETT_ZMAX", 40, "RETQ_ZMIN[1]", 0, "RETQ_ZMAX[1]", 20)),
+    "ext1_aerosol": ("ext1", ("RETT_ZMIN", 0, "RETT_ZMAX", 30, "RETNN", 1, "RETRR", 1, "RETSS", 1)),
+    "aero0_scattering": ("aero0", ("RETT_ZMIN", 0, "RETT_ZMAX", 20)),
+}
+
+
+@pytest.fixture(scope="module")
+def glib():
+    return L.load()
+
+
+@pytest.fixture(scope="module")
+def olib():
+    return oracle_ctypes.load()
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def jacobian(lib, prefix, case, keys):
+    ctl, obs, atm, aero = read_example(lib, prefix, case, keys)
+    r = C.byref
+    if prefix:
+        n = lib.oracle_atm2x(r(ctl), r(atm), r(aero), None, None, None)
+        m = lib.oracle_obs2y(r(ctl), r(obs), None, None, None)
+        k = np.full((m, n), np.nan)
+        lib.oracle_kernel(r(ctl), r(atm), r(obs), r(aero), _dp(k), m, n, n)
+    else:
+        n = lib.jur_state_size(r(ctl), r(atm), r(aero))
+        m = lib.jur_meas_size(r(ctl), r(obs))
+        k = np.full((m, n), np.nan)
+        lib.jur_kernel(r(ctl), r(atm), r(obs), r(aero), _dp(k), m, n, n)
+    rad = np.array([[obs.rad[id][ir] for id in range(ctl.nd)] for ir in range(obs.nr)])
+    p = np.array(atm.p[:atm.np])
+    return k, rad, p
+
+
+def compare(kg, ko, label):
+    assert kg.shape == ko.shape and np.all(np.isfinite(kg))
+    scale = np.max(np.abs(ko), axis=1, keepdims=True)
+    scale[scale == 0] = 1
+    err = np.abs(kg - ko) / scale
+    big = np.abs(ko) >= 0.1 * scale
+    rel = np.abs(kg[big] / ko[big] - 1)
+    print("%s: %s, max |dK|/rowmax %.2e, dominant entries max rel %.2e" %
+          (label, kg.shape, err.max(), rel.max()), flush=True)
+    assert err.max() <= 2e-5
+    assert rel.max() <= 1e-4
+    # structural zeros agree exactly (levels a ray never reaches)
+    assert np.array_equal(kg == 0, ko == 0) or np.sum((kg == 0) != (ko == 0)) <= 0.01 * kg.size
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_kernel_vs_oracle(glib, olib, in_examples, name):
+    case, keys = CASES[name]
+    print("case", name, flush=True)
+    kg, rg, pg = jacobian(glib, "", case, keys)
+    ko, ro, po = jacobian(olib, "oracle_", case, keys)
+    compare(kg, ko, name)
+    # side effects of the unperturbed formod: radiances and hydrostatic p
+    if "WRITE_BBT" in keys:
+        np.testing.assert_allclose(rg, ro, atol=0.01)
+    else:
+        np.testing.assert_allclose(rg, ro, rtol=1e-4)
+    np.testing.assert_allclose(pg, po, rtol=1e-12)
+
+
+@pytest.mark.parametrize("case,tol", [("clear", 2e-6), ("ext1", 2e-6), ("aero0", 1e-4)])
+def test_fp64_formod_vs_oracle(glib, olib, in_examples, case, tol):
+    """The FP64 path integration (option fp64) against the oracle: only the
+    FP32 storage of tables, point records and the Planck table differs."""
+    ctl, obs, atm, aero = read_example(glib, "", case)
+    glib.jur_set_option(b"fp64", 1)
+    try:
+        glib.formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero))
+    finally:
+        glib.jur_set_option(b"fp64", 0)
+    octl, oobs, oatm, oaero = read_example(olib, "oracle_", case)
+    olib.oracle_formod(C.byref(octl), C.byref(oatm), C.byref(oobs), C.byref(oaero))
+    g = np.array([obs.rad[0][i] for i in range(obs.nr)])
+    o = np.array([oobs.rad[0][i] for i in range(oobs.nr)])
+    rel = np.max(np.abs(g / o - 1))
+    print("%s fp64 max rel %.2e" % (case, rel), flush=True)
+    assert rel <= tol
+
+
+def test_kernel_with_fov(glib, olib, in_examples, tmp_path):
+    fov = tmp_path / "fov.tab"
+    dz = np.linspace(-2, 2, 41)
+    fov.write_text("".join("%g %g\n" % (z, np.exp(-0.5 * (z / 0.8) ** 2)) for z in dz))
+    keys = ("FOV", str(fov), "RETT_ZMIN", 0, "RETT_ZMAX", 40)
+    kg, _, _ = jacobian(glib, "", "clear", keys)
+    ko, _, _ = jacobian(olib, "oracle_", "clear", keys)
+    compare(kg, ko, "fov")
+
+
+def test_grouped_calls_match(glib, in_examples):
+    """Perturbed copies split over many batched calls give the same matrix."""
+    k1, _, _ = jacobian(glib, "", "clear", TQ)
+    glib.jur_set_option(b"kernel_rays", 40)
+    try:
+        k2, _, _ = jacobian(glib, "", "clear", TQ)
+    finally:
+        glib.jur_set_option(b"kernel_rays", 1 << 17)
+    d = np.abs(k1 - k2).max() / np.abs(k1).max()
+    print("grouped vs single call: max diff / max |K| %.2e" % d, flush=True)
+    np.testing.assert_array_equal(k1, k2)
+
+
+def test_gsl_matrix_entry_point(glib, in_examples):
+    ctl, obs, atm, aero = read_example(glib, "", "clear", ("RETT_ZMIN", 0, "RETT_ZMAX", 40))
+    r = C.byref
+    n = glib.jur_state_size(r(ctl), r(atm), r(aero))
+    m = glib.jur_meas_size(r(ctl), r(obs))
+    tda = n + 3  # a row stride wider than the matrix, as for a gsl submatrix view
+    buf = np.full((m, tda), -7.0)
+    mat = L.jur_gsl_matrix(m, n, tda, buf.ctypes.data, None, 0)
+    glib.kernel(r(ctl), r(atm), r(obs), r(aero), r(mat))
+    k, _, _ = jacobian(glib, "", "clear", ("RETT_ZMIN", 0, "RETT_ZMAX", 40))
+    np.testing.assert_array_equal(buf[:, :n], k)
+    assert np.all(buf[:, n:] == -7.0)
+
+
+def test_size_mismatch_exits(in_examples):
+    code = (
+        "import ctypes as C, numpy as np, sys; sys.path[:0] = %r\n"
+        "from pyjurassic import lib as L; from helpers import read_example\n"
+        "lib = L.load()\n"
+        "ctl, obs, atm, aero = read_example(lib, '', 'clear', ('RETT_ZMIN', 0, 'RETT_ZMAX', 5))\n"
+        "k = np.zeros((11, 3))\n"
+        "lib.jur_kernel(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero),"
+        " k.ctypes.data_as(C.POINTER(C.c_double)), 11, 3, 3)\n" % (sys.path[:4],))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "Matrix size" in r.stdout
