@@ -250,6 +250,132 @@ def bench_surrogate(args, lib):
     print(json.dumps(line), flush=True)
 
 
+JACOBIAN_TH = np.arange(6.0, 30.01, 1.5)  # 17 limb tangent heights [km]
+
+
+def jacobian_inputs(lib, prefix, w, ctlpath, keys):
+    """ctl / atm / obs of the Jacobian workload read or filled for `lib`."""
+    from pyjurassic import jtypes as J, synth
+    ctl = J.ctl_t()
+    argc, argv = J.argv(ctlpath, *keys)
+    quiet(lambda: getattr(lib, prefix + "read_ctl")(argc, argv, C.byref(ctl)))
+    atm, obs = J.atm_t(), J.obs_t()
+    atm.np = w.nlev
+    for l in range(w.nlev):
+        atm.z[l], atm.p[l], atm.t[l] = w.z[0, l], w.p[0, l], w.t[0, l]
+        for g in range(ctl.ng):
+            atm.q[g][l] = w.q[0, g, l]
+    g = synth._rays_limb(JACOBIAN_TH)
+    obs.nr = g.shape[1]
+    for ir in range(obs.nr):
+        (obs.time[ir], obs.obsz[ir], obs.obslon[ir], obs.obslat[ir], obs.vpz[ir], obs.vplon[ir],
+         obs.vplat[ir]) = g[:, ir]
+    return ctl, atm, obs
+
+
+def bench_jacobian(args, lib):
+    """kernel(), the finite-difference Jacobian (reference
+    src/retrievalmodel.c:201-298): a 17-ray limb scan (tangent heights 6-30 km,
+    observer 800 km) through one atmosphere of the 65,536-ray workload (121
+    levels, 5 emitters, 64 channels); state vector temperature and the five
+    volume mixing ratios on all 121 levels (726 elements: 727 forward models
+    of 17 rays, integrated in FP64).  The CPU leg runs the oracle's kernel()
+    on the same inputs; `accuracy` compares the two matrices."""
+    import torch
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyjurassic import synth
+    tmp = Path(tempfile.mkdtemp(prefix="jur_jac_"))
+    w = synth.make("clear_batch_65536", tmp, nray=1)
+    ctlpath = tmp / "jac.ctl"
+    w.write_ctl(ctlpath)
+    ng = len(w.emitters)
+    keys = ["RETT_ZMIN", 0, "RETT_ZMAX", 120]
+    for g in range(ng):
+        keys += ["RETQ_ZMIN[%d]" % g, 0, "RETQ_ZMAX[%d]" % g, 120]
+    ctl, atm, obs = jacobian_inputs(lib, "", w, ctlpath, keys)
+    r = C.byref
+    n = lib.jur_state_size(r(ctl), r(atm), None)
+    m = lib.jur_meas_size(r(ctl), r(obs))
+    k = np.zeros((m, n))
+    kp = k.ctypes.data_as(C.POINTER(C.c_double))
+
+    def step():
+        lib.jur_kernel(r(ctl), r(atm), r(obs), None, kp, m, n, n)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    steps = args.steps
+    path_ms, ipts = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        path_ms.append(lib.jur_stat(b"path_ms"))
+        ipts.append(lib.jur_stat(b"integrated_points"))
+    dt = time.perf_counter() - t0
+    nd, nr = ctl.nd, obs.nr
+    rays = (n + 1) * nr
+    # the last batched call holds every perturbed copy (one group)
+    alg_bytes = ipts[-1] * nd * (128 * ng + 8) + lib.jur_stat(b"pencils") * nd * 8
+    path_avg = float(np.mean(path_ms))
+    achieved = alg_bytes / (path_avg * 1e-3) / 1e9
+    line = {
+        "metric": "Jacobian: ray x channel radiances/s over the n+1 forward models of kernel()",
+        "value": rays * nd * steps / dt,
+        "unit": "ray*channel/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (path integration of the perturbed states), f64 (ray geometry)",
+        "data": "synthetic (seeded generator, reference file formats)",
+        "config": {"workload": "jacobian_limb_17", "rays": nr, "channels": nd, "emitters": ng,
+                   "levels": w.nlev, "state_elements": n, "measurements": m,
+                   "forward_rays_per_kernel": rays, "columns_per_s": n * steps / dt,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_path64",
+                     "kernel_ms": path_avg, "alg_bytes_per_call": alg_bytes},
+    }
+    if not args.no_cpu:
+        import oracle_ctypes
+        olib = oracle_ctypes.load()
+        octl, oatm, oobs = jacobian_inputs(olib, "oracle_", w, ctlpath, keys)
+        # bounded sample: every 4th column (the unperturbed forward model and
+        # 182 perturbed ones)
+        cols = np.arange(0, n, 4, dtype=np.int32)
+        ko = np.zeros((m, n))
+        c0 = time.perf_counter()
+        quiet(lambda: olib.oracle_kernel_columns(r(octl), r(oatm), r(oobs), None,
+                                                 ko.ctypes.data_as(C.POINTER(C.c_double)), m, n, n,
+                                                 cols.ctypes.data_as(C.POINTER(C.c_int)), len(cols)))
+        cdt = time.perf_counter() - c0
+        ks, kos = k[:, cols], ko[:, cols]
+        # perturbation sizes and unperturbed radiances (oracle): |dK| h / |y|
+        # is the disagreement of the perturbed forward models themselves
+        x = np.zeros(n)
+        iqa = np.zeros(n, dtype=np.int32)
+        olib.oracle_atm2x(r(octl), r(oatm), None, x.ctypes.data_as(C.POINTER(C.c_double)),
+                          iqa.ctypes.data_as(C.POINTER(C.c_int)), None)
+        h = np.array([olib.oracle_perturbation(r(octl), int(iqa[j]), x[j]) for j in cols])
+        y = np.array([[oobs.rad[id][ir] for id in range(nd)] for ir in range(nr)]).ravel()
+        dy_err = np.abs(ks - kos) * h[None, :] / np.abs(y)[:, None]
+        dy_sig = np.abs(kos) * h[None, :] / np.abs(y)[:, None]
+        res = dy_sig >= 1e-8
+        line["cpu_baseline"] = {"value": (len(cols) + 1) * nr * nd / cdt, "unit": "ray*channel/s",
+                                "cores": ncores(), "kind": "port",
+                                "sample": "%d of the %d Jacobian columns (every 4th) plus the unperturbed "
+                                          "forward model, %d rays x %d channels each, FP64 oracle kernel(), "
+                                          "%.1f s wall" % (len(cols), n, nr, nd, cdt)}
+        line["accuracy"] = {
+            "max_rel_perturbed_radiance": float(dy_err.max()),
+            "max_rel_K_resolved": float(np.max(np.abs(ks[res] / kos[res] - 1))),
+            "resolved_entries": int(res.sum()), "entries_compared": int(res.size),
+            "resolved": "entries whose radiance change exceeds 1e-8 of the radiance; below that the "
+                        "finite difference approaches FP64 rounding in the reference too",
+            "columns_compared": len(cols)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -262,6 +388,8 @@ def main():
     lib = L.load()
     if args.workload == "cirrus_16384":
         return bench_surrogate(args, lib)
+    if args.workload == "jacobian_limb_17":
+        return bench_jacobian(args, lib)
     w, ctl, aero, scattering, (argc, argv) = setup(lib, args.workload, args.nray)
     aero_p = C.byref(aero) if aero is not None else None
     if args.variant is not None:

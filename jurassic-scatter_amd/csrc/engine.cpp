@@ -23,6 +23,7 @@ struct Engine {
   std::string tbl_key, chan_key;
   int ng = 0, nd = 0;
   DBuf t_np, t_p, t_nt, t_t, t_ncol, t_col, t_ue, planck, chan, nu_d;
+  DBuf t_p64, t_t64, planck64, chan64;  // FP64 grids, Planck table, channel data
   DBuf g_np, g_p, g_nt, g_t, t_meta;
   int shared = 0;
   unsigned ue_bytes = 0, meta_bytes = 0;
@@ -146,6 +147,59 @@ ChanConst make_chan_const(const ctl_t* ctl, int id) {
   return c;
 }
 
+// The channel's continuum data in FP64 (continua.c:854-869, 1898-1928,
+// 1980-1992, 2037-2048), for the FP64 path.
+ChanConst64 make_chan_const64(const ctl_t* ctl, int id) {
+  ChanConst64 c;
+  memset(&c, 0, sizeof c);
+  const double nu = ctl->nu[id];
+  c.nu = nu;
+  c.window = ctl->window[id];
+  int ig_co2 = find_emitter(ctl, "CO2"), ig_h2o = find_emitter(ctl, "H2O");
+  double xw = nu / 2 + 1;
+  if (ctl->ctm_co2 && ig_co2 >= 0 && xw >= 1 && xw < 2001) {
+    int iw = (int)xw, jw = iw + 1;
+    double dw = xw - iw, ew = 1 - dw;
+    c.co2_on = 1;
+    c.co2_ig = ig_co2;
+    c.co2_cw296 = ew * JUR_CTM_CO2_296[iw - 1] + dw * JUR_CTM_CO2_296[jw - 1];
+    c.co2_cw260 = ew * JUR_CTM_CO2_260[iw - 1] + dw * JUR_CTM_CO2_260[jw - 1];
+    c.co2_cw230 = ew * JUR_CTM_CO2_230[iw - 1] + dw * JUR_CTM_CO2_230[jw - 1];
+  }
+  xw = nu / 10 + 1;
+  if (ctl->ctm_h2o && ig_h2o >= 0 && xw >= 1 && xw < 2001) {
+    int iw = (int)xw, jw = iw + 1;
+    double dw = xw - iw, ew = 1 - dw, sfac;
+    c.h2o_cw296 = ew * JUR_CTM_H2O_SELF296[iw - 1] + dw * JUR_CTM_H2O_SELF296[jw - 1];
+    c.h2o_cw260 = ew * JUR_CTM_H2O_SELF260[iw - 1] + dw * JUR_CTM_H2O_SELF260[jw - 1];
+    double cwfrn = ew * JUR_CTM_H2O_FOREIGN[iw - 1] + dw * JUR_CTM_H2O_FOREIGN[jw - 1];
+    if (nu <= 820 || nu >= 960) {
+      sfac = 1;
+    } else {
+      double xx = (nu - 820) / 10;
+      int ix = (int)xx;
+      double dx = xx - ix;
+      sfac = (1 - dx) * JUR_CTM_H2O_XFCREV[ix] + dx * JUR_CTM_H2O_XFCREV[ix + 1];
+    }
+    double vf2 = (nu - 370) * (nu - 370), vf6 = vf2 * vf2 * vf2;
+    double fscal = 36100 / (vf2 + vf6 * 1e-8 + 36100) * -.25 + 1;
+    c.h2o_on = 1;
+    c.h2o_ig = ig_h2o;
+    c.h2o_sfac = sfac;
+    c.h2o_ctwfrn = cwfrn * fscal;
+  }
+  auto band = [&](const double* nua, const double* ba, const double* bb, int n, double& b, double& beta) {
+    if (nu < nua[0] || nu > nua[n - 1]) return 0;
+    int idx = locate_mono<double>(nua, n, nu);
+    b = lin(nua[idx], ba[idx], nua[idx + 1], ba[idx + 1], nu);
+    beta = lin(nua[idx], bb[idx], nua[idx + 1], bb[idx + 1], nu);
+    return 1;
+  };
+  if (ctl->ctm_n2) c.n2_on = band(JUR_CTM_N2_NU, JUR_CTM_N2_B, JUR_CTM_N2_BETA, 98, c.n2_b, c.n2_beta);
+  if (ctl->ctm_o2) c.o2_on = band(JUR_CTM_O2_NU, JUR_CTM_O2_B, JUR_CTM_O2_BETA, 90, c.o2_b, c.o2_beta);
+  return c;
+}
+
 hipStream_t engine_stream() { return E().stream; }
 
 int engine_option_int(const char* key, int dflt) {
@@ -177,6 +231,7 @@ void engine_prepare(const ctl_t* ctl) {
     std::vector<int> np(ntbl, -1), nt((size_t)ntbl * TBLNPMAX, 0), ncol((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0),
         col((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0);
     std::vector<float> p((size_t)ntbl * TBLNPMAX, 0.f), t((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0.f);
+    std::vector<double> p64((size_t)ntbl * TBLNPMAX, 0.0), t64((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0.0);
     std::vector<float2> ue;
     static char fn[2 * LEN + 64];
     for (int id = 0; id < nd; id++)
@@ -194,10 +249,12 @@ void engine_prepare(const ctl_t* ctl) {
         np[k] = tb.np;
         for (int ip = 0; ip < tb.np; ip++) {
           p[(size_t)k * TBLNPMAX + ip] = (float)tb.p[ip];
+          p64[(size_t)k * TBLNPMAX + ip] = tb.p[ip];
           nt[(size_t)k * TBLNPMAX + ip] = tb.nt[ip];
           for (int it = 0; it < tb.nt[ip]; it++) {
             size_t c = ((size_t)k * TBLNPMAX + ip) * TBLNTMAX + it;
             t[c] = (float)tb.t[ip][it];
+            t64[c] = tb.t[ip][it];
             ncol[c] = tb.nu[ip][it];
             col[c] = (int)ue.size();
             for (int iu = 0; iu < tb.nu[ip][it]; iu++) ue.push_back(make_float2(tb.u[ip][it][iu], tb.eps[ip][it][iu]));
@@ -257,6 +314,8 @@ void engine_prepare(const ctl_t* ctl) {
     upload(e.t_p, p, s);
     upload(e.t_nt, nt, s);
     upload(e.t_t, t, s);
+    upload(e.t_p64, p64, s);
+    upload(e.t_t64, t64, s);
     upload(e.t_ncol, ncol, s);
     upload(e.t_col, col, s);
     upload(e.t_ue, ue, s);
@@ -279,8 +338,9 @@ void engine_prepare(const ctl_t* ctl) {
     upload(dfoff, foff, s);
     upload(dfn, fnn, s);
     e.planck.need((size_t)nd * JUR_NPLANCK * sizeof(float));
-    launch_planck_table(dfnu.as<double>(), dfw.as<double>(), dfoff.as<int>(), dfn.as<int>(), nullptr, nd,
-                        e.planck.as<float>(), s);
+    e.planck64.need((size_t)nd * JUR_NPLANCK * sizeof(double));
+    launch_planck_table(dfnu.as<double>(), dfw.as<double>(), dfoff.as<int>(), dfn.as<int>(), nd,
+                        e.planck.as<float>(), e.planck64.as<double>(), s);
     HIP_OK(hipStreamSynchronize(s));
     dfnu.release();
     dfw.release();
@@ -299,12 +359,15 @@ void engine_prepare(const ctl_t* ctl) {
   for (int id = 0; id < ctl->nd; id++) ck += std::to_string(ctl->window[id]) + ",";
   if (ck != e.chan_key) {
     std::vector<ChanConst> cc(ctl->nd);
+    std::vector<ChanConst64> cc64(ctl->nd);
     std::vector<double> nu(ctl->nd);
     for (int id = 0; id < ctl->nd; id++) {
       cc[id] = make_chan_const(ctl, id);
+      cc64[id] = make_chan_const64(ctl, id);
       nu[id] = ctl->nu[id];
     }
     upload(e.chan, cc, s);
+    upload(e.chan64, cc64, s);
     upload(e.nu_d, nu, s);
     HIP_OK(hipStreamSynchronize(s));
     e.chan_key = ck;
@@ -671,6 +734,8 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   if (engine_option_int("fast", 1) == 0) P.T.shared = 0;
   P.cc = e.chan.as<ChanConst>();
   P.planck = e.planck.as<float>();
+  P.cc64 = e.chan64.as<ChanConst64>();
+  P.planck64 = e.planck64.as<double>();
   P.beta_e = e.beta_e.as<float>();
   P.beta_s = e.beta_s.as<float>();
   P.beta_a = e.beta_a.as<float>();
@@ -801,6 +866,8 @@ DevTables engine_tables() {
   T.gt = e.g_t.as<float>();
   T.meta = e.t_meta.as<int2>();
   T.meta_bytes = e.meta_bytes;
+  T.p64 = e.t_p64.as<double>();
+  T.t64 = e.t_t64.as<double>();
   return T;
 }
 const double* engine_nu_dev() { return E().nu_d.as<double>(); }

@@ -38,6 +38,9 @@ struct DevTables {
   const float* gt;     // [ng][TBLNPMAX][TBLNTMAX]
   const int2* meta;    // [ntbl][TBLNPMAX][TBLNTMAX] = (ncol, col)
   unsigned meta_bytes;
+  // the pressure and temperature grids as read (FP64), for the FP64 path
+  const double* p64;   // [ntbl][TBLNPMAX]
+  const double* t64;   // [ntbl][TBLNPMAX][TBLNTMAX]
 };
 
 // Per-channel constants of the continua and the channel's window (filled on
@@ -60,6 +63,21 @@ struct ChanConst {
   float n2_b, n2_beta;
   int o2_on;
   float o2_b, o2_beta;
+};
+
+// The same channel data in FP64 for the FP64 path, as the continuum
+// functions form them (continua.c:854-869, 1898-1928, 1980-1992, 2037-2048).
+struct ChanConst64 {
+  double nu;
+  int window;
+  int co2_on, co2_ig;
+  double co2_cw230, co2_cw260, co2_cw296;
+  int h2o_on, h2o_ig;
+  double h2o_sfac, h2o_cw296, h2o_cw260, h2o_ctwfrn;
+  int n2_on;
+  double n2_b, n2_beta;
+  int o2_on;
+  double o2_b, o2_beta;
 };
 
 // ---------------------------------------------------------------------------

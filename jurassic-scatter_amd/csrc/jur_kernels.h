@@ -63,6 +63,8 @@ struct PathArgs {
   double *rad64, *tau64;   // [nd][out_ld] (optional)
   const double* rec64;     // FP64 records / surface temperatures (optional, fp64)
   const double* tsurf64;
+  const ChanConst64* cc64; // [nd] (fp64)
+  const double* planck64;  // [nd][JUR_NPLANCK] (fp64)
 };
 int path_variants();
 // Launch the path integration with the given waves per block.
@@ -70,8 +72,7 @@ void launch_path(const PathArgs& A, int waves_per_block, hipStream_t s);
 
 // Filter-weighted Planck table [nd][JUR_NPLANCK] (forwardmodel.c:743-786).
 void launch_planck_table(const double* filt_nu, const double* filt_w, const int* filt_off,
-                         const int* filt_n, const double* nu_unused, int nd, float* table,
-                         hipStream_t s);
+                         const int* filt_n, int nd, float* table, double* table64, hipStream_t s);
 
 // Sort ray ids by descending point count (stable).  temp sized by
 // sort_temp_bytes.

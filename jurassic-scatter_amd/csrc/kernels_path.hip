@@ -585,7 +585,7 @@ __device__ inline double expm1_gsl(double x) {
 }
 
 __global__ void k_planck_table(const double* fnu, const double* fw, const int* foff, const int* fn,
-                               int nd, float* table) {
+                               int nd, float* table, double* table64) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nd * JUR_NPLANCK) return;
   int id = idx / JUR_NPLANCK, it = idx % JUR_NPLANCK;
@@ -599,6 +599,7 @@ __global__ void k_planck_table(const double* fnu, const double* fw, const int* f
     acc += w[i] * (C1 * (v * v * v) / expm1_gsl(C2 * v / temp));
   }
   table[(size_t)id * JUR_NPLANCK + it] = (float)(acc / fsum);
+  table64[(size_t)id * JUR_NPLANCK + it] = acc / fsum;
 }
 
 __global__ void k_iota(int* ids, int n) {
@@ -613,10 +614,12 @@ __global__ void k_zero_int(int* p, int n) {
 // ---------------------------------------------------------------------------
 // FP64 path integration: the reference's arithmetic and operation order
 // (forwardmodel.c:303-399 with intpol_tbl :409-502 and formod_continua
-// :120-152) in double precision on the same FP32 tables, records and
-// per-channel constants.  Used where radiance DIFFERENCES matter: the
-// finite-difference Jacobian perturbs one level by 1 % and its radiance
-// change can be far below FP32 resolution.  One channel per wave.
+// :120-152) in double precision, on the FP32 (u, eps) columns the reference
+// also stores in single precision and on FP64 copies of everything it keeps
+// in double (table grids, Planck table, continuum coefficients, point
+// records).  Used where radiance DIFFERENCES matter: the finite-difference
+// Jacobian perturbs one level by 1 % and its radiance change can be far
+// below FP32 resolution.  One channel per wave.
 // ---------------------------------------------------------------------------
 
 // LIN on float table nodes as the reference's macro evaluates it: under C's
@@ -642,22 +645,7 @@ __device__ inline int locate_col64(const float2* c, int n, double x) {
   }
   return ilo;
 }
-// locate (misc.c:237-265) on a float grid, double key.
-__device__ inline int locate_grid64(const float* xx, int n, double x) {
-  int ilo = 0, ihi = n - 1, i = (ihi + ilo) >> 1;
-  if (xx[i] < xx[i + 1]) {
-    while (ihi > ilo + 1) {
-      i = (ihi + ilo) >> 1;
-      if ((double)xx[i] > x) ihi = i; else ilo = i;
-    }
-  } else {
-    while (ihi > ilo + 1) {
-      i = (ihi + ilo) >> 1;
-      if ((double)xx[i] <= x) ihi = i; else ilo = i;
-    }
-  }
-  return ilo;
-}
+
 // intpol_tbl_u then intpol_tbl_eps of one (p, T) corner (forwardmodel.c:460-474).
 __device__ inline double corner64(const float2* col, int n, double e_path, double du) {
   const int i = locate_col64<true>(col, n, e_path);
@@ -675,14 +663,14 @@ __device__ inline double ega64(const DevTables& T, int tb, double p, double t, d
   } else if (tau_path < 1e-9) {
     eps = 1.0;
   } else {
-    const float* pg = T.p + (size_t)tb * TBLNPMAX;
-    const int ipr = locate_grid64(pg, ntp, p);
+    const double* pg = T.p64 + (size_t)tb * TBLNPMAX;
+    const int ipr = locate_mono<double>(pg, ntp, p);
     const size_t b0 = (size_t)tb * TBLNPMAX + ipr;
     const int nt0 = T.nt[b0], nt1 = T.nt[b0 + 1];
     if (nt0 >= 2 && nt1 >= 2) {
-      const float* t0g = T.t + b0 * TBLNTMAX;
-      const float* t1g = t0g + TBLNTMAX;
-      const int it0 = locate_grid64(t0g, nt0, t), it1 = locate_grid64(t1g, nt1, t);
+      const double* t0g = T.t64 + b0 * TBLNTMAX;
+      const double* t1g = t0g + TBLNTMAX;
+      const int it0 = locate_mono<double>(t0g, nt0, t), it1 = locate_mono<double>(t1g, nt1, t);
       const size_t k00 = b0 * TBLNTMAX + it0, k10 = (b0 + 1) * TBLNTMAX + it1;
       const int n00 = T.ncol[k00], n01 = T.ncol[k00 + 1], n10 = T.ncol[k10], n11 = T.ncol[k10 + 1];
       if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
@@ -701,42 +689,47 @@ __device__ inline double ega64(const DevTables& T, int tb, double p, double t, d
   tau_path *= 1.0 - eps;
   return eps;
 }
-// Continuum extinction [1/km] (forwardmodel.c:120-152) in FP64.
-__device__ inline double continua64(const ChanConst& c, double p, double t, double ds, double qh2o,
+// Continuum extinction [1/km] (forwardmodel.c:120-152) in FP64, each term
+// in the operation order of ctmco2, ctmh2o, ctmn2, ctmo2 (continua.c:854-869,
+// 1898-1928, 1980-1992, 2037-2048).
+__device__ inline double continua64(const ChanConst64& c, double p, double t, double ds, double qh2o,
                                     double uco2, double uh2o, double kw) {
   double beta = kw;
   if (c.co2_on && ds > 0.0) {
-    const double d230 = t - 230.0, d260 = t - 260.0, d296 = t - 296.0;
-    const double ctw = d260 * d296 * c.co2_k230 - d230 * d296 * c.co2_k260 + d230 * d260 * c.co2_k296;
-    beta += uco2 / (JUR_AVOGADRO * 1000.0 * P0) * p * ctw / ds;
+    const double dt230 = t - 230, dt260 = t - 260, dt296 = t - 296;
+    const double ctw = dt260 * 5.050505e-4 * dt296 * c.co2_cw230 - dt230 * 9.259259e-4 * dt296 * c.co2_cw260 +
+                       dt230 * 4.208754e-4 * dt260 * c.co2_cw296;
+    beta += uco2 / JUR_AVOGADRO / 1000 * p / P0 * ctw / ds;
   }
   if (c.h2o_on && ds > 0.0) {
-    const double ctwslf = c.h2o_s296 * exp(c.h2o_lnratio * ((296.0 - t) / 36.0));
-    const double a1 = (double)c.nu * uh2o * tanh(c.h2o_tanhk / t);
-    const double a3 = p / P0 * (qh2o * ctwslf + (1.0 - qh2o) * c.h2o_frn) * 1e-20;
-    beta += a1 * (296.0 / t) * a3 / ds;
+    const double ctwslf = c.h2o_sfac * c.h2o_cw296 * pow(c.h2o_cw260 / c.h2o_cw296, (296 - t) / (296 - 260));
+    const double a1 = c.nu * uh2o * tanh(.7193876 / t * c.nu);
+    const double a2 = 296 / t;
+    const double a3 = p / P0 * (qh2o * ctwslf + (1 - qh2o) * c.h2o_ctwfrn) * 1e-20;
+    beta += a1 * a2 * a3 / ds;
   }
-  if (c.n2_on || c.o2_on) {
-    const double pp = p / P0, tt = 273.0 / t;
-    const double base = 0.1 * pp * pp * tt * tt, ex = 1.0 / 296.0 - 1.0 / t;
-    if (c.n2_on)
-      beta += base * 0.79 * c.n2_b * exp(c.n2_beta * ex) * (0.79 + 0.21 * (1.294 - 0.4545 * t / 296.0));
-    if (c.o2_on) beta += base * 0.21 * c.o2_b * exp(c.o2_beta * ex);
+  if (c.n2_on) {
+    const double x = p / P0, y = 273 / t;
+    beta += 0.1 * (x * x) * (y * y) * 0.79 * c.n2_b * exp(c.n2_beta * (1 / 296.0 - 1 / t)) *
+            (0.79 + (1 - 0.79) * (1.294 - 0.4545 * t / 296));
+  }
+  if (c.o2_on) {
+    const double x = p / P0, y = 273 / t;
+    beta += 0.1 * (x * x) * (y * y) * 0.21 * c.o2_b * exp(c.o2_beta * (1 / 296.0 - 1 / t));
   }
   return beta;
 }
-// Filter-weighted Planck function from the table (forwardmodel.c:786-793).
-__device__ inline double planck64(const float* row, double t) {
-  double x = (t - 100.0) * 4.0;
-  int it;
-  if (!(x == x)) {
-    it = JUR_NPLANCK - 2;
-  } else {
-    x = floor(x);
-    it = x < 0.0 ? 0 : (x > (double)(JUR_NPLANCK - 2) ? JUR_NPLANCK - 2 : (int)x);
+// Filter-weighted Planck function from the FP64 table: locate on the
+// 1201-point grid 100-400 K, then LIN (forwardmodel.c:786-793).
+__device__ inline double planck64(const double* row, double t) {
+  int ilo = 0, ihi = JUR_NPLANCK - 1;
+  while (ihi > ilo + 1) {
+    const int i = (ihi + ilo) >> 1;
+    if (lin(0.0, 100.0, JUR_NPLANCK - 1.0, 400.0, (double)i) > t) ihi = i; else ilo = i;
   }
-  const double t0 = 100.0 + 0.25 * it;
-  return lin<double>(t0, row[it], t0 + 0.25, row[it + 1], t);
+  const double t0 = lin(0.0, 100.0, JUR_NPLANCK - 1.0, 400.0, (double)ilo);
+  const double t1 = lin(0.0, 100.0, JUR_NPLANCK - 1.0, 400.0, (double)(ilo + 1));
+  return lin(t0, row[ilo], t1, row[ilo + 1], t);
 }
 
 template <int NGT>
@@ -750,8 +743,8 @@ __global__ void __launch_bounds__(256) k_path64(PathArgs A) {
   const int npts = valid ? A.np[ray] : 0;
   const int id = chunk;
   const int uo = rec_u(A.nw);
-  const ChanConst& cc = A.cc[id];
-  const float* prow = A.planck + (size_t)id * JUR_NPLANCK;
+  const ChanConst64& cc = A.cc64[id];
+  const double* prow = A.planck64 + (size_t)id * JUR_NPLANCK;
   double tau_path[NGT];
 #pragma unroll
   for (int g = 0; g < NGT; g++) tau_path[g] = 1.0;
@@ -866,11 +859,11 @@ void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   else launch_t<NGMAX, 1, 2>(A, wpb, s);
 }
 
-void launch_planck_table(const double* fnu, const double* fw, const int* foff, const int* fn,
-                         const double*, int nd, float* table, hipStream_t s) {
+void launch_planck_table(const double* fnu, const double* fw, const int* foff, const int* fn, int nd,
+                         float* table, double* table64, hipStream_t s) {
   int n = nd * JUR_NPLANCK;
   hipLaunchKernelGGL(k_planck_table, dim3((n + 255) / 256), dim3(256), 0, s, fnu, fw, foff, fn, nd,
-                     table);
+                     table, table64);
 }
 
 size_t sort_temp_bytes(int n) {

@@ -1778,10 +1778,10 @@ size_t oracle_atm2x(const ctl_t *ctl, const atm_t *atm, const aero_t *aero, doub
     o_atm2x_help(atm, ctl->retk_zmin[iw], ctl->retk_zmax[iw], atm->k[iw], IDXK(iw), x, iqa, ipa, &n);
   const int on[3] = {ctl->retnn, ctl->retrr, ctl->retss};
   const int idx[3] = {IDXNN, IDXRR, IDXSS};
-  const double val[3] = {aero->nn[0], aero->rr[0], aero->ss[0]};
   for (int i = 0; i < 3; i++)
     if (on[i]) {
-      if (x) x[n] = val[i];
+      if (!aero) OFAIL("Aerosol parameters are retrieved but no aerosol data is given!");
+      if (x) x[n] = i == 0 ? aero->nn[0] : i == 1 ? aero->rr[0] : aero->ss[0];
       if (iqa) iqa[n] = idx[i];
       if (ipa) ipa[n] = -1;
       n++;
@@ -1847,6 +1847,13 @@ double oracle_perturbation(const ctl_t *ctl, int iq, double x) {
    (gsl_vector_alloc) or stale from the previous column. */
 void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m, size_t n,
                    size_t ldk) {
+  oracle_kernel_columns(ctl, atm, obs, aero, k, m, n, ldk, NULL, 0);
+}
+
+/* oracle_kernel restricted to the columns cols[0..ncols-1] (all when cols is
+   NULL); the other columns of k are zero. */
+void oracle_kernel_columns(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m,
+                           size_t n, size_t ldk, const int *cols, int ncols) {
   static atm_t atm1;
   static obs_t obs1;
   static aero_t aero1, aero_none;
@@ -1861,7 +1868,10 @@ void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, 
     yy1[i] = NAN;
     for (size_t j = 0; j < n; j++) k[i * ldk + j] = 0;
   }
-  for (size_t j = 0; j < n; j++) {
+  const size_t nc = cols ? (size_t)ncols : n;
+  for (size_t c = 0; c < nc; c++) {
+    const size_t j = cols ? (size_t)cols[c] : c;
+    if (j >= n) OFAIL("Column index out of range!");
     double h = oracle_perturbation(ctl, iqa[j], x0[j]);
     if (h <= 0) OFAIL("Cannot set perturbation size!");
     memcpy(x1, x0, n * sizeof(double));

@@ -111,6 +111,8 @@ size_t oracle_obs2y(const ctl_t *ctl, const obs_t *obs, double *y, int *ida, int
 double oracle_perturbation(const ctl_t *ctl, int iq, double x);
 void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m, size_t n,
                    size_t ldk);
+void oracle_kernel_columns(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m,
+                           size_t n, size_t ldk, const int *cols, int ncols);
 
 /* Drop the cached tables, filter functions and Planck tables. */
 void oracle_reset(void);

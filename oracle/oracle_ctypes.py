@@ -48,6 +48,8 @@ def load():
         "oracle_obs2y": (C.c_size_t, [P, P, D, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "oracle_perturbation": (C.c_double, [P, C.c_int, C.c_double]),
         "oracle_kernel": (None, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t]),
+        "oracle_kernel_columns": (None, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t,
+                                         C.POINTER(C.c_int), C.c_int]),
         "oracle_reset": (None, []),
     }
     for name, (res, args) in sig.items():

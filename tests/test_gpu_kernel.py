@@ -7,10 +7,12 @@ view (convolution within each perturbed copy's rays).
 
 The perturbed paths are integrated in FP64 (option kernel_fp64, default):
 a 1 % perturbation of one level can change a radiance by less than FP32
-resolves.  Bar: every entry within 2e-5 of its row's largest |K| (a Jacobian
-row mixes columns of very different size), and the rows' dominant entries
-(>= 10 % of the row maximum) within 1e-4 relative; what remains is the FP32
-storage of the tables and the Planck table.
+resolves.  Bar per case: every entry within `tol` of its row's largest |K|
+(a Jacobian row mixes columns of very different size) and the rows' dominant
+entries (>= 10 % of the row maximum) within 10 tol relative.  The FP64 path
+follows the reference's operation order on FP64 copies of everything the
+reference keeps in double, so clear-air cases agree to ~1e-9; the
+scattering case carries the FP32 angular integral of the source.
 """
 import ctypes as C
 import subprocess
@@ -29,11 +31,11 @@ TQ = ("RETT_ZMIN", 0, "RETT_ZMAX", 80, "RETQ_ZMIN[0]", 0, "RETQ_ZMAX[0]", 80,
       "RETQ_ZMIN[1]", 0, "RETQ_ZMAX[1]", 30, "RETQ_ZMIN[2]", 10, "RETQ_ZMAX[2]", 50,
       "RETP_ZMIN", 10, "RETP_ZMAX", 20)
 CASES = {
-    "clear_t_p_q": ("clear", TQ),
+    "clear_t_p_q": ("clear", TQ, 1e-8),
     "clear_hydz_bbt": ("clear", ("HYDZ", 15, "WRITE_BBT", 1, "RETP_ZMIN", 10, "RETP_ZMAX", 20,
-                                 "RETT_ZMIN", 0, "RETT_ZMAX", 40, "RETQ_ZMIN[1]", 0, "RETQ_ZMAX[1]", 20)),
-    "ext1_aerosol": ("ext1", ("RETT_ZMIN", 0, "RETT_ZMAX", 30, "RETNN", 1, "RETRR", 1, "RETSS", 1)),
-    "aero0_scattering": ("aero0", ("RETT_ZMIN", 0, "RETT_ZMAX", 20)),
+                                 "RETT_ZMIN", 0, "RETT_ZMAX", 40, "RETQ_ZMIN[1]", 0, "RETQ_ZMAX[1]", 20), 1e-5),
+    "ext1_aerosol": ("ext1", ("RETT_ZMIN", 0, "RETT_ZMAX", 30, "RETNN", 1, "RETRR", 1, "RETSS", 1), 1e-6),
+    "aero0_scattering": ("aero0", ("RETT_ZMIN", 0, "RETT_ZMAX", 20), 5e-5),
 }
 
 
@@ -69,7 +71,7 @@ def jacobian(lib, prefix, case, keys):
     return k, rad, p
 
 
-def compare(kg, ko, label):
+def compare(kg, ko, label, tol):
     assert kg.shape == ko.shape and np.all(np.isfinite(kg))
     scale = np.max(np.abs(ko), axis=1, keepdims=True)
     scale[scale == 0] = 1
@@ -78,19 +80,19 @@ def compare(kg, ko, label):
     rel = np.abs(kg[big] / ko[big] - 1)
     print("%s: %s, max |dK|/rowmax %.2e, dominant entries max rel %.2e" %
           (label, kg.shape, err.max(), rel.max()), flush=True)
-    assert err.max() <= 2e-5
-    assert rel.max() <= 1e-4
+    assert err.max() <= tol
+    assert rel.max() <= 10 * tol
     # structural zeros agree exactly (levels a ray never reaches)
     assert np.array_equal(kg == 0, ko == 0) or np.sum((kg == 0) != (ko == 0)) <= 0.01 * kg.size
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_kernel_vs_oracle(glib, olib, in_examples, name):
-    case, keys = CASES[name]
+    case, keys, tol = CASES[name]
     print("case", name, flush=True)
     kg, rg, pg = jacobian(glib, "", case, keys)
     ko, ro, po = jacobian(olib, "oracle_", case, keys)
-    compare(kg, ko, name)
+    compare(kg, ko, name, tol)
     # side effects of the unperturbed formod: radiances and hydrostatic p
     if "WRITE_BBT" in keys:
         np.testing.assert_allclose(rg, ro, atol=0.01)
@@ -99,10 +101,12 @@ def test_kernel_vs_oracle(glib, olib, in_examples, name):
     np.testing.assert_allclose(pg, po, rtol=1e-12)
 
 
-@pytest.mark.parametrize("case,tol", [("clear", 2e-6), ("ext1", 2e-6), ("aero0", 1e-4)])
+@pytest.mark.parametrize("case,tol", [("clear", 1e-11), ("ext1", 1e-10), ("aero0", 2e-7)])
 def test_fp64_formod_vs_oracle(glib, olib, in_examples, case, tol):
-    """The FP64 path integration (option fp64) against the oracle: only the
-    FP32 storage of tables, point records and the Planck table differs."""
+    """The FP64 path integration (option fp64) against the oracle: the same
+    operation order on the same data; what remains are the device's
+    transcendental functions, the FP32 layer optics and (with scattering)
+    the FP32 angular integral of the source."""
     ctl, obs, atm, aero = read_example(glib, "", case)
     glib.jur_set_option(b"fp64", 1)
     try:
@@ -125,7 +129,7 @@ def test_kernel_with_fov(glib, olib, in_examples, tmp_path):
     keys = ("FOV", str(fov), "RETT_ZMIN", 0, "RETT_ZMAX", 40)
     kg, _, _ = jacobian(glib, "", "clear", keys)
     ko, _, _ = jacobian(olib, "oracle_", "clear", keys)
-    compare(kg, ko, "fov")
+    compare(kg, ko, "fov", 1e-8)
 
 
 def test_grouped_calls_match(glib, in_examples):

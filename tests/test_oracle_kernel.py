@@ -132,7 +132,6 @@ def test_first_order_linearity(olib, in_examples):
     y1, _ = forward(olib, ctl, a, obs, aero)
     lin = k.sum(axis=1)
     np.testing.assert_allclose(lin, y1 - y0, rtol=0.05)
-    # and half the step gives about half the change (first order)
     assert np.all(np.abs(lin) > 0)
 
 
@@ -174,3 +173,17 @@ def test_size_mismatch_exits(olib, in_examples, tmp_path):
         " k.ctypes.data_as(C.POINTER(C.c_double)), 11, 3, 3)\n" % (sys.path[:4],))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True)
     assert r.returncode != 0 and "Matrix size" in r.stdout
+
+
+def test_column_subset(olib, in_examples):
+    ctl, obs, atm, aero = read_example(olib, "oracle_", "clear", ("RETT_ZMIN", 0, "RETT_ZMAX", 40))
+    k = jacobian(olib, ctl, atm, obs, aero)
+    ctl, obs, atm, aero = read_example(olib, "oracle_", "clear", ("RETT_ZMIN", 0, "RETT_ZMAX", 40))
+    m, n = k.shape
+    cols = np.array([0, 7, 19, n - 1], dtype=np.int32)
+    ks = np.full((m, n), np.nan)
+    olib.oracle_kernel_columns(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero), _dp(ks), m, n, n,
+                               cols.ctypes.data_as(C.POINTER(C.c_int)), len(cols))
+    np.testing.assert_array_equal(ks[:, cols], k[:, cols])
+    rest = np.setdiff1d(np.arange(n), cols)
+    assert np.all(ks[:, rest] == 0)
