@@ -87,9 +87,13 @@ T* upload(DBuf& b, const std::vector<T>& h, hipStream_t s) {
 // increased by step[c] (elem < 0 or an aerosol element: the atmosphere
 // unchanged).  Returns the FP64 radiances [nd][C * nr] after formod's
 // epilogue, read back to y.
+struct Copy0 {
+  std::vector<double> tau;  // [nd][nr]
+  std::vector<double> tp;   // [3][nr] tangent point z, lon, lat
+};
 void run_copies(ctl_t* ctl, const atm_t* atm, const obs_t* obs, const aero_t* aero, const StateVec& sv,
                 const std::vector<long>& elem, const std::vector<double>& step, std::vector<double>& y,
-                hipStream_t s) {
+                hipStream_t s, Copy0* c0 = nullptr) {
   JacBufs& b = B();
   const int C = (int)elem.size(), nl = atm->np, nr = obs->nr, ng = ctl->ng, nw = ctl->nw, nd = ctl->nd;
   const size_t N = (size_t)C * nr;
@@ -187,6 +191,16 @@ void run_copies(ctl_t* ctl, const atm_t* atm, const obs_t* obs, const aero_t* ae
   formod_epilogue(ctl, (int)N, nr, R.time, O, &rad, &tau, s);
   y.resize(nd * N);
   HIP_OK(hipMemcpyAsync(y.data(), rad, nd * N * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (c0) {
+    c0->tau.resize((size_t)nd * nr);
+    c0->tp.resize((size_t)3 * nr);
+    for (int id = 0; id < nd; id++)
+      HIP_OK(hipMemcpyAsync(&c0->tau[(size_t)id * nr], tau + (size_t)id * N, nr * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&c0->tp[0], O.tpz, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&c0->tp[nr], O.tplon, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&c0->tp[2 * (size_t)nr], O.tplat, nr * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
   HIP_OK(hipStreamSynchronize(s));
 }
 
@@ -205,44 +219,89 @@ extern "C" size_t jur_meas_size(ctl_t* ctl, obs_t* obs) { return meas_count(ctl,
 extern "C" int jur_kernel(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, double* k, size_t m, size_t n,
                           size_t ldk) {
   static aero_t none, aero1;
+  static int mask[NDMAX][NRMAX];
   if (!aero) aero = &none;
   auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = engine_stream();
+  const int nr = obs->nr, nd = ctl->nd;
   // Path integration in FP64 for the perturbed calls (option "kernel_fp64",
   // default 1): a 1 % perturbation of one level can change a radiance by
   // less than FP32 resolves.
   const double fp64_saved = engine_option_double("fp64", 0);
   const bool fp64 = engine_option_int("kernel_fp64", 1) != 0;
-  // the unperturbed forward model: fills obs, applies hydrostatic balance to
-  // atm and refreshes retrieved aerosol optics, as the reference's first call
-  formod(ctl, atm, obs, aero);
+  // The reference starts with formod on the unperturbed state (filling obs,
+  // applying hydrostatic balance to atm, refreshing retrieved aerosol
+  // optics).  Here the unperturbed state is copy 0 of the first batched
+  // call, with formod's side effects applied around it; formod itself runs
+  // when there is no state vector or a work queue is set (its item count is
+  // checked per call).
+  const bool direct = n == 0 || nr <= 0 || ctl->queue.capacity > 0;
+  if (direct) {
+    formod(ctl, atm, obs, aero);
+  } else {
+    for (int id = 0; id < nd; id++)
+      for (int ir = 0; ir < nr; ir++) mask[id][ir] = !std::isfinite(obs->rad[id][ir]);
+    hydrostatic(ctl, atm);
+    if (ctl->retnn || ctl->retrr || ctl->retss) get_opt_prop(ctl, aero);
+    ctl->queue.state = Queue_Inactive;
+  }
   StateVec sv;
   state_vector(ctl, atm, aero, sv);
-  if (sv.n != n || meas_count(ctl, obs) != m) JUR_FAIL("Matrix size does not match the state and measurement vectors!");
+  if (sv.n != n) JUR_FAIL("Matrix size does not match the state and measurement vectors!");
   if (n > 0 && ldk < n) JUR_FAIL("Leading dimension of the kernel matrix is too small!");
   std::vector<double> h(n);
   for (size_t j = 0; j < n; j++) {
     h[j] = perturbation(ctl, sv.iqa[j], sv.x[j]);
     if (h[j] <= 0) JUR_FAIL("Cannot set perturbation size!");
   }
-  for (size_t i = 0; i < m; i++)
-    for (size_t j = 0; j < n; j++) k[i * ldk + j] = 0;
-  const int nr = obs->nr, nd = ctl->nd;
-  if (n == 0 || m == 0) return 0;
+  if (direct) {
+    if (meas_count(ctl, obs) != m) JUR_FAIL("Matrix size does not match the state and measurement vectors!");
+    for (size_t i = 0; i < m; i++)
+      for (size_t j = 0; j < n; j++) k[i * ldk + j] = 0;
+    if (n == 0 || m == 0) return 0;
+  }
+  jur_set_option("fp64", fp64 ? 1 : fp64_saved);
+  size_t natm = 0;
+  while (natm < n && sv.ipa[natm] >= 0) natm++;
+  const int rays_per_call = engine_option_int("kernel_rays", kRaysPerCall);
+  const size_t per = (size_t)std::max(2, rays_per_call / std::max(nr, 1)) - 1;
   // measurement slots of obs2y (ray-major, channels whose unperturbed
   // radiance is finite; a perturbed formod masks the others)
   std::vector<int> slot_id, slot_ir;
-  for (int ir = 0; ir < nr; ir++)
+  std::vector<double> yy1(m, NAN), y0(m), y;
+  auto set_slots = [&]() {
+    slot_id.clear();
+    slot_ir.clear();
+    for (int ir = 0; ir < nr; ir++)
+      for (int id = 0; id < nd; id++)
+        if (std::isfinite(obs->rad[id][ir])) {
+          slot_id.push_back(id);
+          slot_ir.push_back(ir);
+        }
+    if (slot_id.size() != m) JUR_FAIL("Matrix size does not match the state and measurement vectors!");
+    for (size_t i = 0; i < m; i++)
+      for (size_t j = 0; j < n; j++) k[i * ldk + j] = 0;
+  };
+  if (direct) set_slots();
+  // copy 0 of the first call: formod's output, masked where the caller's
+  // radiances were not finite (forwardmodel.c:26-28, 112-115)
+  auto take_copy0 = [&](const Copy0& c0, size_t N) {
     for (int id = 0; id < nd; id++)
-      if (std::isfinite(obs->rad[id][ir])) {
-        slot_id.push_back(id);
-        slot_ir.push_back(ir);
+      for (int ir = 0; ir < nr; ir++) {
+        obs->rad[id][ir] = mask[id][ir] ? NAN : y[(size_t)id * N + ir];
+        obs->tau[id][ir] = c0.tau[(size_t)id * nr + ir];
       }
+    for (int ir = 0; ir < nr; ir++) {
+      obs->tpz[ir] = c0.tp[ir];
+      obs->tplon[ir] = c0.tp[nr + ir];
+      obs->tplat[ir] = c0.tp[2 * nr + ir];
+    }
+    set_slots();
+  };
   // obs2y of copy c of a batched call with N rays: the finite radiances among
   // the slots (a perturbed radiance that turns non-finite shifts the rest up
   // and leaves the last slots as the previous column set them, as in the
   // reference)
-  std::vector<double> yy1(m, NAN), y0(m), y;
   auto compact = [&](size_t N, size_t c, std::vector<double>& out) {
     size_t i = 0;
     for (size_t sl = 0; sl < m; sl++) {
@@ -253,14 +312,10 @@ extern "C" int jur_kernel(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, doub
   auto column = [&](size_t j, const std::vector<double>& base) {
     for (size_t r = 0; r < m; r++) k[r * ldk + j] = (yy1[r] - base[r]) / h[j];
   };
+  bool first = !direct;
   // Atmospheric elements: groups of perturbed copies, each group with its own
   // unperturbed copy 0, so every difference is taken between rays of the
   // same batched call.
-  jur_set_option("fp64", fp64 ? 1 : fp64_saved);
-  size_t natm = 0;
-  while (natm < n && sv.ipa[natm] >= 0) natm++;
-  const int rays_per_call = engine_option_int("kernel_rays", kRaysPerCall);
-  const size_t per = (size_t)std::max(2, rays_per_call / std::max(nr, 1)) - 1;
   for (size_t j0 = 0; j0 < natm; j0 += per) {
     std::vector<long> elem{-1};
     std::vector<double> step{0};
@@ -268,8 +323,11 @@ extern "C" int jur_kernel(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, doub
       elem.push_back((long)j);
       step.push_back(h[j]);
     }
-    run_copies(ctl, atm, obs, aero, sv, elem, step, y, s);
+    Copy0 c0;
+    run_copies(ctl, atm, obs, aero, sv, elem, step, y, s, first ? &c0 : nullptr);
     const size_t N = elem.size() * (size_t)nr;
+    if (first) take_copy0(c0, N);
+    first = false;
     for (size_t i = 0; i < m; i++) y0[i] = y[(size_t)slot_id[i] * N + slot_ir[i]];
     for (size_t c = 1; c < elem.size(); c++) {
       compact(N, c, yy1);
@@ -277,9 +335,14 @@ extern "C" int jur_kernel(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, doub
     }
   }
   // Aerosol elements (first mode's N, r, sigma): fresh layer optics for the
-  // perturbed parameter, one call each beside an unperturbed call.
-  if (natm < n) {
-    run_copies(ctl, atm, obs, aero, sv, {-1}, {0}, y, s);
+  // perturbed parameter, one call each.  Their base is an unperturbed call of
+  // the same shape (the FP64 path gives every ray the same result in any
+  // batch, so the first call's copy 0 serves when there was one).
+  if (natm < n && (first || !fp64 || natm == 0)) {
+    Copy0 c0;
+    run_copies(ctl, atm, obs, aero, sv, {-1}, {0}, y, s, first ? &c0 : nullptr);
+    if (first) take_copy0(c0, (size_t)nr);
+    first = false;
     for (size_t i = 0; i < m; i++) y0[i] = y[(size_t)slot_id[i] * nr + slot_ir[i]];
   }
   for (size_t j = natm; j < n; j++) {

@@ -646,16 +646,65 @@ __device__ inline int locate_col64(const float2* c, int n, double x) {
   return ilo;
 }
 
+// Bracket search from a hint (the previous point's bracket) on a strictly
+// increasing column: the same index as the bisection (locate_col64), found
+// in a step or two while the LOS state moves slowly.
+template <bool KEY_EPS>
+__device__ inline int near_col64(const float2* c, int n, double x, int h) {
+  h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
+  const bool okl = (h == 0) || ((double)(KEY_EPS ? c[h].y : c[h].x) <= x);
+  const bool okh = (h == n - 2) || (x < (double)(KEY_EPS ? c[h + 1].y : c[h + 1].x));
+  if (okl && okh) return h;
+  if (!(x == x)) return n - 2;
+  if (okl) {
+    for (int s = 0; s < 3; s++) {
+      h++;
+      if (h == n - 2 || x < (double)(KEY_EPS ? c[h + 1].y : c[h + 1].x)) return h;
+    }
+  } else {
+    h--;
+    if (h == 0 || (double)(KEY_EPS ? c[h].y : c[h].x) <= x) return h;
+  }
+  return locate_col64<KEY_EPS>(c, n, x);
+}
+// The same on an increasing grid (locate, misc.c:237-265).
+__device__ inline int near_grid64(const double* xx, int n, double x, int h) {
+  h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
+  const bool okl = (h == 0) || (xx[h] <= x), okh = (h == n - 2) || (x < xx[h + 1]);
+  if (okl && okh) return h;
+  if (!(x == x)) return n - 2;
+  if (okl) {
+    const int g = h + 1;
+    if (g == n - 2 || x < xx[g + 1]) return g;
+  } else {
+    const int g = h - 1;
+    if (g == 0 || xx[g] <= x) return g;
+  }
+  return locate_mono<double>(xx, n, x);
+}
 // intpol_tbl_u then intpol_tbl_eps of one (p, T) corner (forwardmodel.c:460-474).
-__device__ inline double corner64(const float2* col, int n, double e_path, double du) {
-  const int i = locate_col64<true>(col, n, e_path);
+// HINT: strictly increasing columns; h holds the inverse lookup's bracket.
+// Per-lane hint slots in LDS: slot k of a lane at base[64 k].
+struct LaneHints {
+  int* base;
+  __device__ int& operator[](int k) const { return base[k * 64]; }
+};
+
+template <bool HINT>
+__device__ inline double corner64(const float2* col, int n, double e_path, double du, int& h) {
+  const int i = HINT ? near_col64<true>(col, n, e_path, h) : locate_col64<true>(col, n, e_path);
+  h = i;
   const double uu = lin_f(col[i].y, col[i].x, col[i + 1].y, col[i + 1].x, e_path) + du;
-  const int j = locate_col64<false>(col, n, uu);
+  const int j = HINT ? near_col64<false>(col, n, uu, i) : locate_col64<false>(col, n, uu);
   return clamp01d(lin_f(col[j].x, col[j].y, col[j + 1].x, col[j + 1].y, uu));
 }
 // Segment emissivity of one emitter; updates the path transmittance
-// (forwardmodel.c:433-500).
-__device__ inline double ega64(const DevTables& T, int tb, double p, double t, double du, double& tau_path) {
+// (forwardmodel.c:433-500).  hint: brackets of the previous point (p, T at
+// both pressures, the four corners' inverse lookups), used when HINT
+// (increasing grids and columns).
+template <bool HINT>
+__device__ inline double ega64(const DevTables& T, int tb, double p, double t, double du, double& tau_path,
+                               LaneHints hint) {
   double eps = 0.0;
   const int ntp = T.np[tb];
   if (ntp < 2) {
@@ -664,21 +713,25 @@ __device__ inline double ega64(const DevTables& T, int tb, double p, double t, d
     eps = 1.0;
   } else {
     const double* pg = T.p64 + (size_t)tb * TBLNPMAX;
-    const int ipr = locate_mono<double>(pg, ntp, p);
+    const int ipr = HINT ? near_grid64(pg, ntp, p, hint[0]) : locate_mono<double>(pg, ntp, p);
+    hint[0] = ipr;
     const size_t b0 = (size_t)tb * TBLNPMAX + ipr;
     const int nt0 = T.nt[b0], nt1 = T.nt[b0 + 1];
     if (nt0 >= 2 && nt1 >= 2) {
       const double* t0g = T.t64 + b0 * TBLNTMAX;
       const double* t1g = t0g + TBLNTMAX;
-      const int it0 = locate_mono<double>(t0g, nt0, t), it1 = locate_mono<double>(t1g, nt1, t);
+      const int it0 = HINT ? near_grid64(t0g, nt0, t, hint[1]) : locate_mono<double>(t0g, nt0, t);
+      const int it1 = HINT ? near_grid64(t1g, nt1, t, hint[2]) : locate_mono<double>(t1g, nt1, t);
+      hint[1] = it0;
+      hint[2] = it1;
       const size_t k00 = b0 * TBLNTMAX + it0, k10 = (b0 + 1) * TBLNTMAX + it1;
       const int n00 = T.ncol[k00], n01 = T.ncol[k00 + 1], n10 = T.ncol[k10], n11 = T.ncol[k10 + 1];
       if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
         const double ep = 1.0 - tau_path;
-        double e00 = corner64(T.ue + T.col[k00], n00, ep, du);
-        double e01 = corner64(T.ue + T.col[k00 + 1], n01, ep, du);
-        double e10 = corner64(T.ue + T.col[k10], n10, ep, du);
-        double e11 = corner64(T.ue + T.col[k10 + 1], n11, ep, du);
+        double e00 = corner64<HINT>(T.ue + T.col[k00], n00, ep, du, hint[3]);
+        double e01 = corner64<HINT>(T.ue + T.col[k00 + 1], n01, ep, du, hint[4]);
+        double e10 = corner64<HINT>(T.ue + T.col[k10], n10, ep, du, hint[5]);
+        double e11 = corner64<HINT>(T.ue + T.col[k10 + 1], n11, ep, du, hint[6]);
         e00 = clamp01d(lin<double>(t0g[it0], e00, t0g[it0 + 1], e01, t));
         e11 = clamp01d(lin<double>(t1g[it1], e10, t1g[it1 + 1], e11, t));
         e00 = clamp01d(lin<double>(pg[ipr], e00, pg[ipr + 1], e11, p));
@@ -721,18 +774,17 @@ __device__ inline double continua64(const ChanConst64& c, double p, double t, do
 }
 // Filter-weighted Planck function from the FP64 table: locate on the
 // 1201-point grid 100-400 K, then LIN (forwardmodel.c:786-793).
+// The grid nodes 100 + 0.25 i are exact in FP64, so the bisection's bracket
+// is floor(4 (t - 100)) clamped to [0, 1199] (NaN: the last bracket).
 __device__ inline double planck64(const double* row, double t) {
-  int ilo = 0, ihi = JUR_NPLANCK - 1;
-  while (ihi > ilo + 1) {
-    const int i = (ihi + ilo) >> 1;
-    if (lin(0.0, 100.0, JUR_NPLANCK - 1.0, 400.0, (double)i) > t) ihi = i; else ilo = i;
-  }
+  const double x = floor((t - 100.0) * 4.0);
+  const int ilo = !(x == x) ? JUR_NPLANCK - 2 : (x < 0.0 ? 0 : (x > JUR_NPLANCK - 2.0 ? JUR_NPLANCK - 2 : (int)x));
   const double t0 = lin(0.0, 100.0, JUR_NPLANCK - 1.0, 400.0, (double)ilo);
   const double t1 = lin(0.0, 100.0, JUR_NPLANCK - 1.0, 400.0, (double)(ilo + 1));
   return lin(t0, row[ilo], t1, row[ilo + 1], t);
 }
 
-template <int NGT>
+template <int NGT, bool HINT>
 __global__ void __launch_bounds__(256) k_path64(PathArgs A) {
   const int lane = threadIdx.x & 63;
   int chunk, group;
@@ -746,8 +798,16 @@ __global__ void __launch_bounds__(256) k_path64(PathArgs A) {
   const ChanConst64& cc = A.cc64[id];
   const double* prow = A.planck64 + (size_t)id * JUR_NPLANCK;
   double tau_path[NGT];
+  // bracket hints of the previous point, 7 per emitter (p, T at both
+  // pressures, four corners), in LDS
+  __shared__ int hint_lds[HINT ? 4 : 1][HINT ? NGT * 7 : 1][64];
+  int* hbase = &hint_lds[HINT ? (threadIdx.x >> 6) : 0][0][lane];
 #pragma unroll
-  for (int g = 0; g < NGT; g++) tau_path[g] = 1.0;
+  for (int g = 0; g < NGT; g++) {
+    tau_path[g] = 1.0;
+    if (HINT)
+      for (int i = 0; i < 7; i++) hbase[(g * 7 + i) * 64] = 0;
+  }
   double rad = 0.0, tau = 1.0;
   int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
   // point records: the FP64 copy when the trace wrote one
@@ -762,7 +822,9 @@ __global__ void __launch_bounds__(256) k_path64(PathArgs A) {
     double tau_gas = 1.0;
 #pragma unroll
     for (int g = 0; g < NGT; g++)
-      if (g < A.ng) tau_gas *= 1.0 - ega64(A.T, id * A.ng + g, p, t, rec(ip, uo + g), tau_path[g]);
+      if (g < A.ng)
+        tau_gas *= 1.0 - ega64<HINT>(A.T, id * A.ng + g, p, t, rec(ip, uo + g), tau_path[g],
+                                     LaneHints{hbase + g * 7 * 64});
     const double uco2 = cc.co2_on ? rec(ip, uo + cc.co2_ig) : 0.0;
     const double uh2o = cc.h2o_on ? rec(ip, uo + cc.h2o_ig) : 0.0;
     const double kw = A.nw > 0 ? rec(ip, REC_K + cc.window) : 0.0;
@@ -810,7 +872,10 @@ void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
   const int ngroups = (A.nray + 63) / 64;
   long nb = (long)A.nd * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;
-  hipLaunchKernelGGL((k_path64<NGT>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  if (A.T.shared && NGT <= 8)
+    hipLaunchKernelGGL((k_path64<NGT, NGT <= 8>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  else
+    hipLaunchKernelGGL((k_path64<NGT, false>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
 }
 
 template <int NGT, int CPW, int OCC>
