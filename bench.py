@@ -39,7 +39,10 @@ CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 32768, "aerosol_limb_819
 # commands and the raw counter values.
 MEASURED_TRAFFIC = {
     ("clear_batch_65536", 65536, 6, 2): (1.9978e11, "profiles/k_path_fast_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 6, 4): (1.9911e+11, "profiles/k_path_fast_clear_batch_65536.md"),
 }
+# The same for the FP64 path kernel of kernel() on the Jacobian workload.
+MEASURED_TRAFFIC_JACOBIAN = (6.8150e+10, "profiles/k_path64_jacobian_limb_17.md")
 
 
 def parse():
@@ -333,7 +336,8 @@ def bench_jacobian(args, lib):
                    "forward_rays_per_kernel": rays, "columns_per_s": n * steps / dt,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_path64",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_JACOBIAN[0],
+                     "traffic_source": MEASURED_TRAFFIC_JACOBIAN[1], "kernel": "k_path64",
                      "kernel_ms": path_avg, "alg_bytes_per_call": alg_bytes},
     }
     if not args.no_cpu:
