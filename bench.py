@@ -29,8 +29,13 @@ sys.path.insert(0, str(ROOT / "jurassic-scatter_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# Secondary pencils per scattering point: the aerosol configuration asks for
+# 16; the cirrus configuration's 32 has no form in the reference's fan grid
+# construction (angles would leave 0-180 deg), so it keeps the reference's 28.
+FAN = {"aerosol_limb_8192": 16}
+
 # Rays of the CPU oracle sample per workload (about 10-30 s on 16 cores).
-CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 32768, "aerosol_limb_8192": 192, "cirrus_4096": 2}
+CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 32768, "aerosol_limb_8192": 320, "cirrus_4096": 2}
 
 # HBM bytes per launch of the path kernel measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a plain copy
@@ -62,6 +67,8 @@ def parse():
                     help="1: replay the clear-air call from a captured hipGraph, 0: eager calls (default)")
     ap.add_argument("--weights", default=None,
                     help="trained surrogate for --workload cirrus_16384 (default: the packaged one)")
+    ap.add_argument("--fan", type=int, default=None,
+                    help="secondary pencils per scattering point (default: the workload's, else 28)")
     ap.add_argument("--plain-rays", type=int, default=64,
                     help="rays of the plain-method comparison for --workload cirrus_16384")
     return ap.parse_args()
@@ -131,13 +138,14 @@ def ncores():
         return os.cpu_count() or 1
 
 
-def cpu_oracle(w, argc, argv, sel, scattering):
+def cpu_oracle(w, argc, argv, sel, scattering, fan=28):
     """The FP64 oracle on rays `sel` of w with all host cores: (rad, tau
     [ns][nd], seconds of the timed call, cores)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_ctypes
     from pyjurassic import jtypes as J
     olib = oracle_ctypes.load()
+    olib.oracle_set_fan(fan)
     octl = J.ctl_t()
     quiet(lambda: olib.oracle_read_ctl(argc, argv, C.byref(octl)))
     oaero = None
@@ -400,6 +408,8 @@ def main():
         lib.jur_set_option(b"variant", args.variant)
     if args.wpb is not None:
         lib.jur_set_option(b"wpb", args.wpb)
+    fan = args.fan or FAN.get(args.workload, 28)
+    lib.jur_set_option(b"fan", fan)
     nd, ng, nray = ctl.nd, ctl.ng, w.nray
     b, (rad, tau, tpz, npts), keep = device_batch(w, nd)
 
@@ -467,6 +477,7 @@ def main():
         "data": "synthetic (seeded generator, reference file formats)",
         "config": {"workload": args.workload, "rays": nray, "channels": nd, "emitters": ng,
                    "levels": w.nlev, "los_points": pts, "scattering_depth": scattering,
+                   "fan": fan if scattering else None,
                    "pencils": pencils, "integrated_points": ipts_call, "hip_graph": graph,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -481,7 +492,7 @@ def main():
         ns = min(args.cpu_sample or CPU_SAMPLE[args.workload], nray)
         # every (nray/ns)-th ray: the sample spans limb and nadir geometries
         sel = np.linspace(0, nray - 1, ns).astype(np.int64)
-        orad, otau, cdt, nc = cpu_oracle(w, argc, argv, sel, scattering)
+        orad, otau, cdt, nc = cpu_oracle(w, argc, argv, sel, scattering, fan)
         line["cpu_baseline"] = {"value": ns * nd / cdt, "unit": "ray*channel/s", "cores": nc,
                                 "kind": "port",
                                 "sample": "%d of the %d rays (evenly spaced), all %d channels, FP64 oracle, "

@@ -149,7 +149,8 @@ int jur_init(int device);
    work buffers keep their addresses; a replay whose rays overflow the
    recorded LOS capacity is detected and the call re-run), "surrogate",
    "hull_margin", "collect" (learned scattering source, below),
-   "kernel_rays" (rays per batched call of kernel/jur_kernel, default
+   "fan" (secondary pencils per scattering point, even, 10-28; default the
+   reference's 28), "kernel_rays" (rays per batched call of kernel/jur_kernel, default
    131072: the perturbed atmospheres are grouped into calls of about this
    many rays), "fp64" (1: path integration in FP64 arithmetic, the
    reference's operation order, with FP64 radiances for formod; default 0:

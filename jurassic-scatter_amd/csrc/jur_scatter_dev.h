@@ -4,8 +4,12 @@
 
 #include "jur_dev.h"
 
-// Secondary pencils per scattering point (reference scatter.c:680, nalpha).
+// Secondary pencils per scattering point: the reference's nalpha = 28
+// (scatter.c:680); its Fibonacci grid construction (scatter.c:698-715)
+// gives increasing angles in [0, 180] deg for every even count from 10 to 28
+// (option "fan").
 #define NALPHA 28
+#define NALPHA_MIN 10
 
 // One scattering point: Cartesian position, LOS tangent, layer, atmosphere
 // and time of its pencil.
@@ -37,8 +41,9 @@ struct SunArgs {
 
 struct AngArgs {
   int npt, nd;
+  int nalpha;                // pencils per fan
   const ScaPt* pts;
-  const double* alpha;       // [NALPHA] fan zenith angles
+  const double* alpha;       // [nalpha] fan zenith angles
   const double* phase;       // [NLMAX][nd][NTHETA] layer phase functions (FP64)
   const float* rad;          // [nd][nchild] radiances of the fans (child 28 k + j)
   int nchild;
@@ -64,7 +69,7 @@ void launch_count_points(int nray, const int* np, const float* rec, int R, int c
 void launch_emit_points(int nray, const int* np, const float* rec, int R, int cap, const double4* geo,
                         const double4* geo2, const int* geo_final, const int* jbase, const double* time,
                         const int* atm_idx, ScaPt* pts, hipStream_t s);
-void launch_children(int npt, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s);
+void launch_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s);
 void launch_sun(int npt, const ScaPt* pts, const SunArgs& S, int* valid, double* theta, RayArrays out,
                 hipStream_t s);
 void launch_angular(const AngArgs& G, hipStream_t s);

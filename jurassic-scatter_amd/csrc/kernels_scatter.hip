@@ -124,13 +124,13 @@ __global__ void k_emit_points(int nray, const int* np, const float* rec, int R, 
   }
 }
 
-// The fan of NALPHA secondary pencils of each point (scatter.c:717-740):
+// The fan of nalpha secondary pencils of each point (scatter.c:717-740):
 // view points 10 km away at zenith angles alpha in the local vertical-north
 // plane.
-__global__ void k_children(int npt, const ScaPt* pts, const double* alpha, RayArrays out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npt * NALPHA) return;
-  int k = i / NALPHA, ir = i % NALPHA;
+__global__ void k_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)npt * nalpha) return;
+  int k = (int)(i / nalpha), ir = (int)(i % nalpha);
   const ScaPt P = pts[k];
   double lx[3], ly[3], lz[3], xv[3];
   local_frame(P.x, lx, ly, lz);
@@ -220,9 +220,10 @@ __global__ void k_sun(int npt, const ScaPt* pts, SunArgs S, int* valid, double* 
 // not depend on the channel, so per point the kernel first accumulates
 // A[theta][k] = sum over phi of the weight of fan ray k, then
 // J = sum_theta w(theta) sum_k A[theta][k] rad_k / (180 sum_theta w(theta)).
+template <int NA>
 __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
-  __shared__ double Aw[180][NALPHA];
-  __shared__ double alpha[NALPHA];
+  __shared__ double Aw[180][NA];
+  __shared__ double alpha[NA];
   const int k = blockIdx.x;
   const ScaPt P = G.pts[k];
   const int sk = G.sub_idx ? G.sub_idx[k] : k;
@@ -245,8 +246,8 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
     }
     return;
   }
-  for (int i = threadIdx.x; i < 180 * NALPHA; i += blockDim.x) (&Aw[0][0])[i] = 0.0;
-  if (threadIdx.x < NALPHA) alpha[threadIdx.x] = G.alpha[threadIdx.x];
+  for (int i = threadIdx.x; i < 180 * NA; i += blockDim.x) (&Aw[0][0])[i] = 0.0;
+  if (threadIdx.x < NA) alpha[threadIdx.x] = G.alpha[threadIdx.x];
   __syncthreads();
   double lx[3], ly[3], lz[3], sx[3], sy[3], sz[3];
   local_frame(P.x, lx, ly, lz);
@@ -261,18 +262,18 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
       const double sp = sin(phi), cp = cos(phi);
       for (int i = 0; i < 3; i++) ek[i] = st * sp * sx[i] + st * cp * sy[i] + ct * sz[i];
       const double a2 = angle3(mlz, ek);
-      const int idx = locate_mono<double>(alpha, NALPHA, a2);
+      const int idx = locate_mono<double>(alpha, NA, a2);
       const double f = (a2 - alpha[idx]) / (alpha[idx + 1] - alpha[idx]);
       Aw[it][idx] += 1.0 - f;
       Aw[it][idx + 1] += f;
     }
   }
   __syncthreads();
-  const size_t c0 = (size_t)sk * NALPHA;
+  const size_t c0 = (size_t)sk * NA;
   const double* ph = G.phase + (size_t)P.il * G.nd * NTHETA;
   for (int id = threadIdx.x; id < G.nd; id += blockDim.x) {
-    double r[NALPHA];
-    for (int j = 0; j < NALPHA; j++) r[j] = G.rad[(size_t)id * G.nchild + c0 + j];
+    double r[NA];
+    for (int j = 0; j < NA; j++) r[j] = G.rad[(size_t)id * G.nchild + c0 + j];
     const double* p = ph + (size_t)id * NTHETA;
     double num = 0, den = 0;
     for (int it = 0; it < 180; it++) {
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
       const double t0 = (double)it * M_PI / 180., t1 = (double)(it + 1) * M_PI / 180.;
       const double w = sin(th) * lin(t0, p[it], t1, p[it + 1], th);
       double s = 0;
-      for (int j = 0; j < NALPHA; j++) s += Aw[it][j] * r[j];
+      for (int j = 0; j < NA; j++) s += Aw[it][j] * r[j];
       num += w * s;
       den += 180.0 * w;
     }
@@ -332,16 +333,32 @@ void launch_emit_points(int nray, const int* np, const float* rec, int R, int ca
     hipLaunchKernelGGL(k_emit_points, dim3(nblk(nray, 128)), dim3(128), 0, s, nray, np, rec, R, cap, geo, geo2,
                        geo_final, jbase, time, atm_idx, pts);
 }
-void launch_children(int npt, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s) {
-  long n = (long)npt * NALPHA;
-  if (n > 0) hipLaunchKernelGGL(k_children, dim3(nblk(n, 128)), dim3(128), 0, s, npt, pts, alpha, out);
+void launch_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s) {
+  long n = (long)npt * nalpha;
+  if (n > 0) hipLaunchKernelGGL(k_children, dim3(nblk(n, 128)), dim3(128), 0, s, npt, nalpha, pts, alpha, out);
 }
 void launch_sun(int npt, const ScaPt* pts, const SunArgs& S, int* valid, double* theta, RayArrays out,
                 hipStream_t s) {
   if (npt > 0) hipLaunchKernelGGL(k_sun, dim3(nblk(npt, 64)), dim3(64), 0, s, npt, pts, S, valid, theta, out);
 }
+template <int NA>
+void launch_angular_t(const AngArgs& G, hipStream_t s) {
+  hipLaunchKernelGGL(k_angular<NA>, dim3(G.npt), dim3(256), 0, s, G);
+}
 void launch_angular(const AngArgs& G, hipStream_t s) {
-  if (G.npt > 0) hipLaunchKernelGGL(k_angular, dim3(G.npt), dim3(256), 0, s, G);
+  if (G.npt <= 0) return;
+  switch (G.nalpha) {
+    case 10: launch_angular_t<10>(G, s); break;
+    case 12: launch_angular_t<12>(G, s); break;
+    case 14: launch_angular_t<14>(G, s); break;
+    case 16: launch_angular_t<16>(G, s); break;
+    case 18: launch_angular_t<18>(G, s); break;
+    case 20: launch_angular_t<20>(G, s); break;
+    case 22: launch_angular_t<22>(G, s); break;
+    case 24: launch_angular_t<24>(G, s); break;
+    case 26: launch_angular_t<26>(G, s); break;
+    default: launch_angular_t<28>(G, s); break;
+  }
 }
 void launch_sun_compact(int npt, const int* valid, const int* pos, int* sun_idx, hipStream_t s) {
   if (npt > 0) hipLaunchKernelGGL(k_sun_compact, dim3(nblk(npt, 256)), dim3(256), 0, s, npt, valid, pos, sun_idx);

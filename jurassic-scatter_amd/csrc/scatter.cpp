@@ -6,7 +6,7 @@
 // the leaf pencils.  Here the whole recursion tree is built level by level
 // on the device (a device-side queue of ray batches): level 0 holds the
 // primaries; the scattering points of level L (LOS points with aerofac > 0)
-// each spawn NALPHA secondary rays in level L+1 and one solar ray.  The tree
+// each spawn nalpha secondary rays in level L+1 and one solar ray.  The tree
 // is then integrated bottom-up: leaves (depth 0, extinction only) and solar
 // rays first, then per level the angular integral gives J at every point
 // and the level's rays are integrated with that source.  Large levels are
@@ -67,6 +67,7 @@ struct Level {
 
 struct State {
   DBuf geo, geo2, rec, npts, tsurf, gf, scan, alpha, theta_grid, bsun, phase, status, tpd;
+  int nalpha = 0;  // fan size of the uploaded alpha grid
   int nd_phase = 0;
   // layer phase-function moments for the surrogate features ([nl][nd])
   DBuf asym, pfwd, pback;
@@ -132,8 +133,7 @@ void integrate_set(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& R, i
 // The 28 fan zenith angles of srcfunc_sca_1d (scatter.c:698-715): 83..90
 // deg in 1 deg steps, then steps growing along 3, 3, 5, 8, 13, 13, ... deg
 // towards 0 and 180 deg.
-void alpha_grid(double* alpha) {
-  const int nalpha = NALPHA;
+void alpha_grid(double* alpha, int nalpha) {
   int n1 = 1, n2 = 2;
   double midang = 83, up = 92, down = 81, step = 1 + 2;
   alpha[0] = 0;
@@ -199,11 +199,9 @@ void scatter_set_phase(const ctl_t* ctl, const aero_t* aero, hipStream_t s) {
   upload_vec(st.asym, g, s);
   upload_vec(st.pfwd, pf, s);
   upload_vec(st.pback, pb, s);
-  std::vector<double> alpha(NALPHA), tg(NTHETA), bs(nd);
-  alpha_grid(alpha.data());
+  std::vector<double> tg(NTHETA), bs(nd);
   for (int i = 0; i < NTHETA; i++) tg[i] = M_PI * (double)i / (NTHETA - 1);
   for (int id = 0; id < nd; id++) bs[id] = planck(TSUN, ctl->nu[id]);
-  upload_vec(st.alpha, alpha, s);
   upload_vec(st.theta_grid, tg, s);
   upload_vec(st.bsun, bs, s);
   st.nd_phase = nd;
@@ -221,6 +219,15 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
   st.ncollect = 0;
   engine_set_stat("surrogate_points", 0);
   engine_set_stat("fallback_points", 0);
+  // pencils per fan (option "fan", default the reference's 28)
+  const int nalpha = engine_option_int("fan", NALPHA);
+  if (nalpha < NALPHA_MIN || nalpha > NALPHA || nalpha % 2) JUR_FAIL("Unsupported number of secondary pencils!");
+  if (st.nalpha != nalpha) {
+    std::vector<double> alpha(nalpha);
+    alpha_grid(alpha.data(), nalpha);
+    upload_vec(st.alpha, alpha, s);
+    st.nalpha = nalpha;
+  }
   static std::vector<Level> lv;
   if ((int)lv.size() < S + 1) lv.resize(S + 1);
   lv[0].rays = rays;
@@ -330,10 +337,10 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     }
     // secondary fans -> level L+1
     Level& W = lv[L + 1];
-    long nchild = (long)nfan * NALPHA;
+    long nchild = (long)nfan * nalpha;
     if (nchild > 0x7fffffffL) JUR_FAIL("Too many queue items!");
     W.own.alloc((int)nchild);
-    launch_children(nfan, fan_pts, st.alpha.as<double>(), W.own.arrays(), s);
+    launch_children(nfan, nalpha, fan_pts, st.alpha.as<double>(), W.own.arrays(), s);
     W.rays = W.own.view();
     pencils += nchild;
     // solar rays of the points
@@ -388,6 +395,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     G.nd = nd;
     G.pts = V.pts.as<ScaPt>();
     G.alpha = st.alpha.as<double>();
+    G.nalpha = nalpha;
     G.phase = st.phase.as<double>();
     G.rad = lv[L + 1].rad.as<float>();
     G.nchild = lv[L + 1].rays.nray;

@@ -1443,8 +1443,16 @@ void oracle_suncoord(double sec, double lon, double lat, double *azi, double *sz
 /* The 28 zenith angles of the secondary-ray fan (scatter.c:698-715):
    83..90 deg in 1 deg steps around the horizon, then steps growing along the
    Fibonacci sequence (1+2, 3, 5, 8, 13, 13, ...) towards 0 and 180 deg. */
-static void o_alpha_grid(double *alpha) {
-  const int nalpha = 28;
+/* Pencils per fan: the reference's 28 (scatter.c:680); oracle_set_fan
+   selects another even count from 10 to 28, for which the same Fibonacci
+   construction gives increasing angles in [0, 180] deg. */
+static int o_nalpha = 28;
+void oracle_set_fan(int nalpha) {
+  if (nalpha < 10 || nalpha > 28 || nalpha % 2) OFAIL("Unsupported number of secondary pencils!");
+  o_nalpha = nalpha;
+}
+
+static void o_alpha_grid(double *alpha, int nalpha) {
   int n1 = 1, n2 = 2;
   double midang = 83, up = 92, down = 81, step = 1 + 2;
   alpha[0] = 0;
@@ -1478,6 +1486,7 @@ static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, d
                              const double *x, const double *dx, int il, double *src_sca,
                              int scattering) {
   enum { NALPHA = 28 };
+  const int nalpha = o_nalpha;
   double alpha[NALPHA], dnorth[3], ek[3], lx[3], ly[3], lz[3], sx[3], sy[3], sz[3],
          theta[NTHETA], wsum[NDMAX];
   double *rad2 = (double *)malloc(sizeof(double) * NALPHA * NDMAX);
@@ -1487,11 +1496,11 @@ static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, d
   dnorth[1] = -x[1];
   dnorth[2] = 2 * RE - x[2];
   o_bascoord(x, dnorth, lx, ly, lz);
-  o_alpha_grid(alpha);
+  o_alpha_grid(alpha, nalpha);
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(o_inner) schedule(dynamic) if (o_inner > 1 && omp_get_level() == 1)
 #endif
-  for (int ir = 0; ir < NALPHA; ir++) {
+  for (int ir = 0; ir < nalpha; ir++) {
     double xv[3];
     for (int i = 0; i < 3; i++)
       xv[i] = x[i] + 10. * cos(alpha[ir]) * (-1) * lz[i] + 10. * sin(alpha[ir]) * ly[i];
@@ -1517,7 +1526,7 @@ static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, d
                 cos(theta2) * sz[i];
       int idp = o_locate(theta, NTHETA, theta2);
       double alpha2 = o_angle(mlz, ek);
-      int idx = o_locate(alpha, NALPHA, alpha2);
+      int idx = o_locate(alpha, nalpha, alpha2);
       for (int id = 0; id < ctl->nd; id++) {
         double phase2 = o_lin(theta[idp], aero->p[il][id][idp], theta[idp + 1],
                               aero->p[il][id][idp + 1], theta2);

@@ -27,8 +27,10 @@
  *       their bands); checked against direct evaluation at table nodes.
  *   formod_fov: parity unpinned by fixtures; checked against a delta-shaped
  *       field of view reproducing the unconvolved radiance.
- *   Scattering depth >= 2: parity unpinned by fixtures (no reference test
- *       uses SCA_MULT >= 2); checked for the isotropic-field closure only.
+ *   Scattering depth >= 2 and fan sizes other than 28: parity unpinned by
+ *       fixtures (no reference test uses SCA_MULT >= 2); checked by the
+ *       closure of an isothermal opaque atmosphere (the source J equals the
+ *       Planck radiance, radiance B(T) (1 - tau) or B(T) over the ground).
  *   kernel (finite-difference Jacobian): parity unpinned by fixtures (the
  *       reference ships no kernel output); its columns are checked against
  *       formod on independently perturbed atmospheres, first-order
@@ -113,6 +115,9 @@ void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, 
                    size_t ldk);
 void oracle_kernel_columns(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m,
                            size_t n, size_t ldk, const int *cols, int ncols);
+
+/* Secondary pencils per scattering point (reference: 28); even, 10-28. */
+void oracle_set_fan(int nalpha);
 
 /* Drop the cached tables, filter functions and Planck tables. */
 void oracle_reset(void);

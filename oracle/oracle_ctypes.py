@@ -50,6 +50,7 @@ def load():
         "oracle_kernel": (None, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t]),
         "oracle_kernel_columns": (None, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t,
                                          C.POINTER(C.c_int), C.c_int]),
+        "oracle_set_fan": (None, [C.c_int]),
         "oracle_reset": (None, []),
     }
     for name, (res, args) in sig.items():

@@ -101,3 +101,40 @@ def test_depth2_cirrus_daytime(glib, olib, tmp_path):
     got = gpu_batch(glib, w, tmp_path / "g.ctl")
     print("pencils", glib.jur_stat(b"pencils"), "points", glib.jur_stat(b"scattering_points"), flush=True)
     check(got, oracle_batch(olib, w, tmp_path / "o.ctl"))
+
+
+@pytest.mark.parametrize("fan", [16, 10])
+def test_single_scattering_fan(glib, olib, tmp_path, fan):
+    """Fewer secondary pencils per scattering point (option fan; 16 is the
+    aerosol configuration's count) on the same rays as the 28-pencil test,
+    against the oracle with the same fan."""
+    w = synth.make("aerosol_limb_8192", tmp_path, nray=64)
+    w = w.select(rays=_crossing(w, 20.0, 4, 0), nd=8)
+    glib.jur_set_option(b"fan", fan)
+    olib.oracle_set_fan(fan)
+    try:
+        got = gpu_batch(glib, w, tmp_path / "g.ctl")
+        pencils = glib.jur_stat(b"pencils")
+        points = glib.jur_stat(b"scattering_points")
+        ref = oracle_batch(olib, w, tmp_path / "o.ctl")
+    finally:
+        glib.jur_set_option(b"fan", 28)
+        olib.oracle_set_fan(28)
+    print("fan", fan, "pencils", pencils, "points", points, flush=True)
+    # primaries + a fan and one solar pencil per scattering point
+    assert pencils <= w.nray + (fan + 1) * points
+    check(got, ref)
+
+
+def test_unsupported_fan_exits(tmp_path):
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path[:0] = %r\n"
+        "from pathlib import Path\n"
+        "from pyjurassic import lib as L, synth; from helpers import gpu_batch\n"
+        "lib = L.load(); lib.jur_set_option(b'fan', 32)\n"
+        "w = synth.make('aerosol_limb_8192', Path(%r), nray=2)\n"
+        "gpu_batch(lib, w, Path(%r) / 'g.ctl')\n" % (sys.path[:4], str(tmp_path), str(tmp_path)))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "Unsupported number of secondary pencils" in r.stdout
