@@ -69,7 +69,7 @@ def parse():
                     help="trained surrogate for --workload cirrus_16384 (default: the packaged one)")
     ap.add_argument("--fan", type=int, default=None,
                     help="secondary pencils per scattering point (default: the workload's, else 28)")
-    ap.add_argument("--plain-rays", type=int, default=64,
+    ap.add_argument("--plain-rays", type=int, default=256,
                     help="rays of the plain-method comparison for --workload cirrus_16384")
     return ap.parse_args()
 
@@ -190,7 +190,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 def bench_surrogate(args, lib):
     """Learned scattering source on the held-out cirrus set (16,384 rays x
     32 channels, depth 2, other atmospheres than the training set): the
-    network (trained on the plain method over cirrus_4096 by
+    network (trained on the plain method over cirrus_train_8192 by
     pyjurassic.surrogate) gives J at the primaries' scattering points, the
     fused matrix-core kernel evaluates it; the plain method on the first
     --plain-rays rays judges it (radiance difference, speedup of the call)."""
@@ -198,7 +198,7 @@ def bench_surrogate(args, lib):
     from pyjurassic import surrogate as S
     weights = Path(args.weights) if args.weights else SURROGATE_WEIGHTS
     if not weights.exists():
-        sys.exit("no trained surrogate at %s: python -m pyjurassic.surrogate --workload cirrus_4096 --out %s"
+        sys.exit("no trained surrogate at %s: python -m pyjurassic.surrogate --workload cirrus_train_8192 --out %s"
                  % (weights, weights))
     lib.jur_mlp_load(str(weights).encode())
     info = weights.with_suffix(".json")

@@ -31,8 +31,10 @@ bool mlp_trained();
 //   12..16 ln B(T) at the point, the ground, z + 3 km, z - 3 km and the
 //   coldest level above, 17 / 18 ln H2O column above / below,
 //   19 ln(vertical aerosol optical depth of the layer),
-//   20 / 21 sin / cos of 2 pi nu / 200, 22 / 23 ln of the layer's vertical
-//   optical depth above / below the point (+1e-6).
+//   20 / 21 ln(L_zenith / B(T)), ln(L_nadir / B(T)): the radiance summary,
+//   extinction-only radiances of the point's vertical pencils,
+//   22 / 23 ln of the layer's vertical optical depth above / below the
+//   point (+1e-6).
 struct FeatArgs {
   int npt, nd;
   const ScaPt* pts;
@@ -51,6 +53,7 @@ struct FeatArgs {
   int* out_hull;        // [npt]: set to 1 when a channel leaves the hull
   float margin;         // hull margin in normalised units
   float* lnb;           // [npt nd] ln B(T) at the point (feature 12), or null
+  const float* ud;      // [nd][2 npt] radiances of the zenith (2k) and nadir (2k+1) pencils
 };
 void launch_features(const FeatArgs& a, hipStream_t s);
 void launch_fallback_compact(int npt, const int* flag, const int* pos, const ScaPt* pts, int* sub,

@@ -74,8 +74,10 @@ __global__ void k_features(FeatArgs F) {
   f[18] = (float)log(coldn + 1e-30);
   const double tau_l = ext * (F.top[il] - F.bottom[il]);
   f[19] = (float)log(tau_l > 1e-20 ? tau_l : 1e-20);
-  f[20] = (float)sin(2 * M_PI * nu / 200.0);
-  f[21] = (float)cos(2 * M_PI * nu / 200.0);
+  const size_t nud = 2 * (size_t)F.npt;
+  const double lup = F.ud[(size_t)id * nud + 2 * k], ldn = F.ud[(size_t)id * nud + 2 * k + 1];
+  f[20] = (float)(log(fmax(lup, 1e-30)) - f[12]);
+  f[21] = (float)(log(fmax(ldn, 1e-30)) - f[12]);
   // vertical optical depth of the layer above and below the point
   const double ta = ext * fmax(F.top[il] - z, 0.0), tb = ext * fmax(z - F.bottom[il], 0.0);
   f[22] = (float)log(ta + 1e-6);

@@ -73,6 +73,10 @@ struct State {
   DBuf asym, pfwd, pback;
   // surrogate: features, network output, hull flags, fallback compaction
   DBuf fx, fy, fflag, fpos, fsub, fpts, flnb;
+  // the two vertical pencils of every surrogate point (radiance summary
+  // features): zenith angles, rays, radiances
+  DBuf ud_alpha, ud_rad, ud_tau;
+  RayBuf ud;
   // collected training pairs of the last call (raw features, thermal J)
   DBuf craw, cjth;
   long ncollect = 0;
@@ -283,8 +287,19 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     int nfan = V.npt;
     if (L == 0 && (use_sur || collect) && V.npt > 0) {
       const long nrow = (long)V.npt * nd;
+      // radiance summary of every point: the extinction-only radiances of
+      // its zenith and nadir pencils (the fan's first and last directions)
+      const std::vector<double> ud_ang = {0.0, M_PI};
+      upload_vec(st.ud_alpha, ud_ang, s);
+      st.ud.alloc(2 * V.npt);
+      launch_children(V.npt, 2, pts, st.ud_alpha.as<double>(), st.ud.arrays(), s);
+      float* ud_rad = (float*)st.ud_rad.need((size_t)nd * 2 * V.npt * sizeof(float));
+      float* ud_tau = (float*)st.ud_tau.need((size_t)nd * 2 * V.npt * sizeof(float));
+      integrate_set(ctl, atm, st.ud.view(), 0, nullptr, nullptr, ud_rad, ud_tau, nullptr, nullptr, nullptr, s);
+      pencils += 2L * V.npt;
       FeatArgs F;
       memset(&F, 0, sizeof F);
+      F.ud = ud_rad;
       F.npt = V.npt;
       F.nd = nd;
       F.pts = pts;

@@ -6,7 +6,7 @@ option "collect", which keeps (24 features, thermal source J) of every
 ln(J / B(T)) (B(T): feature 12) with Adam on the GPU, one captured hipGraph
 per step, and saved with its normalisation and training hull.
 
-    python -m pyjurassic.surrogate --workload cirrus_4096 --out net.bin
+    python -m pyjurassic.surrogate --workload cirrus_train_8192 --out net.bin
 
 writes net.bin and net.json (pairs, loss history summary, timings).
 """
@@ -118,7 +118,7 @@ def fit(lib, feat, jth, steps=20000, batch=8192, lr=1e-3, seed=1, log=print):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", default="cirrus_4096")
+    ap.add_argument("--workload", default="cirrus_train_8192")
     ap.add_argument("--nray", type=int, default=None)
     ap.add_argument("--chunk", type=int, default=256)
     ap.add_argument("--steps", type=int, default=20000)

@@ -28,7 +28,10 @@ Workloads (names used by bench.py and the tests):
   4,096 limb and nadir rays, 32 channels, scattering depth 2.
 * ``cirrus_16384``: the same cloud and channels (same layer and refractive
   indices), 16,384 other rays and atmospheres (own seed): the held-out set
-  of the scattering-source surrogate trained on ``cirrus_4096``.
+  of the scattering-source surrogate.
+* ``cirrus_train_8192``: the same cloud and channels, 8,192 further rays and
+  atmospheres (own seed): the surrogate's training set (about 4 M
+  (point, channel) pairs of the plain method).
 
 The scattering workloads observe at 2000-01-01 12:00 UTC (time 43200 s), so
 the solar source term is active; the layer optics use the reference's fixed
@@ -56,6 +59,7 @@ WORKLOADS = {
     "aerosol_limb_8192": 2,
     "cirrus_4096": 3,
     "cirrus_16384": 4,
+    "cirrus_train_8192": 5,
 }
 
 
@@ -219,7 +223,7 @@ def make(name, outdir, nray=None, seed=None, nu=None):
         nu = np.asarray(nu_arg, dtype=float)
     tblbase = str(outdir / "syn")
     # the held-out cirrus set shares the training set's emissivity tables
-    trng = np.random.default_rng(SEED0 + 100 + (3 if cfg == 4 else cfg))
+    trng = np.random.default_rng(SEED0 + 100 + (3 if cfg in (4, 5) else cfg))
     for e in emitters:
         for v in nu:
             write_table("%s_%.4f_%s.bin" % (tblbase, v, e), trng)
@@ -240,7 +244,7 @@ def make(name, outdir, nray=None, seed=None, nu=None):
         t = smooth3(tbase + rng.normal(0.0, 1.0, nlev))[None, :]
         q = qbase[None]
     else:
-        total = {1: 65536, 2: 8192, 3: 4096, 4: 16384}[cfg]
+        total = {1: 65536, 2: 8192, 3: 4096, 4: 16384, 5: 8192}[cfg]
         n = total if nray is None else min(nray, total)
         if cfg == 1:
             nlimb = int(round(0.6 * total))
@@ -267,7 +271,7 @@ def make(name, outdir, nray=None, seed=None, nu=None):
         ih = emitters.index("H2O")
         q[:, ih, :] *= mult[:, None]
         total = n
-        if cfg in (2, 3, 4):
+        if cfg in (2, 3, 4, 5):
             # the cloud of the held-out cirrus set is the training set's
             arng = np.random.default_rng(SEED0 + 200 + min(cfg, 3))
             if cfg == 2:

@@ -110,15 +110,15 @@ def test_surrogate_depth2_speed(glib, case2):
     """Depth 2: the network replaces two levels of secondary fans; the
     scattering call is >= 10x faster than the plain method.  Trained on
     ~90k pairs, the network's radiances differ from the plain method's by
-    <= 0.1 % on average and <= 2 % at most on held-out rays (the largest
+    <= 0.05 % on average and <= 0.5 % at most on held-out rays (the largest
     differences: nadir views of the thick cloud in window channels, where
     the radiance follows J almost one to one)."""
     tmp, train, test = case2
     train_on(glib, train, tmp / "t.ctl", 8, 4000, 2048)
     rel, nsur, nfb, speed = compare(glib, tmp, test)
     assert nsur > 0.5 * (nsur + nfb)
-    assert rel.mean() <= 1e-3
-    assert rel.max() <= 2e-2
+    assert rel.mean() <= 5e-4
+    assert rel.max() <= 5e-3
     assert speed >= 10.0
 
 
