@@ -233,70 +233,81 @@ __global__ void k_fill_f32(float* p, long n, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: 64 samples per block, 4 waves; wave w computes rows
-// 32 (w & 1) .. +32 and columns 128 (w >> 1) .. +128 (four 32 x 32 tiles) of
-// every hidden layer.  Activations stay in LDS (BF16), the transposed
-// weights are read from L2 as 16-byte B fragments.
+// Fused inference: 128 samples per block, 4 waves; wave w computes rows
+// 64 (w & 1) .. +64 (two 32-row tiles) and columns 128 (w >> 1) .. +128
+// (four 32-column tiles) of every hidden layer, so each 16-byte weight
+// fragment read from L2 feeds two matrix-core operations.  The activations
+// stay in one LDS buffer (BF16): a layer's outputs wait in the accumulators
+// until every wave has read its inputs, then overwrite them.  (Staging the
+// weight slices through LDS instead, one block per CU, measured 1.7x
+// slower: the per-slice barriers and the lost second block per CU cost more
+// than the halved L2 reads.)
 // ---------------------------------------------------------------------------
 namespace {
+constexpr int INF_M = 128;  // samples per block
+
 template <int K>
-__device__ inline void hidden_layer(const uint16_t (*in)[LDH], const uint16_t* wt, const float* bias,
-                                    uint16_t (*out)[LDH], int wm, int nb, int lane) {
+__device__ inline void hidden_layer(uint16_t (*act)[LDH], const uint16_t* wt, const float* bias, int wm, int nb,
+                                    int lane) {
   const int h = lane >> 5, r = lane & 31;
-  f32x16 acc[4];
+  f32x16 acc[2][4];
 #pragma unroll
-  for (int t = 0; t < 4; t++) acc[t] = zero16();
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[u][t] = zero16();
   for (int kk = 0; kk < K; kk += 16) {
-    const bf16x8 a = *(const bf16x8*)&in[wm + r][kk + 8 * h];
+    const bf16x8 a0 = *(const bf16x8*)&act[wm + r][kk + 8 * h];
+    const bf16x8 a1 = *(const bf16x8*)&act[wm + 32 + r][kk + 8 * h];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
       const bf16x8 b = *(const bf16x8*)&wt[(long)(nb + 32 * t + r) * K + kk + 8 * h];
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][t], 0, 0, 0);
     }
   }
+  __syncthreads();  // every wave has read this layer's inputs
 #pragma unroll
   for (int t = 0; t < 4; t++) {
     const int j = nb + 32 * t + r;
     const float bj = bias[j];
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int i = wm + 8 * (q >> 2) + 4 * h + (q & 3);
-      out[i][j] = f2bf(silu(acc[t][q] + bj));
-    }
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int i = wm + 32 * u + 8 * (q >> 2) + 4 * h + (q & 3);
+        act[i][j] = f2bf(silu(acc[u][t][q] + bj));
+      }
   }
+  __syncthreads();
 }
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
-  __shared__ __align__(16) uint16_t X0[64][LDH];
-  __shared__ __align__(16) uint16_t H0[64][LDH];
+// two waves per SIMD (at most 256 registers per lane): two blocks per CU
+// hide each other's weight loads
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_infer(InferArgs a) {
+  __shared__ __align__(16) uint16_t A[INF_M][LDH];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const long s0 = (long)blockIdx.x * 64;
-  // x rows (32 BF16 each) into X0[:, 0..31]
-  for (int i = tid; i < 64 * (MLP_INP / 8); i += 256) {
+  const long s0 = (long)blockIdx.x * INF_M;
+  // x rows (32 BF16 each) into A[:, 0..31]
+  for (int i = tid; i < INF_M * (MLP_INP / 8); i += 256) {
     const int r = i / (MLP_INP / 8), c = (i % (MLP_INP / 8)) * 8;
     bf16x8 v;
 #pragma unroll
     for (int e = 0; e < 8; e++) v[e] = 0;
     if (s0 + r < a.n) v = *(const bf16x8*)&a.x[(s0 + r) * MLP_INP + c];
-    *(bf16x8*)&X0[r][c] = v;
+    *(bf16x8*)&A[r][c] = v;
   }
   __syncthreads();
-  const int wm = (w & 1) * 32, nb = (w >> 1) * 128;
-  hidden_layer<MLP_INP>(X0, a.w1t, a.params + MLP_B1, H0, wm, nb, lane);
-  __syncthreads();
-  hidden_layer<MLP_H>(H0, a.w2t, a.params + MLP_B2, X0, wm, nb, lane);
-  __syncthreads();
-  hidden_layer<MLP_H>(X0, a.w3t, a.params + MLP_B3, H0, wm, nb, lane);
-  __syncthreads();
-  hidden_layer<MLP_H>(H0, a.w4t, a.params + MLP_B4, X0, wm, nb, lane);
-  __syncthreads();
-  // output: 4 threads per sample, 64 columns each
-  const int smp = tid >> 2, part = tid & 3;
+  const int wm = (w & 1) * 64, nb = (w >> 1) * 128;
+  hidden_layer<MLP_INP>(A, a.w1t, a.params + MLP_B1, wm, nb, lane);
+  hidden_layer<MLP_H>(A, a.w2t, a.params + MLP_B2, wm, nb, lane);
+  hidden_layer<MLP_H>(A, a.w3t, a.params + MLP_B3, wm, nb, lane);
+  hidden_layer<MLP_H>(A, a.w4t, a.params + MLP_B4, wm, nb, lane);
+  // output: 2 threads per sample, 128 columns each
+  const int smp = tid >> 1, part = tid & 1;
   float s = 0.f;
-  for (int j = part * 64; j < part * 64 + 64; j++) s += bf2f(X0[smp][j]) * bf2f(a.wbf[MLP_W5 + j]);
+  for (int j = part * 128; j < part * 128 + 128; j++) s += bf2f(A[smp][j]) * bf2f(a.wbf[MLP_W5 + j]);
   s += __shfl_xor(s, 1, 64);
-  s += __shfl_xor(s, 2, 64);
   if (part == 0 && s0 + smp < a.n) a.y[s0 + smp] = s + a.params[MLP_B5];
 }
 
@@ -349,5 +360,5 @@ void launch_fill_f32(float* p, long n, float v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_f32, dim3(nb256(n)), dim3(256), 0, s, p, n, v);
 }
 void launch_mlp_infer(const InferArgs& a, hipStream_t s) {
-  if (a.n > 0) hipLaunchKernelGGL(k_mlp_infer, dim3((a.n + 63) / 64), dim3(256), 0, s, a);
+  if (a.n > 0) hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((a.n + INF_M - 1) / INF_M)), dim3(256), 0, s, a);
 }
