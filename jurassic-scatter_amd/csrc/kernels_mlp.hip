@@ -18,9 +18,13 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ inline uint16_t f2bf(float f) { return mlp_f2bf(f); }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ inline float silu(float z) { return z / (1.f + expf(-z)); }
+// SiLU and its derivative with the hardware exponential and reciprocal
+// (v_exp_f32, v_rcp_f32: about 1 ulp); the results are rounded to BF16
+// (8-bit mantissa) or feed BF16 products, far coarser than these errors.
+__device__ inline float sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid_fast(z); }
 __device__ inline float dsilu(float z) {
-  const float s = 1.f / (1.f + expf(-z));
+  const float s = sigmoid_fast(z);
   return s * (1.f + z * (1.f - s));
 }
 
@@ -233,51 +237,66 @@ __global__ void k_fill_f32(float* p, long n, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: 128 samples per block, 4 waves; wave w computes rows
-// 64 (w & 1) .. +64 (two 32-row tiles) and columns 128 (w >> 1) .. +128
-// (four 32-column tiles) of every hidden layer, so each 16-byte weight
-// fragment read from L2 feeds two matrix-core operations.  The activations
-// stay in one LDS buffer (BF16): a layer's outputs wait in the accumulators
-// until every wave has read its inputs, then overwrite them.  (Staging the
-// weight slices through LDS instead, one block per CU, measured 1.7x
-// slower: the per-slice barriers and the lost second block per CU cost more
-// than the halved L2 reads.)
+// Fused inference: 128 samples per block, 4 waves; wave w computes samples
+// 64 (w & 1) .. +64 (two 32-sample tiles) and neurons 128 (w >> 1) .. +128
+// (four 32-neuron tiles) of every hidden layer, so each 16-byte weight
+// fragment read from L2 feeds two matrix-core operations.  The weights are
+// the A operand and the activations the B operand, so a lane's accumulator
+// holds four consecutive neurons of one sample: the epilogue converts them
+// with the packed BF16 instruction and stores 8 bytes at a time.  The
+// activations stay in one LDS buffer (BF16, sample-major): a layer's
+// outputs wait in the accumulators until every wave has read its inputs,
+// then overwrite them.  (Staging the weight slices through LDS instead, one
+// block per CU, measured 1.7x slower: the per-slice barriers and the lost
+// second block per CU cost more than the halved L2 reads.)
 // ---------------------------------------------------------------------------
 namespace {
 constexpr int INF_M = 128;  // samples per block
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values to packed BF16, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ inline uint32_t pack_bf16(float lo, float hi) {
+  f32x2 v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
 
 template <int K>
 __device__ inline void hidden_layer(uint16_t (*act)[LDH], const uint16_t* wt, const float* bias, int wm, int nb,
                                     int lane) {
   const int h = lane >> 5, r = lane & 31;
-  f32x16 acc[2][4];
+  f32x16 acc[2][4];  // [sample tile][neuron tile]
 #pragma unroll
   for (int u = 0; u < 2; u++)
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[u][t] = zero16();
   for (int kk = 0; kk < K; kk += 16) {
-    const bf16x8 a0 = *(const bf16x8*)&act[wm + r][kk + 8 * h];
-    const bf16x8 a1 = *(const bf16x8*)&act[wm + 32 + r][kk + 8 * h];
+    const bf16x8 x0 = *(const bf16x8*)&act[wm + r][kk + 8 * h];
+    const bf16x8 x1 = *(const bf16x8*)&act[wm + 32 + r][kk + 8 * h];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      const bf16x8 b = *(const bf16x8*)&wt[(long)(nb + 32 * t + r) * K + kk + 8 * h];
-      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][t], 0, 0, 0);
-      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][t], 0, 0, 0);
+      const bf16x8 wf = *(const bf16x8*)&wt[(long)(nb + 32 * t + r) * K + kk + 8 * h];
+      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf, x0, acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf, x1, acc[1][t], 0, 0, 0);
     }
   }
   __syncthreads();  // every wave has read this layer's inputs
 #pragma unroll
-  for (int t = 0; t < 4; t++) {
-    const int j = nb + 32 * t + r;
-    const float bj = bias[j];
+  for (int t = 0; t < 4; t++)
 #pragma unroll
-    for (int u = 0; u < 2; u++)
+    for (int g = 0; g < 4; g++) {
+      // neurons n0 .. n0 + 3 of accumulator entries 4 g .. 4 g + 3
+      const int n0 = nb + 32 * t + 8 * g + 4 * h;
+      const float4 bj = *(const float4*)&bias[n0];
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int i = wm + 32 * u + 8 * (q >> 2) + 4 * h + (q & 3);
-        act[i][j] = f2bf(silu(acc[u][t][q] + bj));
+      for (int u = 0; u < 2; u++) {
+        const f32x16& c = acc[u][t];
+        uint2 v;
+        v.x = pack_bf16(silu(c[4 * g] + bj.x), silu(c[4 * g + 1] + bj.y));
+        v.y = pack_bf16(silu(c[4 * g + 2] + bj.z), silu(c[4 * g + 3] + bj.w));
+        *(uint2*)&act[wm + 32 * u + r][n0] = v;
       }
-  }
+    }
   __syncthreads();
 }
 }  // namespace
