@@ -270,14 +270,26 @@ __device__ inline void hidden_layer(uint16_t (*act)[LDH], const uint16_t* wt, co
   for (int u = 0; u < 2; u++)
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[u][t] = zero16();
+  // weight fragments of the next K step are loaded while the current step's
+  // matrix operations run
+  const uint16_t* wrow = wt + (long)(nb + r) * K + 8 * h;
+  bf16x8 wn[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) wn[t] = *(const bf16x8*)&wrow[(long)32 * t * K];
   for (int kk = 0; kk < K; kk += 16) {
+    bf16x8 wc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) wc[t] = wn[t];
+    if (kk + 16 < K) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) wn[t] = *(const bf16x8*)&wrow[(long)32 * t * K + kk + 16];
+    }
     const bf16x8 x0 = *(const bf16x8*)&act[wm + r][kk + 8 * h];
     const bf16x8 x1 = *(const bf16x8*)&act[wm + 32 + r][kk + 8 * h];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      const bf16x8 wf = *(const bf16x8*)&wt[(long)(nb + 32 * t + r) * K + kk + 8 * h];
-      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf, x0, acc[0][t], 0, 0, 0);
-      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf, x1, acc[1][t], 0, 0, 0);
+      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[t], x0, acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[t], x1, acc[1][t], 0, 0, 0);
     }
   }
   __syncthreads();  // every wave has read this layer's inputs
