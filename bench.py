@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 FAN = {"aerosol_limb_8192": 16}
 
 # Rays of the CPU oracle sample per workload (about 10-30 s on 16 cores).
-CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 32768, "aerosol_limb_8192": 320, "cirrus_4096": 2}
+CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 32768, "aerosol_limb_8192": 320, "cirrus_4096": 4}
 
 # HBM bytes per launch of the path kernel measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a plain copy
