@@ -102,6 +102,21 @@ int jur_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size
 size_t jur_state_size(ctl_t *ctl, atm_t *atm, aero_t *aero);
 size_t jur_meas_size(ctl_t *ctl, obs_t *obs);
 
+/* ---- Retrieval: optimal estimation -------------------------------------- */
+
+/* read_ret (reference src/retrievalmodel.h, src/retrievalmodel.c:710-757):
+   iteration control and error keys (KERNEL_RECOMP, CONV_ITMAX, CONV_DMIN,
+   RESMAX, ERR_ANA, ERR_FORMOD[], ERR_NOISE[], ERR_PRESS*, ERR_TEMP*, ERR_Q*,
+   ERR_K*, ERR_NN/RR/SS). */
+void read_ret(int argc, char *argv[], ctl_t *ctl, ret_t *ret);
+/* optimal_estimation (reference src/retrievalmodel.h,
+   src/retrievalmodel.c:407-706): Levenberg-Marquardt retrieval of atm_i /
+   aero_i from obs_meas with a priori atm_apr / aero_apr; writes costs.tab,
+   obs_final.tab, atm_final.tab, aero_final.tab and (ERR_ANA) the error
+   analysis files into ret->dir, the matrices with WRITE_MATRIX. */
+void optimal_estimation(ret_t *ret, ctl_t *ctl, obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr, atm_t *atm_i,
+                        aero_t *aero_apr, aero_t *aero_i);
+
 /* ---- Batched extension (no reference counterpart) --------------------- */
 
 /* Rays with their own atmospheres.  Profiles: z, p, t [natm][nlev];

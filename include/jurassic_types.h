@@ -201,6 +201,33 @@ typedef struct {
   int init;
 } atm_t;
 
+/* Retrieval control parameters (reference src/jurassic.h:573-645). */
+typedef struct {
+  char dir[LEN];             /* working directory */
+  int kernel_recomp;         /* recompute the kernel every n iterations */
+  int conv_itmax;            /* maximum number of iterations */
+  double conv_dmin;          /* minimum normalised step size in state space */
+  double resmax;             /* radiance residual threshold [%] (-999: off) */
+  int err_ana;               /* error analysis (0 = no, 1 = yes) */
+  double err_formod[NDMAX];  /* forward model error [%] */
+  double err_noise[NDMAX];   /* noise error [W/(m^2 sr cm^-1)] */
+  double err_press;          /* pressure error [%] */
+  double err_press_cz;       /* vertical / horizontal correlation lengths [km] */
+  double err_press_ch;
+  double err_temp;           /* temperature error [K] */
+  double err_temp_cz;
+  double err_temp_ch;
+  double err_nn;             /* particle concentration error [cm^-3] */
+  double err_rr;             /* particle radius error */
+  double err_ss;             /* size distribution width error */
+  double err_q[NGMAX];       /* volume mixing ratio error [%] */
+  double err_q_cz[NGMAX];
+  double err_q_ch[NGMAX];
+  double err_k[NWMAX];       /* extinction error [1/km] */
+  double err_k_cz[NWMAX];
+  double err_k_ch[NWMAX];
+} ret_t;
+
 #ifdef __cplusplus
 }
 #endif
@@ -221,5 +248,7 @@ JUR_STATIC_ASSERT(offsetof(obs_t, rad) == 1120008, "obs_t.rad offset");
 JUR_STATIC_ASSERT(sizeof(los_t) == 5000016, "los_t layout");
 JUR_STATIC_ASSERT(sizeof(aero_t) == 2215336, "aero_t layout");
 JUR_STATIC_ASSERT(offsetof(aero_t, p) == 332936, "aero_t.p offset");
+JUR_STATIC_ASSERT(sizeof(ret_t) == 7904, "ret_t layout");
+JUR_STATIC_ASSERT(offsetof(ret_t, err_formod) == 5032, "ret_t.err_formod offset");
 
 #endif

@@ -12,61 +12,13 @@
 
 #include "jur_host.h"
 #include "jur_kernels_misc.h"
+#include "jur_retrieval.h"
 
 namespace {
 
 // Rays per batched forward call (a group of atmosphere copies); option
 // "kernel_rays".
 constexpr int kRaysPerCall = 1 << 17;
-
-struct StateVec {
-  std::vector<double> x;
-  std::vector<int> iqa, ipa;
-  size_t n = 0;
-  void add(double v, int iq, int ip) {
-    x.push_back(v);
-    iqa.push_back(iq);
-    ipa.push_back(ip);
-    n++;
-  }
-};
-
-// atm2x / atm2x_help (atmosphere.c:5-95): p, T, each q, each k on the levels
-// zmin <= z <= zmax of their quantity, then the first mode's N, r, sigma.
-void state_vector(const ctl_t* ctl, const atm_t* atm, const aero_t* aero, StateVec& sv) {
-  auto levels = [&](double zmin, double zmax, const double* v, int iq) {
-    for (int ip = 0; ip < atm->np; ip++)
-      if (atm->z[ip] >= zmin && atm->z[ip] <= zmax) sv.add(v[ip], iq, ip);
-  };
-  levels(ctl->retp_zmin, ctl->retp_zmax, atm->p, IDXP);
-  levels(ctl->rett_zmin, ctl->rett_zmax, atm->t, IDXT);
-  for (int ig = 0; ig < ctl->ng; ig++) levels(ctl->retq_zmin[ig], ctl->retq_zmax[ig], atm->q[ig], IDXQ(ig));
-  for (int iw = 0; iw < ctl->nw; iw++) levels(ctl->retk_zmin[iw], ctl->retk_zmax[iw], atm->k[iw], IDXK(iw));
-  if (ctl->retnn) sv.add(aero->nn[0], IDXNN, -1);
-  if (ctl->retrr) sv.add(aero->rr[0], IDXRR, -1);
-  if (ctl->retss) sv.add(aero->ss[0], IDXSS, -1);
-}
-
-// Perturbation size (retrievalmodel.c:242-268); <= 0 when none is defined.
-double perturbation(const ctl_t* ctl, int iq, double x) {
-  double h = -999;
-  if (iq == IDXP) h = std::fmax(std::fabs(0.01 * x), 1e-7);
-  if (iq == IDXT) h = 1;
-  if (iq >= IDXQ(0) && iq < IDXQ(ctl->ng)) h = std::fmax(std::fabs(0.01 * x), 1e-15);
-  if (iq >= IDXK(0) && iq < IDXK(ctl->nw)) h = 1e-4;
-  if (iq == IDXNN) h = std::fmax(std::fabs(0.5 * x), 0.1);
-  if (iq == IDXRR) h = std::fmax(std::fabs(0.25 * x), 0.1);
-  if (iq == IDXSS) h = std::fmax(std::fabs(0.1 * x), 0.01);
-  return h;
-}
-
-// obs2y (retrievalmodel.c:379-404): finite radiances, ray-major.
-size_t meas_count(const ctl_t* ctl, const obs_t* obs) {
-  size_t m = 0;
-  for (int ir = 0; ir < obs->nr; ir++)
-    for (int id = 0; id < ctl->nd; id++) m += std::isfinite(obs->rad[id][ir]) ? 1 : 0;
-  return m;
-}
 
 struct JacBufs {
   DBuf z, p, t, q, k, h2o, lat, p0, p1, geo[7], idx, rad, tau, rad64, tau64, tpz, tplon, tplat;

@@ -84,8 +84,21 @@ class atm_t(C.Structure):
     ]
 
 
+class ret_t(C.Structure):
+    _fields_ = [
+        ("dir", C.c_char * LEN), ("kernel_recomp", C.c_int), ("conv_itmax", C.c_int),
+        ("conv_dmin", C.c_double), ("resmax", C.c_double), ("err_ana", C.c_int),
+        ("err_formod", C.c_double * NDMAX), ("err_noise", C.c_double * NDMAX),
+        ("err_press", C.c_double), ("err_press_cz", C.c_double), ("err_press_ch", C.c_double),
+        ("err_temp", C.c_double), ("err_temp_cz", C.c_double), ("err_temp_ch", C.c_double),
+        ("err_nn", C.c_double), ("err_rr", C.c_double), ("err_ss", C.c_double),
+        ("err_q", C.c_double * NGMAX), ("err_q_cz", C.c_double * NGMAX), ("err_q_ch", C.c_double * NGMAX),
+        ("err_k", C.c_double * NWMAX), ("err_k_cz", C.c_double * NWMAX), ("err_k_ch", C.c_double * NWMAX),
+    ]
+
+
 SIZES = {"queue_t": 24, "ctl_t": 142240, "atm_t": 288016, "obs_t": 2160008,
-         "los_t": 5000016, "aero_t": 2215336}
+         "los_t": 5000016, "aero_t": 2215336, "ret_t": 7904}
 
 
 def check_layout():

@@ -35,6 +35,8 @@ SIGNATURES = {
     "jur_kernel": (C.c_int, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t]),
     "jur_state_size": (C.c_size_t, [P, P, P]),
     "jur_meas_size": (C.c_size_t, [P, P]),
+    "read_ret": (None, [C.c_int, P, P, P]),
+    "optimal_estimation": (None, [P, P, P, P, P, P, P, P]),
     "jur_formod_batch": (C.c_int, [P, P, P, C.c_int]),
     "jur_formod_batch_device": (C.c_int, [P, P, P, C.c_int, P]),
     "jur_sync": (C.c_int, []),

@@ -1901,6 +1901,363 @@ void oracle_kernel_columns(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, dou
   free(ipa);
 }
 
+/* ------------------------------------------------------------------------
+   Optimal estimation (reference src/retrievalmodel.c:407-706) with its
+   covariances (:761-898) and cost function (:112-167); files are not
+   written, the results stay in the structs and the arrays below.
+   ------------------------------------------------------------------------ */
+
+/* copy_atm / copy_obs (atmosphere.c:98-135, misc.c:20-57) */
+static void o_copy_atm(const ctl_t *ctl, atm_t *d, const atm_t *s, int init) {
+  size_t n = (size_t)s->np * sizeof(double);
+  d->np = s->np;
+  memcpy(d->time, s->time, n);
+  memcpy(d->z, s->z, n);
+  memcpy(d->lon, s->lon, n);
+  memcpy(d->lat, s->lat, n);
+  memcpy(d->p, s->p, n);
+  memcpy(d->t, s->t, n);
+  for (int ig = 0; ig < ctl->ng; ig++) memcpy(d->q[ig], s->q[ig], n);
+  for (int iw = 0; iw < ctl->nw; iw++) memcpy(d->k[iw], s->k[iw], n);
+  d->init = s->init;
+  if (init)
+    for (int ip = 0; ip < d->np; ip++) {
+      d->p[ip] = d->t[ip] = 0;
+      for (int ig = 0; ig < ctl->ng; ig++) d->q[ig][ip] = 0;
+      for (int iw = 0; iw < ctl->nw; iw++) d->k[iw][ip] = 0;
+    }
+}
+static void o_copy_obs(const ctl_t *ctl, obs_t *d, const obs_t *s, int init) {
+  size_t n = (size_t)s->nr * sizeof(double);
+  d->nr = s->nr;
+  memcpy(d->time, s->time, n);
+  memcpy(d->obsz, s->obsz, n);
+  memcpy(d->obslon, s->obslon, n);
+  memcpy(d->obslat, s->obslat, n);
+  memcpy(d->vpz, s->vpz, n);
+  memcpy(d->vplon, s->vplon, n);
+  memcpy(d->vplat, s->vplat, n);
+  memcpy(d->tpz, s->tpz, n);
+  memcpy(d->tplon, s->tplon, n);
+  memcpy(d->tplat, s->tplat, n);
+  for (int id = 0; id < ctl->nd; id++) {
+    memcpy(d->rad[id], s->rad[id], n);
+    memcpy(d->tau[id], s->tau[id], n);
+  }
+  if (init)
+    for (int ir = 0; ir < d->nr; ir++)
+      for (int id = 0; id < ctl->nd; id++)
+        if (isfinite(d->rad[id][ir])) d->rad[id][ir] = d->tau[id][ir] = 0;
+}
+
+/* In-place Cholesky factor L (lower, row-major) of an SPD matrix. */
+static void o_cholesky(double *a, size_t n) {
+  for (size_t j = 0; j < n; j++) {
+    double d = a[j * n + j];
+    for (size_t k = 0; k < j; k++) d -= a[j * n + k] * a[j * n + k];
+    if (!(d > 0)) OFAIL("Matrix is not positive definite!");
+    d = sqrt(d);
+    a[j * n + j] = d;
+    for (size_t i = j + 1; i < n; i++) {
+      double s = a[i * n + j];
+      for (size_t k = 0; k < j; k++) s -= a[i * n + k] * a[j * n + k];
+      a[i * n + j] = s / d;
+    }
+  }
+}
+static void o_chol_solve(const double *l, size_t n, const double *b, double *x) {
+  for (size_t i = 0; i < n; i++) {
+    double s = b[i];
+    for (size_t k = 0; k < i; k++) s -= l[i * n + k] * x[k];
+    x[i] = s / l[i * n + i];
+  }
+  for (size_t i = n; i-- > 0;) {
+    double s = x[i];
+    for (size_t k = i + 1; k < n; k++) s -= l[k * n + i] * x[k];
+    x[i] = s / l[i * n + i];
+  }
+}
+/* matrix_invert (retrievalmodel.c:302-327) */
+static void o_matrix_invert(double *a, size_t n) {
+  int diag = 1;
+  for (size_t i = 0; i < n && diag; i++)
+    for (size_t j = i + 1; j < n; j++)
+      if (a[i * n + j] != 0) {
+        diag = 0;
+        break;
+      }
+  if (diag) {
+    for (size_t i = 0; i < n; i++) a[i * n + i] = 1 / a[i * n + i];
+    return;
+  }
+  double *l = malloc(n * n * sizeof(double)), *e = calloc(n, sizeof(double)), *c = malloc(n * sizeof(double));
+  memcpy(l, a, n * n * sizeof(double));
+  o_cholesky(l, n);
+  for (size_t j = 0; j < n; j++) {
+    e[j] = 1;
+    o_chol_solve(l, n, e, c);
+    for (size_t i = 0; i < n; i++) a[i * n + j] = c[i];
+    e[j] = 0;
+  }
+  free(l);
+  free(e);
+  free(c);
+}
+/* matrix_product (retrievalmodel.c:331-375), transpose 1: C = A^T diag(b^2) A */
+static void o_atba(const double *a, size_t m, size_t n, const double *b, double *c) {
+  for (size_t i = 0; i < n; i++)
+    for (size_t j = 0; j < n; j++) {
+      double s = 0;
+      for (size_t k = 0; k < m; k++) s += (b[k] * a[k * n + i]) * (b[k] * a[k * n + j]);
+      c[i * n + j] = s;
+    }
+}
+/* corr_function (retrievalmodel.c:91-108) */
+static double o_corr(double z0, double lon0, double lat0, double z1, double lon1, double lat1, double cz,
+                     double ch) {
+  double x0[3], x1[3];
+  o_geo2cart(0, lon0, lat0, x0);
+  o_geo2cart(0, lon1, lat1, x1);
+  return exp(-o_dist(x0, x1) / ch - fabs(z0 - z1) / cz);
+}
+
+void oracle_read_ret(int argc, char *argv[], ctl_t *ctl, ret_t *ret) {
+  ret->kernel_recomp = (int)oracle_scan_ctl(argc, argv, "KERNEL_RECOMP", -1, "1", NULL);
+  ret->conv_itmax = (int)oracle_scan_ctl(argc, argv, "CONV_ITMAX", -1, "20", NULL);
+  ret->conv_dmin = oracle_scan_ctl(argc, argv, "CONV_DMIN", -1, "0.1", NULL);
+  ret->resmax = oracle_scan_ctl(argc, argv, "RESMAX", -1, "-999", NULL);
+  ret->err_ana = (int)oracle_scan_ctl(argc, argv, "ERR_ANA", -1, "1", NULL);
+  for (int id = 0; id < ctl->nd; id++) ret->err_formod[id] = oracle_scan_ctl(argc, argv, "ERR_FORMOD", id, "0", NULL);
+  for (int id = 0; id < ctl->nd; id++) ret->err_noise[id] = oracle_scan_ctl(argc, argv, "ERR_NOISE", id, "0", NULL);
+  ret->err_press = oracle_scan_ctl(argc, argv, "ERR_PRESS", -1, "0", NULL);
+  ret->err_press_cz = oracle_scan_ctl(argc, argv, "ERR_PRESS_CZ", -1, "-999", NULL);
+  ret->err_press_ch = oracle_scan_ctl(argc, argv, "ERR_PRESS_CH", -1, "-999", NULL);
+  ret->err_temp = oracle_scan_ctl(argc, argv, "ERR_TEMP", -1, "0", NULL);
+  ret->err_temp_cz = oracle_scan_ctl(argc, argv, "ERR_TEMP_CZ", -1, "-999", NULL);
+  ret->err_temp_ch = oracle_scan_ctl(argc, argv, "ERR_TEMP_CH", -1, "-999", NULL);
+  ret->err_nn = oracle_scan_ctl(argc, argv, "ERR_NN", -1, "0", NULL);
+  ret->err_rr = oracle_scan_ctl(argc, argv, "ERR_RR", -1, "0", NULL);
+  ret->err_ss = oracle_scan_ctl(argc, argv, "ERR_SS", -1, "0", NULL);
+  for (int ig = 0; ig < ctl->ng; ig++) {
+    ret->err_q[ig] = oracle_scan_ctl(argc, argv, "ERR_Q", ig, "0", NULL);
+    ret->err_q_cz[ig] = oracle_scan_ctl(argc, argv, "ERR_Q_CZ", ig, "-999", NULL);
+    ret->err_q_ch[ig] = oracle_scan_ctl(argc, argv, "ERR_Q_CH", ig, "-999", NULL);
+  }
+  for (int iw = 0; iw < ctl->nw; iw++) {
+    ret->err_k[iw] = oracle_scan_ctl(argc, argv, "ERR_K", iw, "0", NULL);
+    ret->err_k_cz[iw] = oracle_scan_ctl(argc, argv, "ERR_K_CZ", iw, "-999", NULL);
+    ret->err_k_ch[iw] = oracle_scan_ctl(argc, argv, "ERR_K_CH", iw, "-999", NULL);
+  }
+}
+
+/* set_cov_apr (retrievalmodel.c:761-857): the a priori covariance S_a
+   (n x n, row-major) of the state vector of atm_apr / aero_apr. */
+void oracle_cov_apr(const ret_t *ret, ctl_t *ctl, atm_t *atm_apr, aero_t *aero_apr, double *s_a) {
+  static aero_t aero_none;
+  if (!aero_apr) aero_apr = &aero_none;
+  size_t n = oracle_atm2x(ctl, atm_apr, aero_apr, NULL, NULL, NULL);
+  int *iqa = malloc(n * sizeof(int)), *ipa = malloc(n * sizeof(int));
+  oracle_atm2x(ctl, atm_apr, aero_apr, NULL, iqa, ipa);
+  double *sg = malloc(n * sizeof(double));
+  oracle_atm2x(ctl, atm_apr, aero_apr, sg, NULL, NULL);
+  for (size_t i = 0; i < n; i++) {
+    if (iqa[i] == IDXP) sg[i] = ret->err_press / 100 * sg[i];
+    if (iqa[i] == IDXT) sg[i] = ret->err_temp;
+    for (int ig = 0; ig < ctl->ng; ig++)
+      if (iqa[i] == IDXQ(ig)) sg[i] = ret->err_q[ig] / 100 * sg[i];
+    for (int iw = 0; iw < ctl->nw; iw++)
+      if (iqa[i] == IDXK(iw)) sg[i] = ret->err_k[iw];
+    if (iqa[i] == IDXNN) sg[i] = ret->err_nn;
+    if (iqa[i] == IDXRR) sg[i] = ret->err_rr;
+    if (iqa[i] == IDXSS) sg[i] = ret->err_ss;
+  }
+  for (size_t i = 0; i < n; i++)
+    for (size_t j = 0; j < n; j++) {
+      s_a[i * n + j] = i == j ? sg[i] * sg[i] : 0;
+      if (i != j && iqa[i] == iqa[j]) {
+        double cz = 0, ch = 0;
+        if (iqa[i] == IDXP) {
+          cz = ret->err_press_cz;
+          ch = ret->err_press_ch;
+        }
+        if (iqa[i] == IDXT) {
+          cz = ret->err_temp_cz;
+          ch = ret->err_temp_ch;
+        }
+        for (int ig = 0; ig < ctl->ng; ig++)
+          if (iqa[i] == IDXQ(ig)) {
+            cz = ret->err_q_cz[ig];
+            ch = ret->err_q_ch[ig];
+          }
+        for (int iw = 0; iw < ctl->nw; iw++)
+          if (iqa[i] == IDXK(iw)) {
+            cz = ret->err_k_cz[iw];
+            ch = ret->err_k_ch[iw];
+          }
+        if (cz > 0 && ch > 0)
+          s_a[i * n + j] = sg[i] * sg[j] *
+                               o_corr(atm_apr->z[ipa[i]], atm_apr->lon[ipa[i]], atm_apr->lat[ipa[i]],
+                                      atm_apr->z[ipa[j]], atm_apr->lon[ipa[j]], atm_apr->lat[ipa[j]], cz, ch);
+      }
+    }
+  free(sg);
+  free(iqa);
+  free(ipa);
+}
+
+/* set_cov_meas (retrievalmodel.c:861-898): noise and forward-model
+   errors of the measurement vector and 1 / sqrt of their squares' sum. */
+void oracle_cov_meas(const ret_t *ret, ctl_t *ctl, obs_t *obs_meas, double *sig_noise, double *sig_formod,
+                     double *sig_eps_inv) {
+  size_t m = oracle_obs2y(ctl, obs_meas, NULL, NULL, NULL);
+  static obs_t err;
+  o_copy_obs(ctl, &err, obs_meas, 1);
+  for (int ir = 0; ir < err.nr; ir++)
+    for (int id = 0; id < ctl->nd; id++)
+      err.rad[id][ir] = isfinite(obs_meas->rad[id][ir]) ? ret->err_noise[id] : NAN;
+  oracle_obs2y(ctl, &err, sig_noise, NULL, NULL);
+  o_copy_obs(ctl, &err, obs_meas, 1);
+  for (int ir = 0; ir < err.nr; ir++)
+    for (int id = 0; id < ctl->nd; id++) err.rad[id][ir] = fabs(ret->err_formod[id] / 100 * obs_meas->rad[id][ir]);
+  oracle_obs2y(ctl, &err, sig_formod, NULL, NULL);
+  for (size_t i = 0; i < m; i++)
+    sig_eps_inv[i] = 1 / sqrt(sig_noise[i] * sig_noise[i] + sig_formod[i] * sig_formod[i]);
+}
+
+/* cost_function (retrievalmodel.c:112-167) without the file */
+static double o_cost(const double *dx, size_t n, const double *dy, size_t m, const double *s_a_inv,
+                     const double *sig_eps_inv) {
+  double cm = 0, ca = 0;
+  for (size_t i = 0; i < m; i++) cm += (dy[i] * sig_eps_inv[i]) * (dy[i] * sig_eps_inv[i]);
+  cm /= (double)m;
+  for (size_t i = 0; i < n; i++) {
+    double s = 0;
+    for (size_t j = 0; j < n; j++) s += s_a_inv[i * n + j] * dx[j];
+    ca += dx[i] * s;
+  }
+  ca /= (double)m;
+  return cm + ca;
+}
+
+/* The retrieval; chisq[] receives every cost-function value (at most
+   1 + 20 conv_itmax), *nchisq their number, err_total[n] the square roots of
+   the retrieval covariance's diagonal when ret->err_ana. */
+void oracle_optimal_estimation(const ret_t *ret, ctl_t *ctl, obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
+                               atm_t *atm_i, aero_t *aero_apr, aero_t *aero_i, double *chisq_out, int *nchisq,
+                               double *err_total) {
+  static aero_t aero_none;
+  if (!aero_apr) aero_apr = &aero_none;
+  if (!aero_i) aero_i = &aero_none;
+  size_t m = oracle_obs2y(ctl, obs_meas, NULL, NULL, NULL);
+  size_t n = oracle_atm2x(ctl, atm_apr, aero_apr, NULL, NULL, NULL);
+  if (m <= 0 || n <= 0) OFAIL("Check problem definition!");
+  int *iqa = malloc(n * sizeof(int)), *ipa = malloc(n * sizeof(int));
+  double *x_a = malloc(n * sizeof(double)), *x_i = malloc(n * sizeof(double)), *dx = malloc(n * sizeof(double));
+  double *b = malloc(n * sizeof(double)), *x_step = calloc(n, sizeof(double));
+  double *y_m = malloc(m * sizeof(double)), *y_i = malloc(m * sizeof(double)), *dy = malloc(m * sizeof(double));
+  double *sig_noise = malloc(m * sizeof(double)), *sig_formod = malloc(m * sizeof(double));
+  double *sig_eps_inv = malloc(m * sizeof(double));
+  double *s_a_inv = malloc(n * n * sizeof(double)), *cov = malloc(n * n * sizeof(double));
+  double *a = malloc(n * n * sizeof(double)), *k_i = malloc(m * n * sizeof(double));
+  oracle_atm2x(ctl, atm_apr, aero_apr, NULL, iqa, ipa);
+  o_copy_atm(ctl, atm_i, atm_apr, 0);
+  o_copy_obs(ctl, obs_i, obs_meas, 0);
+  *aero_i = *aero_apr;
+  oracle_formod(ctl, atm_i, obs_i, aero_i);
+  oracle_atm2x(ctl, atm_apr, aero_apr, x_a, NULL, NULL);
+  oracle_atm2x(ctl, atm_i, aero_i, x_i, NULL, NULL);
+  oracle_obs2y(ctl, obs_meas, y_m, NULL, NULL);
+  oracle_obs2y(ctl, obs_i, y_i, NULL, NULL);
+  oracle_cov_apr(ret, ctl, atm_apr, aero_apr, s_a_inv);
+  o_matrix_invert(s_a_inv, n);
+  oracle_cov_meas(ret, ctl, obs_meas, sig_noise, sig_formod, sig_eps_inv);
+  int nc = 0;
+  for (size_t i = 0; i < n; i++) dx[i] = x_i[i] - x_a[i];
+  for (size_t i = 0; i < m; i++) dy[i] = y_m[i] - y_i[i];
+  double chisq = o_cost(dx, n, dy, m, s_a_inv, sig_eps_inv);
+  chisq_out[nc++] = chisq;
+  if (isfinite(chisq)) {
+    oracle_kernel(ctl, atm_i, obs_i, aero_i, k_i, m, n, n);
+    double lmpar = 0.001;
+    for (int it = 1; it <= ret->conv_itmax; it++) {
+      double chisq_old = chisq;
+      if (it % ret->kernel_recomp == 0 && !(ret->kernel_recomp == 1 && it == 1))
+        oracle_kernel(ctl, atm_i, obs_i, aero_i, k_i, m, n, n);
+      if (it % ret->kernel_recomp == 0 || it == 1) o_atba(k_i, m, n, sig_eps_inv, cov);
+      for (size_t i = 0; i < n; i++) dx[i] = x_i[i] - x_a[i];
+      for (size_t i = 0; i < m; i++) dy[i] = y_m[i] - y_i[i];
+      for (size_t j = 0; j < n; j++) {
+        double s = 0, t = 0;
+        for (size_t i = 0; i < m; i++) s += k_i[i * n + j] * dy[i] * sig_eps_inv[i] * sig_eps_inv[i];
+        for (size_t i = 0; i < n; i++) t += s_a_inv[j * n + i] * dx[i];
+        b[j] = s - t;
+      }
+      for (int it2 = 0; it2 < 20; it2++) {
+        for (size_t i = 0; i < n * n; i++) a[i] = s_a_inv[i] * (1 + lmpar) + cov[i];
+        o_cholesky(a, n);
+        o_chol_solve(a, n, b, x_step);
+        for (size_t i = 0; i < n; i++) x_i[i] += x_step[i];
+        o_copy_atm(ctl, atm_i, atm_apr, 0);
+        o_copy_obs(ctl, obs_i, obs_meas, 0);
+        *aero_i = *aero_apr;
+        oracle_x2atm(ctl, x_i, atm_i, aero_i);
+        for (int ip = 0; ip < atm_i->np; ip++) {
+          atm_i->p[ip] = o_min(o_max(atm_i->p[ip], 5e-7), 5e4);
+          atm_i->t[ip] = o_min(o_max(atm_i->t[ip], 100), 400);
+          for (int ig = 0; ig < ctl->ng; ig++) atm_i->q[ig][ip] = o_min(o_max(atm_i->q[ig][ip], 0), 1);
+          for (int iw = 0; iw < ctl->nw; iw++) atm_i->k[iw][ip] = o_max(atm_i->k[iw][ip], 0);
+        }
+        for (int im = 0; im < aero_i->nm; im++) {
+          aero_i->nn[im] = o_min(o_max(aero_i->nn[im], 1e-6), 1e4);
+          aero_i->rr[im] = o_min(o_max(aero_i->rr[im], 1e-2), 100);
+          aero_i->ss[im] = o_min(o_max(aero_i->ss[im], 1.1), 3);
+        }
+        oracle_formod(ctl, atm_i, obs_i, aero_i);
+        oracle_obs2y(ctl, obs_i, y_i, NULL, NULL);
+        for (size_t i = 0; i < n; i++) dx[i] = x_i[i] - x_a[i];
+        for (size_t i = 0; i < m; i++) dy[i] = y_m[i] - y_i[i];
+        chisq = o_cost(dx, n, dy, m, s_a_inv, sig_eps_inv);
+        chisq_out[nc++] = chisq;
+        if (chisq > chisq_old) {
+          lmpar *= 10;
+          for (size_t i = 0; i < n; i++) x_i[i] -= x_step[i];
+        } else {
+          lmpar /= 10;
+          break;
+        }
+      }
+      double disq = 0;
+      for (size_t i = 0; i < n; i++) disq += x_step[i] * b[i];
+      disq /= (double)n;
+      if (disq < ret->conv_dmin) break;
+    }
+    if (ret->err_ana && err_total) {
+      o_atba(k_i, m, n, sig_eps_inv, cov);
+      for (size_t i = 0; i < n * n; i++) cov[i] += s_a_inv[i];
+      o_matrix_invert(cov, n);
+      for (size_t i = 0; i < n; i++) err_total[i] = sqrt(cov[i * n + i]);
+    }
+  }
+  *nchisq = nc;
+  free(iqa);
+  free(ipa);
+  free(x_a);
+  free(x_i);
+  free(dx);
+  free(b);
+  free(x_step);
+  free(y_m);
+  free(y_i);
+  free(dy);
+  free(sig_noise);
+  free(sig_formod);
+  free(sig_eps_inv);
+  free(s_a_inv);
+  free(cov);
+  free(a);
+  free(k_i);
+}
+
 /* Batched clear-air forward model, one atmosphere per ray. */
 void oracle_formod_batch(ctl_t *ctl, int nray, int nlev, const double *z, const double *p,
                          const double *t, const double *q, const double *k, const double *time,

@@ -116,6 +116,20 @@ void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, 
 void oracle_kernel_columns(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m,
                            size_t n, size_t ldk, const int *cols, int ncols);
 
+/* Optimal estimation (reference src/retrievalmodel.c:407-706, read_ret
+   :710-757) without its files: chisq[] receives every cost-function value
+   (at most 1 + 20 conv_itmax), *nchisq their number, err_total[n] the
+   retrieval errors (square roots of the covariance diagonal, ERR_ANA). */
+void oracle_read_ret(int argc, char *argv[], ctl_t *ctl, ret_t *ret);
+/* The a priori covariance S_a (n x n) and the measurement errors (m each)
+   of the retrieval (set_cov_apr, set_cov_meas: retrievalmodel.c:761-898). */
+void oracle_cov_apr(const ret_t *ret, ctl_t *ctl, atm_t *atm_apr, aero_t *aero_apr, double *s_a);
+void oracle_cov_meas(const ret_t *ret, ctl_t *ctl, obs_t *obs_meas, double *sig_noise, double *sig_formod,
+                     double *sig_eps_inv);
+void oracle_optimal_estimation(const ret_t *ret, ctl_t *ctl, obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
+                               atm_t *atm_i, aero_t *aero_apr, aero_t *aero_i, double *chisq, int *nchisq,
+                               double *err_total);
+
 /* Secondary pencils per scattering point (reference: 28); even, 10-28. */
 void oracle_set_fan(int nalpha);
 

@@ -51,6 +51,10 @@ def load():
         "oracle_kernel_columns": (None, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t,
                                          C.POINTER(C.c_int), C.c_int]),
         "oracle_set_fan": (None, [C.c_int]),
+        "oracle_read_ret": (None, [C.c_int, P, P, P]),
+        "oracle_cov_apr": (None, [P, P, P, P, D]),
+        "oracle_cov_meas": (None, [P, P, P, D, D, D]),
+        "oracle_optimal_estimation": (None, [P, P, P, P, P, P, P, P, D, C.POINTER(C.c_int), D]),
         "oracle_reset": (None, []),
     }
     for name, (res, args) in sig.items():
