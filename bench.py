@@ -388,6 +388,133 @@ def bench_jacobian(args, lib):
     print(json.dumps(line), flush=True)
 
 
+def retrieval_truth(lib, prefix, ctl, atm, obs):
+    """Measurements of a perturbed truth (temperature +2 K around 20 km,
+    H2O x1.2 below 15 km, O3 x0.9) by `lib`'s forward model in FP64."""
+    from pyjurassic import jtypes as J
+    tru, meas = J.atm_t(), J.obs_t()
+    C.memmove(C.addressof(tru), C.addressof(atm), C.sizeof(atm))
+    C.memmove(C.addressof(meas), C.addressof(obs), C.sizeof(obs))
+    for l in range(atm.np):
+        z = atm.z[l]
+        tru.t[l] += 2.0 * np.exp(-((z - 20) / 6) ** 2)
+        if z < 15:
+            tru.q[1][l] *= 1.2
+        tru.q[2][l] *= 0.9
+    if prefix:
+        quiet(lambda: lib.oracle_formod(C.byref(ctl), C.byref(tru), C.byref(meas), None))
+    else:
+        lib.jur_set_option(b"fp64", 1)
+        quiet(lambda: lib.formod(C.byref(ctl), C.byref(tru), C.byref(meas), None))
+        lib.jur_set_option(b"fp64", 0)
+    return meas
+
+
+def bench_retrieval(args, lib):
+    """optimal_estimation (reference src/retrievalmodel.c:407-706): the
+    17-ray, 64-channel limb scan of the Jacobian workload; state vector
+    temperature, H2O and O3 on all 121 levels (363 elements, 1088
+    measurements), a priori errors 5 K / 50 % / 30 % with 5 km vertical
+    correlation, 1 % measurement errors; measurements of a perturbed truth.
+    One step is one whole retrieval (Levenberg-Marquardt iterations with a
+    Jacobian each, error analysis, output files).  The CPU leg runs the
+    oracle's restatement on the same inputs; `accuracy` compares the
+    retrieved states."""
+    import torch
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyjurassic import jtypes as J, synth
+    tmp = Path(tempfile.mkdtemp(prefix="jur_ret_"))
+    w = synth.make("clear_batch_65536", tmp, nray=1)
+    ctlpath = tmp / "ret.ctl"
+    w.write_ctl(ctlpath)
+    keys = ["RETT_ZMIN", 0, "RETT_ZMAX", 120, "RETQ_ZMIN[1]", 0, "RETQ_ZMAX[1]", 120,
+            "RETQ_ZMIN[2]", 0, "RETQ_ZMAX[2]", 120, "ERR_TEMP", 5, "ERR_TEMP_CZ", 5, "ERR_TEMP_CH", 500,
+            "ERR_Q[1]", 50, "ERR_Q_CZ[1]", 5, "ERR_Q_CH[1]", 500, "ERR_Q[2]", 30, "ERR_Q_CZ[2]", 5,
+            "ERR_Q_CH[2]", 500]
+    for d in range(len(w.nu)):
+        keys += ["ERR_FORMOD[%d]" % d, 1]
+
+    def inputs(L, prefix):
+        ctl, atm, obs = jacobian_inputs(L, prefix, w, ctlpath, keys)
+        ret = J.ret_t()
+        argc, argv = J.argv(ctlpath, *keys)
+        quiet(lambda: getattr(L, prefix + "read_ret")(argc, argv, C.byref(ctl), C.byref(ret)))
+        return ctl, atm, obs, ret
+
+    ctl, atm, obs, ret = inputs(lib, "")
+    ret.dir = str(tmp).encode()
+    meas = retrieval_truth(lib, "", ctl, atm, obs)
+    r = C.byref
+    n = lib.jur_state_size(r(ctl), r(atm), None)
+    m = lib.jur_meas_size(r(ctl), r(meas))
+    aero_apr, aero_i = J.aero_t(), J.aero_t()
+    obs_i, atm_i = J.obs_t(), J.atm_t()
+
+    def step():
+        quiet(lambda: lib.optimal_estimation(r(ret), r(ctl), r(meas), r(obs_i), r(atm), r(atm_i), r(aero_apr),
+                                             r(aero_i)))
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    steps = args.steps
+    tot = lambda: np.array([lib.jur_stat(k) for k in (b"calls_total", b"path_ms_total",
+                                                      b"integrated_points_total", b"pencils_total")])
+    s0 = tot()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    dt = time.perf_counter() - t0
+    calls, path_ms, ipts, pencils = (tot() - s0) / steps
+    costs = [ln.split() for ln in (tmp / "costs.tab").read_text().splitlines() if ln and ln[0].isdigit()]
+    nd, ng = ctl.nd, ctl.ng
+    alg_bytes = ipts * nd * (128 * ng + 8) + pencils * nd * 8
+    achieved = alg_bytes / (path_ms * 1e-3) / 1e9
+    line = {
+        "metric": "retrievals/s (optimal_estimation: forward models, Jacobians, linear algebra, outputs)",
+        "value": steps / dt,
+        "unit": "retrieval/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, reference file formats)",
+        "config": {"workload": "retrieval_limb_17", "rays": obs.nr, "channels": nd, "emitters": ng,
+                   "levels": w.nlev, "state_elements": n, "measurements": m,
+                   "cost_evaluations": len(costs), "forward_calls": calls,
+                   "forward_rays": pencils, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_path64",
+                     "kernel_ms": path_ms, "kernel_share": path_ms / (1e3 * dt / steps),
+                     "alg_bytes_per_retrieval": alg_bytes,
+                     "note": "path kernel time and algorithmic bytes summed over the retrieval's calls"},
+    }
+    if not args.no_cpu:
+        import oracle_ctypes
+        olib = oracle_ctypes.load()
+        octl, oatm, oobs, oret = inputs(olib, "oracle_")
+        ometa = retrieval_truth(olib, "oracle_", octl, oatm, oobs)
+        chisq = np.zeros(1 + 20 * oret.conv_itmax)
+        nc = C.c_int(0)
+        o_obs, o_atm = J.obs_t(), J.atm_t()
+        c0 = time.perf_counter()
+        quiet(lambda: olib.oracle_optimal_estimation(r(oret), r(octl), r(ometa), r(o_obs), r(oatm), r(o_atm),
+                                                     None, None, chisq.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     r(nc), None))
+        cdt = time.perf_counter() - c0
+        line["cpu_baseline"] = {"value": 1 / cdt, "unit": "retrieval/s", "cores": ncores(), "kind": "port",
+                                "sample": "one whole retrieval of the same inputs by the FP64 oracle (its "
+                                          "error analysis skipped), %.1f s wall" % cdt}
+        tg, to = np.array(atm_i.t[:atm_i.np]), np.array(o_atm.t[:o_atm.np])
+        qg = np.array([atm_i.q[g][:atm_i.np] for g in (1, 2)])
+        qo = np.array([o_atm.q[g][:o_atm.np] for g in (1, 2)])
+        line["accuracy"] = {"max_abs_T_K": float(np.max(np.abs(tg - to))),
+                            "max_rel_q": float(np.max(np.abs(qg / qo - 1))),
+                            "cost_evaluations_gpu": len(costs), "cost_evaluations_oracle": nc.value,
+                            "final_chi2_gpu": float(costs[-1][1]), "final_chi2_oracle": float(chisq[nc.value - 1])}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -402,6 +529,8 @@ def main():
         return bench_surrogate(args, lib)
     if args.workload == "jacobian_limb_17":
         return bench_jacobian(args, lib)
+    if args.workload == "retrieval_limb_17":
+        return bench_retrieval(args, lib)
     w, ctl, aero, scattering, (argc, argv) = setup(lib, args.workload, args.nray)
     aero_p = C.byref(aero) if aero is not None else None
     if args.variant is not None:

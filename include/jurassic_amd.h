@@ -172,7 +172,7 @@ int jur_init(int device);
    FP32 with a compensated radiance sum), "kernel_fp64" (default 1: kernel
    and jur_kernel integrate the perturbed paths in FP64, since a 1 %
    perturbation of one level can change a radiance by less than FP32
-   resolves).
+   resolves; optimal_estimation's forward calls follow the same setting).
    Returns 0. */
 int jur_set_option(const char *key, double value);
 /* Statistics of the last call: "path_ms" (path-integration kernel time,
@@ -185,7 +185,9 @@ int jur_set_option(const char *key, double value);
    "fallback_points", "mlp_ms", "mlp_samples", "train_ms",
    "shared_grids" (1: the fast table layout is in use), "mie_spheres"
    (spheres of the last get_opt_prop), "kernel_ms" (wall time of the last
-   kernel call), "kernel_columns".  Unknown keys give NaN. */
+   kernel call), "kernel_columns"; running totals over all calls since
+   the library loaded: "calls_total", "path_ms_total",
+   "integrated_points_total", "pencils_total".  Unknown keys give NaN. */
 double jur_stat(const char *key);
 
 /* ---- Scattering-source surrogate (no reference counterpart) ---------- */

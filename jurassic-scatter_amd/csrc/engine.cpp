@@ -573,6 +573,11 @@ void end_call(hipStream_t s) {
   e.stats["path_ms"] = ms;
   e.stats["path_launches"] = (double)e.path_ev_used;
   e.stats["integrated_points"] = (double)pts;
+  // running totals over calls (a retrieval makes many)
+  e.stats["calls_total"] += 1;
+  e.stats["path_ms_total"] += ms;
+  e.stats["integrated_points_total"] += (double)pts;
+  e.stats["pencils_total"] += e.stats["pencils"];
 }
 }  // namespace
 
