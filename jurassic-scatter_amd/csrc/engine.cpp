@@ -481,7 +481,7 @@ void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays
     A.vplon = rays.vplon;
     A.vplat = rays.vplat;
     A.atm_idx = rays.atm_idx;
-    A.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw};
+    A.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw, atm.geo};
     A.rayds = ctl->rayds;
     A.raydz = ctl->raydz;
     A.transs = ctl->transs;
@@ -669,7 +669,9 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
                     const OutDev& out, hipStream_t s) {
   Engine& e = E();
   if (rays.nray <= 0) return;
-  if (ctl->ip != 1) JUR_FAIL("Only IP=1 (vertical profile interpolation) is supported!");
+  if (ctl->ip < 1 || ctl->ip > 3) JUR_FAIL("Unknown interpolation method, check IP!");
+  if (ctl->ip != atm.geo.ip && !(ctl->ip == 1 && atm.geo.ip == 0))
+    JUR_FAIL("The atmosphere was staged for another interpolation method (IP)!");
   const int sca_on = ctl->sca_n > 0;
   begin_call(s);
   if (sca_on && scattering > 0) {
@@ -679,7 +681,7 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
   }
   TraceBufs tb{&e.geo, &e.geo2, &e.rec, &e.npts, &e.tsurf, nullptr, nullptr, 0};
   // FP64 calls are not recorded (the graph key does not hold the precision)
-  const bool graph = engine_option_int("graph", 0) != 0 && engine_option_int("fp64", 0) == 0;
+  const bool graph = engine_option_int("graph", 0) != 0 && engine_option_int("fp64", 0) == 0 && ctl->ip == 1;
   if (graph && e.g_exec && e.g_key == graph_key(ctl, atm, rays, out, scattering, s)) {
     if (replay_forward(rays, s)) return;
   }

@@ -32,6 +32,56 @@ T* put(DBuf& b, const T* h, size_t n, hipStream_t s) {
   return d;
 }
 
+}  // namespace
+
+// intpol_atm_2d / _3d set-up (atmosphere.c:339-367, 441-445): profiles are
+// runs of equal (lon, lat); surface positions in Cartesian coordinates.
+DevGeo stage_geo(const ctl_t* ctl, const atm_t* atm, hipStream_t s) {
+  static DBuf b_idx, b_nz, b_x, b_lat;
+  DevGeo G{};
+  G.ip = ctl->ip;
+  if (ctl->ip == 1) return G;
+  if (ctl->ip != 2 && ctl->ip != 3) JUR_FAIL("Unknown interpolation method, check IP!");
+  const int n = atm->np;
+  std::vector<double> x;
+  if (ctl->ip == 2) {
+    std::vector<int> idx, nz;
+    double lon1 = -999, lat1 = -999;
+    for (int ip = 0; ip < n; ip++) {
+      if (atm->lon[ip] != lon1 || atm->lat[ip] != lat1) {
+        if ((int)idx.size() >= NPMAX) JUR_FAIL("Too many profiles!");
+        lon1 = atm->lon[ip];
+        lat1 = atm->lat[ip];
+        idx.push_back(ip);
+        nz.push_back(0);
+        double c[3];
+        geo2cart(0, lon1, lat1, c);
+        x.insert(x.end(), c, c + 3);
+      }
+      nz.back()++;
+    }
+    for (int v : nz)
+      if (v <= 1) JUR_FAIL("Cannot identify profiles. Check ordering of data points!");
+    G.nx = (int)idx.size();
+    G.pidx = put(b_idx, idx.data(), idx.size(), s);
+    G.pnz = put(b_nz, nz.data(), nz.size(), s);
+  } else {
+    for (int ip = 0; ip < n; ip++) {
+      double c[3];
+      geo2cart(0, atm->lon[ip], atm->lat[ip], c);
+      x.insert(x.end(), c, c + 3);
+    }
+    G.lat = put(b_lat, atm->lat, n, s);
+    G.cz = ctl->cz;
+    G.cx = ctl->cx;
+  }
+  G.px = put(b_x, x.data(), x.size(), s);
+  HIP_OK(hipStreamSynchronize(s));  // the host vectors end here
+  return G;
+}
+
+namespace {
+
 AtmSetDev stage_atm(const ctl_t* ctl, const atm_t* atm, hipStream_t s) {
   Staging& g = S();
   const int n = atm->np;
@@ -46,6 +96,7 @@ AtmSetDev stage_atm(const ctl_t* ctl, const atm_t* atm, hipStream_t s) {
   A.t = put(g.at, atm->t, n, s);
   A.q = put(g.aq, q.data(), q.size(), s);
   A.k = put(g.ak, k.data(), k.size(), s);
+  A.geo = stage_geo(ctl, atm, s);
   HIP_OK(hipStreamSynchronize(s));
   return A;
 }
@@ -248,7 +299,7 @@ extern "C" void raytrace(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, los_t
   HIP_OK(hipMemcpyAsync(&tsv, ts64.p, sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   const int R_ = rec_len(ctl->ng, ctl->nw);
-  DevAtm M = {A.z, A.p, A.t, A.q, A.k, A.nlev, ctl->ng, ctl->nw};
+  DevAtm M = {A.z, A.p, A.t, A.q, A.k, A.nlev, ctl->ng, ctl->nw, A.geo};
   const double4* G = (const double4*)(final2 ? geo2.p : geo.p);
   double* zll = (double*)g.zll.need(3 * (size_t)np * sizeof(double) + 8);
   double* pt = (double*)g.pt.need(2 * (size_t)np * sizeof(double) + 8);
@@ -328,6 +379,7 @@ static void check_batch(const ctl_t* ctl, const jur_batch_t* b) {
   if (!b->z || !b->p || !b->t || (!b->q && ctl->ng > 0) || !b->obsz || !b->vpz || !b->rad || !b->tau)
     JUR_FAIL("Missing batch array!");
   if (b->nlev > NPMAX) JUR_FAIL("Too many data points!");
+  if (ctl->ip != 1) JUR_FAIL("The batched entry point takes vertical profiles (IP=1)!");
 }
 
 extern "C" int jur_formod_batch_device(ctl_t* ctl, const jur_batch_t* b, const aero_t* aero, int scattering,

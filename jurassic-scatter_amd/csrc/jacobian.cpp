@@ -90,6 +90,7 @@ void run_copies(ctl_t* ctl, const atm_t* atm, const obs_t* obs, const aero_t* ae
   A.t = dt;
   A.q = upload(b.q, q, s);
   A.k = upload(b.k, k, s);
+  A.geo = stage_geo(ctl, atm, s);
   if (ctl->hydz >= 0 && nl > 0) {
     // hydrostatic (atmosphere.c:163-184) on every copy: profiles are runs of
     // equal (lon, lat)

@@ -164,6 +164,20 @@ __host__ __device__ inline double angle3(const double* a, const double* b) {
 
 // Atmosphere set on the device: profiles of nlev levels, ray-independent
 // layout [a][lev], q [a][ig][lev], k [a][iw][lev] (FP64).
+// IP 2 and 3 (atmosphere.c:324-483): the profile is a set of points with
+// geographic positions.  Surface positions are built on the host; the
+// value arrays of DevAtm hold the points ([a][np] per copy a).  ip <= 1
+// (the zero default): a single vertical profile.
+struct DevGeo {
+  int ip;
+  int nx;              // IP 2: profiles (runs of equal lon, lat)
+  const int* pidx;     // IP 2: first point of each profile
+  const int* pnz;      // IP 2: points of each profile
+  const double* px;    // IP 2: [nx][3]; IP 3: [np][3] surface positions
+  const double* lat;   // IP 3: latitude of every point
+  double cz, cx;       // IP 3: vertical / horizontal influence ranges [km]
+};
+
 struct DevAtm {
   const double* z;
   const double* p;
@@ -171,6 +185,7 @@ struct DevAtm {
   const double* q;
   const double* k;
   int nlev, ng, nw;
+  DevGeo geo;
 };
 
 // intpol_atm_1d (atmosphere.c:298-320): p exponential (EXP, jurassic.h:55-58),
@@ -221,4 +236,96 @@ __device__ inline double atm_k(const DevAtm& A, int a, int iw, double z0, int ip
   const double* z = A.z + (size_t)a * A.nlev;
   const double* k = A.k + ((size_t)a * A.nw + iw) * A.nlev;
   return lin(z[ip], k[ip], z[ip + 1], k[ip + 1], z0);
+}
+
+// Where one evaluation at (z0, lon0, lat0) reads the point set
+// (intpol_atm_geo, atmosphere.c:269-294).  IP 1 and 2: brackets j0 (and j1)
+// of the one or two profiles, weight r of the second (intpol_atm_2d,
+// :324-418: the two profiles nearest at the surface, the point projected
+// onto the chord between them).  IP 3: the point itself (:422-483).
+struct GeoLoc {
+  int j0, j1;
+  double r;
+  double z0, lat0, x0[3];
+};
+__device__ inline double dist2_3(const double* a, const double* b) {
+  double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+__device__ inline GeoLoc geo_locate(const DevAtm& A, int a, double z0, double lon0, double lat0) {
+  GeoLoc L;
+  L.j0 = L.j1 = 0;
+  L.r = 0;
+  L.z0 = z0;
+  L.lat0 = lat0;
+  if (A.geo.ip >= 2) geo2cart(0, lon0, lat0, L.x0);
+  const double* z = A.z + (size_t)a * A.nlev;
+  if (A.geo.ip <= 1) {
+    L.j0 = L.j1 = locate_mono<double>(z, A.nlev, z0);
+  } else if (A.geo.ip == 2) {
+    double d0 = 1e99, d1 = 1e99;
+    int i0 = 0, i1 = 0;
+    for (int ix = 0; ix < A.geo.nx; ix++) {
+      const double dh = dist2_3(L.x0, A.geo.px + 3 * ix);
+      if (dh <= d0) {
+        d1 = d0;
+        i1 = i0;
+        d0 = dh;
+        i0 = ix;
+      } else if (dh <= d1) {
+        d1 = dh;
+        i1 = ix;
+      }
+    }
+    const int b0 = A.geo.pidx[i0], b1 = A.geo.pidx[i1];
+    L.j0 = b0 + locate_mono<double>(z + b0, A.geo.pnz[i0], z0);
+    L.j1 = b1 + locate_mono<double>(z + b1, A.geo.pnz[i1], z0);
+    const double x2 = dist2_3(A.geo.px + 3 * i0, A.geo.px + 3 * i1), x = sqrt(x2);
+    const double r0 = (d0 - d1 + x2) / (2 * x), r1 = x - r0;
+    L.r = r0 <= 0 ? 0.0 : (r1 <= 0 ? 1.0 : r0 / (r0 + r1));
+  }
+  return L;
+}
+// A field of copy a (f points at its first value) at the located point:
+// exponential in altitude for the pressure (EXP, jurassic.h:55-58), linear
+// otherwise.
+template <bool EXPO>
+__device__ inline double geo_1d(const double* z, const double* f, int j, double z0) {
+  const double y0 = f[j], y1 = f[j + 1];
+  if (EXPO && y0 > 0 && y1 > 0) return y0 * exp(log(y1 / y0) / (z[j + 1] - z[j]) * (z0 - z[j]));
+  return lin(z[j], y0, z[j + 1], y1, z0);
+}
+template <bool EXPO>
+__device__ inline double geo_field(const DevAtm& A, int a, const GeoLoc& L, const double* f) {
+  const double* z = A.z + (size_t)a * A.nlev;
+  if (A.geo.ip <= 1) return geo_1d<EXPO>(z, f, L.j0, L.z0);
+  if (A.geo.ip == 2)
+    return (1 - L.r) * geo_1d<EXPO>(z, f, L.j0, L.z0) + L.r * geo_1d<EXPO>(z, f, L.j1, L.z0);
+  // IP 3: weights (1 - dz/CZ)(1 - dh/CX) of the points within range, in
+  // the reference's point order
+  double v = 0, wsum = 0;
+  const double cz = A.geo.cz, cx = A.geo.cx;
+  for (int ip = 0; ip < A.nlev; ip++) {
+    const double dz = fabs(z[ip] - L.z0);
+    if (dz > cz) continue;
+    if (fabs(A.geo.lat[ip] - L.lat0) * 111.13 > cx) continue;
+    const double dx2 = dist2_3(L.x0, A.geo.px + 3 * ip);
+    if (dx2 > cx * cx) continue;
+    const double w = (1 - dz / cz) * (1 - sqrt(dx2) / cx);
+    wsum += w;
+    v += w * f[ip];
+  }
+  return v / wsum;
+}
+__device__ inline AtmPoint geo_pt(const DevAtm& A, int a, const GeoLoc& L) {
+  AtmPoint r;
+  r.p = geo_field<true>(A, a, L, A.p + (size_t)a * A.nlev);
+  r.t = geo_field<false>(A, a, L, A.t + (size_t)a * A.nlev);
+  return r;
+}
+__device__ inline double geo_q(const DevAtm& A, int a, int ig, const GeoLoc& L) {
+  return geo_field<false>(A, a, L, A.q + ((size_t)a * A.ng + ig) * A.nlev);
+}
+__device__ inline double geo_k(const DevAtm& A, int a, int iw, const GeoLoc& L) {
+  return geo_field<false>(A, a, L, A.k + ((size_t)a * A.nw + iw) * A.nlev);
 }

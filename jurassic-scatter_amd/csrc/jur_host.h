@@ -76,7 +76,12 @@ struct RaySetDev {
 struct AtmSetDev {
   int natm = 0, nlev = 0;
   const double *z = nullptr, *p = nullptr, *t = nullptr, *q = nullptr, *k = nullptr;
+  DevGeo geo{};  // IP 2 / 3: geographic positions of the points (shared by the copies)
 };
+// The geographic part of an atmosphere for ctl->ip (2: profiles, 3: points;
+// 1: none), on the device; exits like the reference on unknown IP or
+// profiles that cannot be identified.
+DevGeo stage_geo(const ctl_t* ctl, const atm_t* atm, hipStream_t s);
 struct OutDev {
   float* rad = nullptr;   // [nd][nray]
   float* tau = nullptr;   // [nd][nray]

@@ -99,6 +99,10 @@ __device__ MarchOut march_ray(const DevAtm& M, int a, double obsz, double obslon
       // point is kept as found here.
       if (np > 0) cur = chord_point(prev, cur, (stop == 2 ? zmin : zmax));
       cur.ds = 0.;
+    } else if (M.geo.ip >= 2) {
+      AtmPoint ap = geo_pt(M, a, geo_locate(M, a, z, lon, lat));
+      pl = ap.p;
+      tl = ap.t;
     } else {
       ih = atm_locate_h(M, a, z, ih);
       AtmPoint ap = atm_pt(M, a, z, ih);
@@ -121,7 +125,25 @@ __device__ MarchOut march_ray(const DevAtm& M, int a, double obsz, double obslon
     }
     double n = (refrac && z <= 60) ? 1 + refractivity({pl, tl}) : 1;
     for (int i = 0; i < 3; i++) ex1[i] = ex0[i] * n;
-    if (refrac && z <= 60) {
+    if (refrac && z <= 60 && M.geo.ip >= 2) {
+      // IP 2 / 3: the gradient points' own positions (lineofsight.c:123-146)
+      auto at = [&](const double* xp) {
+        double zz, lo, la;
+        cart2geo(xp, &zz, &lo, &la);
+        return geo_pt(M, a, geo_locate(M, a, zz, lo, la));
+      };
+      for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
+      AtmPoint ap = at(xh);
+      double n0 = refractivity(ap);
+      for (int i = 0; i < 3; i++) {
+        xh[i] += 0.02;
+        ap = at(xh);
+        ng[i] = (refractivity(ap) - n0) / 0.02;
+        xh[i] -= 0.02;
+      }
+      pl = ap.p;
+      tl = ap.t;
+    } else if (refrac && z <= 60) {
       // the refractivity gradient needs only the altitudes of the points
       for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
       double zz = cart2z(xh);

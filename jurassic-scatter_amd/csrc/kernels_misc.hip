@@ -136,8 +136,8 @@ __global__ void k_los64(DevAtm M, int a, const double4* geo, const float* rec, i
   int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np) return;
   double4 g = geo[ip];
-  int il = atm_locate(M, a, g.x);
-  AtmPoint ap = atm_pt(M, a, g.x, il);
+  const GeoLoc L = geo_locate(M, a, g.x, g.y, g.z);
+  AtmPoint ap = geo_pt(M, a, L);
   out_zll[3 * ip] = g.x;
   out_zll[3 * ip + 1] = g.y;
   out_zll[3 * ip + 2] = g.z;
@@ -145,11 +145,11 @@ __global__ void k_los64(DevAtm M, int a, const double4* geo, const float* rec, i
   out_pt[2 * ip + 1] = ap.t;
   out_ds[ip] = g.w;
   for (int ig = 0; ig < ng; ig++) {
-    double q = atm_q(M, a, ig, g.x, il);
+    double q = geo_q(M, a, ig, L);
     out_q[ip * NGMAX + ig] = q;
     out_u[ip * NGMAX + ig] = 10 * q * ap.p / (JUR_KBOLTZ * ap.t) * g.w;
   }
-  for (int iw = 0; iw < nw; iw++) out_k[ip * NWMAX + iw] = atm_k(M, a, iw, g.x, il);
+  for (int iw = 0; iw < nw; iw++) out_k[ip * NWMAX + iw] = geo_k(M, a, iw, L);
   out_af[ip] = rec[(size_t)ip * R + REC_AF];
   out_ai[ip] = (int)rec[(size_t)ip * R + REC_AI];
 }

@@ -188,8 +188,10 @@ __global__ void k_records(TraceArgs A) {
   pr[REC_AF] = af;
   pr[REC_AI] = ai;
   const int uo = rec_u(A.nw);
-  const int il = atm_locate(M, a, g.x);
-  const AtmPoint ap = atm_pt(M, a, g.x, il);
+  // the atmosphere at the point (IP 1: its altitude bracket; IP 2 / 3 also
+  // its longitude and latitude)
+  const GeoLoc L = geo_locate(M, a, g.x, g.y, g.z);
+  const AtmPoint ap = geo_pt(M, a, L);
   pr[REC_P] = (float)ap.p;
   pr[REC_T] = (float)ap.t;
   pr[REC_DS] = (float)ds;
@@ -203,7 +205,7 @@ __global__ void k_records(TraceArgs A) {
   }
   double qh = 0;
   for (int ig = 0; ig < M.ng; ig++) {
-    const double q = atm_q(M, a, ig, g.x, il);
+    const double q = geo_q(M, a, ig, L);
     if (ig == A.ig_h2o) qh = q;
     const double u = 10 * q * ap.p / (JUR_KBOLTZ * ap.t) * ds;
     pr[uo + ig] = (float)u;
@@ -212,7 +214,7 @@ __global__ void k_records(TraceArgs A) {
   pr[REC_QH2O] = (float)qh;
   if (pd) pd[REC_QH2O] = qh;
   for (int iw = 0; iw < M.nw; iw++) {
-    const double k = atm_k(M, a, iw, g.x, il);
+    const double k = geo_k(M, a, iw, L);
     pr[REC_K + iw] = (float)k;
     if (pd) pd[REC_K + iw] = k;
   }

@@ -215,6 +215,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
                      hipStream_t s) {
   State& st = ST();
   const int nd = ctl->nd, Rl = rec_len(ctl->ng, ctl->nw);
+  if (ctl->ip != 1) JUR_FAIL("Scattering with IP != 1 (srcfunc_sca_3d) is not supported!");
   // surrogate for the thermal source at the primaries' points (option
   // "surrogate", once a network is trained or loaded); "collect" keeps the
   // plain method's (features, thermal J) pairs of those points
@@ -303,7 +304,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
       F.npt = V.npt;
       F.nd = nd;
       F.pts = pts;
-      F.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw};
+      F.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw, atm.geo};
       F.ig_h2o = engine_find_emitter(ctl, "H2O");
       F.beta_e = engine_beta_e();
       F.beta_s = engine_beta_s();
@@ -365,7 +366,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     int* sidx = (int*)V.sun_idx.need((size_t)(V.npt + 1) * sizeof(int));
     V.sun_tmp.alloc(V.npt);
     SunArgs SA;
-    SA.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw};
+    SA.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw, atm.geo};
     SA.rayds = ctl->rayds;
     SA.raydz = ctl->raydz;
     SA.refrac = ctl->refrac;

@@ -727,18 +727,123 @@ rest:
    Atmosphere
    ------------------------------------------------------------------------ */
 
-/* intpol_atm_1d (atmosphere.c:298-320), the IP=1 branch of intpol_atm_geo
-   (atmosphere.c:269-294). */
-static void o_intpol_atm(const ctl_t *ctl, const atm_t *atm, double z0, double *p, double *t,
-                         double *q, double *k) {
-  if (ctl->ip != 1) OFAIL("Only IP=1 (vertical profile interpolation) is supported!");
-  int ip = o_locate(atm->z, atm->np, z0);
+/* intpol_atm_1d (atmosphere.c:298-320) on the profile of n points from idx0 */
+static void o_intpol_1d(const ctl_t *ctl, const atm_t *atm, int idx0, int n, double z0, double *p, double *t,
+                        double *q, double *k) {
+  int ip = idx0 + o_locate(&atm->z[idx0], n, z0);
   *p = o_exp(atm->z[ip], atm->p[ip], atm->z[ip + 1], atm->p[ip + 1], z0);
   *t = o_lin(atm->z[ip], atm->t[ip], atm->z[ip + 1], atm->t[ip + 1], z0);
   for (int ig = 0; ig < ctl->ng; ig++)
     q[ig] = o_lin(atm->z[ip], atm->q[ig][ip], atm->z[ip + 1], atm->q[ig][ip + 1], z0);
   for (int iw = 0; iw < ctl->nw; iw++)
     k[iw] = o_lin(atm->z[ip], atm->k[iw][ip], atm->z[ip + 1], atm->k[iw][ip + 1], z0);
+}
+
+static double o_dist2(const double *a, const double *b) {
+  return (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+}
+
+/* intpol_atm_2d (atmosphere.c:324-418): vertical interpolation in the two
+   profiles (runs of equal lon, lat) nearest at the surface, then linear
+   along the chord between them.  The reference caches the profile list in
+   statics on the first call per atmosphere; it is rebuilt here each time. */
+static void o_intpol_2d(const ctl_t *ctl, const atm_t *atm, double z0, double lon0, double lat0, double *p,
+                        double *t, double *q, double *k) {
+  /* per thread: the oracle traces rays in parallel */
+  static _Thread_local double x1[NPMAX][3];
+  static _Thread_local int idx[NPMAX], nz[NPMAX];
+  int nx = 0;
+  double lon1 = -999, lat1 = -999;
+  for (int ip = 0; ip < atm->np; ip++) {
+    if (atm->lon[ip] != lon1 || atm->lat[ip] != lat1) {
+      if ((++nx) > NPMAX) OFAIL("Too many profiles!");
+      nz[nx - 1] = 0;
+      lon1 = atm->lon[ip];
+      lat1 = atm->lat[ip];
+      o_geo2cart(0, lon1, lat1, x1[nx - 1]);
+      idx[nx - 1] = ip;
+    }
+    nz[nx - 1]++;
+  }
+  for (int ix = 0; ix < nx; ix++)
+    if (nz[ix] <= 1) OFAIL("Cannot identify profiles. Check ordering of data points!");
+  double x0[3], d0 = 1e99, d1 = 1e99;
+  int i0 = 0, i1 = 0;
+  o_geo2cart(0, lon0, lat0, x0);
+  for (int ix = 0; ix < nx; ix++) {
+    double dh = o_dist2(x0, x1[ix]);
+    if (dh <= d0) {
+      d1 = d0;
+      i1 = i0;
+      d0 = dh;
+      i0 = ix;
+    } else if (dh <= d1) {
+      d1 = dh;
+      i1 = ix;
+    }
+  }
+  double p0, p1, t0, t1, q0[NGMAX], q1[NGMAX], k0[NWMAX], k1[NWMAX];
+  o_intpol_1d(ctl, atm, idx[i0], nz[i0], z0, &p0, &t0, q0, k0);
+  o_intpol_1d(ctl, atm, idx[i1], nz[i1], z0, &p1, &t1, q1, k1);
+  double x2 = o_dist2(x1[i0], x1[i1]), x = sqrt(x2);
+  double r0 = (d0 - d1 + x2) / (2 * x), r1 = x - r0, r;
+  if (r0 <= 0)
+    r = 0;
+  else if (r1 <= 0)
+    r = 1;
+  else
+    r = r0 / (r0 + r1);
+  *p = (1 - r) * p0 + r * p1;
+  *t = (1 - r) * t0 + r * t1;
+  for (int ig = 0; ig < ctl->ng; ig++) q[ig] = (1 - r) * q0[ig] + r * q1[ig];
+  for (int iw = 0; iw < ctl->nw; iw++) k[iw] = (1 - r) * k0[iw] + r * k1[iw];
+}
+
+/* intpol_atm_3d (atmosphere.c:422-483): average of the points within CZ
+   vertically and CX horizontally, weighted (1 - dz/CZ)(1 - dh/CX). */
+static void o_intpol_3d(const ctl_t *ctl, const atm_t *atm, double z0, double lon0, double lat0, double *p,
+                        double *t, double *q, double *k) {
+  double x0[3], x1[3], wsum = 0;
+  *p = *t = 0;
+  for (int ig = 0; ig < ctl->ng; ig++) q[ig] = 0;
+  for (int iw = 0; iw < ctl->nw; iw++) k[iw] = 0;
+  for (int ip = 0; ip < atm->np; ip++) {
+    double dz = fabs(atm->z[ip] - z0);
+    if (dz > ctl->cz) continue;
+    if (fabs(atm->lat[ip] - lat0) * 111.13 > ctl->cx) continue;
+    o_geo2cart(0, lon0, lat0, x0);
+    o_geo2cart(0, atm->lon[ip], atm->lat[ip], x1);
+    double dx2 = o_dist2(x0, x1);
+    if (dx2 > ctl->cx * ctl->cx) continue;
+    double w = (1 - dz / ctl->cz) * (1 - sqrt(dx2) / ctl->cx);
+    wsum += w;
+    *p += w * atm->p[ip];
+    *t += w * atm->t[ip];
+    for (int ig = 0; ig < ctl->ng; ig++) q[ig] += w * atm->q[ig][ip];
+    for (int iw = 0; iw < ctl->nw; iw++) k[iw] += w * atm->k[iw][ip];
+  }
+  *p /= wsum;
+  *t /= wsum;
+  for (int ig = 0; ig < ctl->ng; ig++) q[ig] /= wsum;
+  for (int iw = 0; iw < ctl->nw; iw++) k[iw] /= wsum;
+}
+
+/* intpol_atm_geo (atmosphere.c:269-294) */
+static void o_intpol_atm(const ctl_t *ctl, const atm_t *atm, double z0, double lon0, double lat0, double *p,
+                         double *t, double *q, double *k) {
+  if (ctl->ip == 1)
+    o_intpol_1d(ctl, atm, 0, atm->np, z0, p, t, q, k);
+  else if (ctl->ip == 2)
+    o_intpol_2d(ctl, atm, z0, lon0, lat0, p, t, q, k);
+  else if (ctl->ip == 3)
+    o_intpol_3d(ctl, atm, z0, lon0, lat0, p, t, q, k);
+  else
+    OFAIL("Unknown interpolation method, check IP!");
+}
+
+void oracle_intpol_atm_geo(const ctl_t *ctl, const atm_t *atm, double z0, double lon0, double lat0, double *p,
+                           double *t, double *q, double *k) {
+  o_intpol_atm(ctl, atm, z0, lon0, lat0, p, t, q, k);
 }
 
 /* gravity (atmosphere.c:153-159) */
@@ -815,7 +920,7 @@ static void o_intersection(const ctl_t *ctl, const atm_t *atm, double znew, cons
   o_geo2cart(los->z[ip], los->lon[ip], los->lat[ip], x2);
   for (int i = 0; i < 3; i++) x2[i] = x1[i] + frac * (x2[i] - x1[i]);
   o_cart2geo(x2, &los2->z[jp], &los2->lon[jp], &los2->lat[jp]);
-  o_intpol_atm(ctl, atm, los2->z[jp], &los2->p[jp], &los2->t[jp], los2->q[jp], los2->k[jp]);
+  o_intpol_atm(ctl, atm, los2->z[jp], los2->lon[jp], los2->lat[jp], &los2->p[jp], &los2->t[jp], los2->q[jp], los2->k[jp]);
 }
 
 /* tangent_point (lineofsight.c:411-452) */
@@ -1051,11 +1156,11 @@ static void o_raytrace(const ctl_t *ctl, const atm_t *atm, oray_t *ray, const ae
       if (los->np > 0)
         o_intersection(ctl, atm, (z < zmin + 0.001 ? zmin : zmax), los, los->np, los, los->np);
       else
-        o_intpol_atm(ctl, atm, z, &los->p[0], &los->t[0], los->q[0], los->k[0]);
+        o_intpol_atm(ctl, atm, z, lon, lat, &los->p[0], &los->t[0], los->q[0], los->k[0]);
       los->ds[los->np] = 0.;
     }
     if (stop == 0) {
-      o_intpol_atm(ctl, atm, z, &p, &t, q, k);
+      o_intpol_atm(ctl, atm, z, lon, lat, &p, &t, q, k);
       los->lon[los->np] = lon;
       los->lat[los->np] = lat;
       los->z[los->np] = z;
@@ -1079,12 +1184,12 @@ static void o_raytrace(const ctl_t *ctl, const atm_t *atm, oray_t *ray, const ae
     if (ctl->refrac && z <= zrefrac) {
       for (int i = 0; i < 3; i++) xh[i] = x[i] + 0.5 * ds * ex0[i];
       o_cart2geo(xh, &z, &lon, &lat);
-      o_intpol_atm(ctl, atm, z, &p, &t, q, k);
+      o_intpol_atm(ctl, atm, z, lon, lat, &p, &t, q, k);
       n = 7.753e-05 * p / t;
       for (int i = 0; i < 3; i++) {
         xh[i] += h;
         o_cart2geo(xh, &z, &lon, &lat);
-        o_intpol_atm(ctl, atm, z, &p, &t, q, k);
+        o_intpol_atm(ctl, atm, z, lon, lat, &p, &t, q, k);
         naux = 7.753e-05 * p / t;
         ng[i] = (naux - n) / h;
         xh[i] -= h;

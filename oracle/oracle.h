@@ -76,6 +76,10 @@ double oracle_planck(double t, double nu);          /* forwardmodel.c:569-573 */
 double oracle_brightness(double rad, double nu);    /* forwardmodel.c:7-11 */
 void oracle_suncoord(double sec, double lon, double lat, double *azi, double *sza); /* scatter.c:999-1041 */
 void oracle_hydrostatic(ctl_t *ctl, atm_t *atm);    /* atmosphere.c:163-243 */
+/* intpol_atm_geo (atmosphere.c:269-483): IP 1 vertical profile, 2 nearest two
+   profiles, 3 distance-weighted average within CZ / CX. */
+void oracle_intpol_atm_geo(const ctl_t *ctl, const atm_t *atm, double z0, double lon0, double lat0, double *p,
+                           double *t, double *q, double *k);
 /* Continuum absorption of one LOS point for all channels (forwardmodel.c:120-152). */
 void oracle_continua(ctl_t *ctl, double p, double t, const double *q, const double *u,
                      double ds, const double *k, double *beta);

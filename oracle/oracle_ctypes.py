@@ -37,6 +37,7 @@ def load():
         "oracle_brightness": (C.c_double, [C.c_double, C.c_double]),
         "oracle_suncoord": (None, [C.c_double, C.c_double, C.c_double, D, D]),
         "oracle_hydrostatic": (None, [P, P]),
+        "oracle_intpol_atm_geo": (None, [P, P, C.c_double, C.c_double, C.c_double, D, D, D, D]),
         "oracle_continua": (None, [P, C.c_double, C.c_double, D, D, C.c_double, D, D]),
         "oracle_ctmn2": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "oracle_ctmo2": (C.c_double, [C.c_double, C.c_double, C.c_double]),
