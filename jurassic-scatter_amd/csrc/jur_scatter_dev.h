@@ -73,6 +73,15 @@ void launch_children(int npt, int nalpha, const ScaPt* pts, const double* alpha,
 void launch_sun(int npt, const ScaPt* pts, const SunArgs& S, int* valid, double* theta, RayArrays out,
                 hipStream_t s);
 void launch_angular(const AngArgs& G, hipStream_t s);
+// srcfunc_sca_3d (scatter.c:811-895), the method for IP 2 / 3: a direct
+// fan of DIR3D_NT x DIR3D_NPHI pencils per point (children k DIR3D_N +
+// itheta DIR3D_NPHI + iphi) and its weighted sum (thermal source) plus the
+// solar term.  G.nalpha is not used.
+#define DIR3D_NT 180
+#define DIR3D_NPHI 180
+#define DIR3D_N (DIR3D_NT * DIR3D_NPHI)
+void launch_children3d(int npt, const ScaPt* pts, RayArrays out, hipStream_t s);
+void launch_angular3d(const AngArgs& G, hipStream_t s);
 void launch_sun_compact(int npt, const int* valid, const int* pos, int* sun_idx, hipStream_t s);
 void launch_gather_rays(int npt, const int* valid, const int* pos, RayArrays in, RayArrays out, hipStream_t s);
 size_t scan_temp_bytes(int n);

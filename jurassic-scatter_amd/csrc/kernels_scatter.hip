@@ -320,6 +320,84 @@ __global__ void k_gather_rays(int npt, const int* valid, const int* pos, RayArra
   out.atm[j] = in.atm[k];
 }
 
+// ---- srcfunc_sca_3d (scatter.c:811-895) ---------------------------------------
+// The direct fan of each point: view points 10 km away in the directions of
+// polar angle theta2 = (0.5 + itheta) pi / 180 from the LOS tangent and
+// azimuth phi = (0.5 + iphi) 2 pi / 180 (basis of bascoord(dx, x)).
+__global__ void k_children3d(int npt, const ScaPt* pts, RayArrays out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)npt * DIR3D_N) return;
+  const int k = (int)(i / DIR3D_N), c = (int)(i % DIR3D_N);
+  const int itheta = c / DIR3D_NPHI, iphi = c % DIR3D_NPHI;
+  const ScaPt P = pts[k];
+  double sx[3], sy[3], sz[3], xv[3];
+  bascoord(P.dx, P.x, sx, sy, sz);
+  const double theta2 = (0.5 + itheta) / DIR3D_NT * M_PI, phi = (0.5 + iphi) / DIR3D_NPHI * 2 * M_PI;
+  for (int j = 0; j < 3; j++)
+    xv[j] = P.x[j] + 10 * sin(theta2) * sin(phi) * sx[j] + 10 * sin(theta2) * cos(phi) * sy[j] +
+            10 * cos(theta2) * sz[j];
+  double z, lon, lat;
+  cart2geo(P.x, &z, &lon, &lat);
+  out.obsz[i] = z;
+  out.obslon[i] = lon;
+  out.obslat[i] = lat;
+  cart2geo(xv, &z, &lon, &lat);
+  out.vpz[i] = z;
+  out.vplon[i] = lon;
+  out.vplat[i] = lat;
+  out.time[i] = P.time;
+  out.atm[i] = P.atm;
+}
+
+// Weight of polar angle bin it of one channel: solid angle times the phase
+// function interpolated at theta2 (scatter.c:869-879).
+__device__ inline double w3d(const double* theta_grid, const double* p, int it) {
+  const double theta2 = (0.5 + it) / DIR3D_NT * M_PI;
+  const int idx = locate_mono<double>(theta_grid, NTHETA, theta2);
+  const double phase2 = lin(theta_grid[idx], p[idx], theta_grid[idx + 1], p[idx + 1], theta2);
+  return M_PI / DIR3D_NT * 2 * M_PI * sin(theta2) / DIR3D_NPHI * phase2;
+}
+__device__ inline double block_sum256(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+// One block per (point, channel): J = sum_k w_k rad_k / W, W the sum of the
+// weights over the fan AND over all channels (scatter.c:883, 891, as the
+// reference normalises), then the solar term as for srcfunc_sca_1d.
+__global__ void __launch_bounds__(256) k_angular3d(AngArgs G) {
+  __shared__ double wl[DIR3D_NT];
+  __shared__ double red[4];
+  const int k = blockIdx.x, id = blockIdx.y;
+  const ScaPt P = G.pts[k];
+  const double* ph = G.phase + (size_t)P.il * G.nd * NTHETA;
+  const double* p = ph + (size_t)id * NTHETA;
+  for (int it = threadIdx.x; it < DIR3D_NT; it += blockDim.x) wl[it] = w3d(G.theta_grid, p, it);
+  double wpart = 0;
+  for (int j = threadIdx.x; j < G.nd * DIR3D_NT; j += blockDim.x)
+    wpart += w3d(G.theta_grid, ph + (size_t)(j / DIR3D_NT) * NTHETA, j % DIR3D_NT);
+  __syncthreads();
+  const float* r = G.rad + (size_t)id * G.nchild + (size_t)k * DIR3D_N;
+  double num = 0;
+  for (int c = threadIdx.x; c < DIR3D_N; c += blockDim.x) num += wl[c / DIR3D_NPHI] * (double)r[c];
+  num = block_sum256(num, red);
+  __syncthreads();
+  const double wsum = DIR3D_NPHI * block_sum256(wpart, red);
+  if (threadIdx.x != 0) return;
+  double src = num / wsum;
+  if (G.jth) G.jth[(size_t)k * G.nd + id] = (float)src;
+  const int si = G.sun_idx[k];
+  if (si >= 0) {
+    const double t2 = G.sun_theta[k];
+    int idx = locate_mono<double>(G.theta_grid, NTHETA, t2);
+    double p2 = lin(G.theta_grid[idx], p[idx], G.theta_grid[idx + 1], p[idx + 1], t2);
+    src += 6.764e-5 * p2 * G.bsun[id] * (double)G.tau_sun[(size_t)id * G.nsun + si];
+  }
+  G.J[(size_t)k * G.nd + id] = (float)src;
+}
+
 // ---- launchers --------------------------------------------------------------
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -359,6 +437,13 @@ void launch_angular(const AngArgs& G, hipStream_t s) {
     case 26: launch_angular_t<26>(G, s); break;
     default: launch_angular_t<28>(G, s); break;
   }
+}
+void launch_children3d(int npt, const ScaPt* pts, RayArrays out, hipStream_t s) {
+  const long n = (long)npt * DIR3D_N;
+  if (n > 0) hipLaunchKernelGGL(k_children3d, dim3(nblk(n, 128)), dim3(128), 0, s, npt, pts, out);
+}
+void launch_angular3d(const AngArgs& G, hipStream_t s) {
+  if (G.npt > 0 && G.nd > 0) hipLaunchKernelGGL(k_angular3d, dim3(G.npt, G.nd), dim3(256), 0, s, G);
 }
 void launch_sun_compact(int npt, const int* valid, const int* pos, int* sun_idx, hipStream_t s) {
   if (npt > 0) hipLaunchKernelGGL(k_sun_compact, dim3(nblk(npt, 256)), dim3(256), 0, s, npt, valid, pos, sun_idx);

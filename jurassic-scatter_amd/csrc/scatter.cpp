@@ -215,12 +215,14 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
                      hipStream_t s) {
   State& st = ST();
   const int nd = ctl->nd, Rl = rec_len(ctl->ng, ctl->nw);
-  if (ctl->ip != 1) JUR_FAIL("Scattering with IP != 1 (srcfunc_sca_3d) is not supported!");
+  // IP 2 / 3: srcfunc_sca_3d's direct fans (scatter.c:654-657)
+  const bool dir3d = ctl->ip != 1;
   // surrogate for the thermal source at the primaries' points (option
   // "surrogate", once a network is trained or loaded); "collect" keeps the
   // plain method's (features, thermal J) pairs of those points
-  const bool use_sur = engine_option_int("surrogate", 0) && mlp_trained();
-  const bool collect = engine_option_int("collect", 0) != 0;
+  // the surrogate learns srcfunc_sca_1d's source: vertical profiles only
+  const bool use_sur = engine_option_int("surrogate", 0) && mlp_trained() && !dir3d;
+  const bool collect = engine_option_int("collect", 0) != 0 && !dir3d;
   st.ncollect = 0;
   engine_set_stat("surrogate_points", 0);
   engine_set_stat("fallback_points", 0);
@@ -353,10 +355,13 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     }
     // secondary fans -> level L+1
     Level& W = lv[L + 1];
-    long nchild = (long)nfan * nalpha;
+    long nchild = (long)nfan * (dir3d ? DIR3D_N : nalpha);
     if (nchild > 0x7fffffffL) JUR_FAIL("Too many queue items!");
     W.own.alloc((int)nchild);
-    launch_children(nfan, nalpha, fan_pts, st.alpha.as<double>(), W.own.arrays(), s);
+    if (dir3d)
+      launch_children3d(nfan, fan_pts, W.own.arrays(), s);
+    else
+      launch_children(nfan, nalpha, fan_pts, st.alpha.as<double>(), W.own.arrays(), s);
     W.rays = W.own.view();
     pencils += nchild;
     // solar rays of the points
@@ -436,7 +441,10 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
       G.ysd = mlp_norm().ysd;
     }
     if (L == 0 && collect && V.npt > 0) G.jth = (float*)st.cjth.need((size_t)V.npt * nd * sizeof(float));
-    launch_angular(G, s);
+    if (dir3d)
+      launch_angular3d(G, s);
+    else
+      launch_angular(G, s);
     if (L == 0) {
       integrate_set(ctl, atm, V.rays, S, J, V.jbase.as<int>(), out.rad, out.tau, out.tpz, out.tplon, out.tplat, s,
                     out.rad64, out.tau64);

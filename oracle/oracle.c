@@ -1699,12 +1699,69 @@ static void o_srcfunc_sca_sun(ctl_t *ctl, const atm_t *atm, const aero_t *aero, 
   o_los_pop();
 }
 
+/* srcfunc_sca_3d (scatter.c:811-895): direct pencils 10 km long on a
+   180 x 180 grid of directions about the LOS tangent (polar angle theta2
+   from it, azimuth phi), weighted by solid angle and the phase function at
+   theta2.  The normalisation sums the weights over all channels
+   (scatter.c:883, 891), as the reference does. */
+static void o_srcfunc_sca_3d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double time,
+                             const double *x, const double *dx, int il, double *src_sca,
+                             int scattering) {
+  enum { NT2 = 180, NPHI = 180 };
+  const int nd = ctl->nd;
+  double theta[NTHETA], sx[3], sy[3], sz[3], wsum = 0;
+  double *rad2 = (double *)malloc(sizeof(double) * NT2 * NPHI * nd);
+  double *tau2 = (double *)malloc(sizeof(double) * NT2 * NPHI * nd);
+  for (int it = 0; it < NTHETA; it++) theta[it] = M_PI * (double)it / (NTHETA - 1);
+  o_bascoord(dx, x, sx, sy, sz);
+#ifdef _OPENMP
+  /* outside a parallel region all threads, inside the rays' region the
+     threads spare for the fans */
+  const int nt = omp_get_level() == 0 ? omp_get_max_threads() : (omp_get_level() == 1 ? o_inner : 1);
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 64) if (nt > 1)
+#endif
+  for (int k = 0; k < NT2 * NPHI; k++) {
+    const int itheta = k / NPHI, iphi = k % NPHI;
+    const double theta2 = (0.5 + itheta) / NT2 * M_PI, phi = (0.5 + iphi) / NPHI * 2 * M_PI;
+    double xv[3];
+    for (int i = 0; i < 3; i++)
+      xv[i] = x[i] + 10 * sin(theta2) * sin(phi) * sx[i] + 10 * sin(theta2) * cos(phi) * sy[i] +
+              10 * cos(theta2) * sz[i];
+    oray_t r2;
+    memset(&r2, 0, sizeof(r2));
+    /* DEVIATION: as for srcfunc_sca_1d, the secondary pencils carry the
+       primary's time (the reference leaves it uninitialised). */
+    r2.time = time;
+    o_cart2geo(x, &r2.obsz, &r2.obslon, &r2.obslat);
+    o_cart2geo(xv, &r2.vpz, &r2.vplon, &r2.vplat);
+    o_pencil(ctl, atm, aero, &r2, scattering - 1, &rad2[(size_t)k * nd], &tau2[(size_t)k * nd]);
+  }
+  for (int id = 0; id < nd; id++) src_sca[id] = 0;
+  for (int itheta = 0; itheta < NT2; itheta++) {
+    const double theta2 = (0.5 + itheta) / NT2 * M_PI;
+    for (int iphi = 0; iphi < NPHI; iphi++) {
+      const int idx = o_locate(theta, NTHETA, theta2);
+      for (int id = 0; id < nd; id++) {
+        double phase2 = o_lin(theta[idx], aero->p[il][id][idx], theta[idx + 1], aero->p[il][id][idx + 1], theta2);
+        double w = M_PI / NT2 * 2 * M_PI * sin(theta2) / NPHI * phase2;
+        src_sca[id] += w * rad2[((size_t)itheta * NPHI + iphi) * nd + id];
+        wsum += w;
+      }
+    }
+  }
+  for (int id = 0; id < nd; id++) src_sca[id] /= wsum;
+  free(rad2);
+  free(tau2);
+}
+
 /* srcfunc_sca (scatter.c:643-662) */
 static void o_srcfunc_sca(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double sec,
                           const double *x, const double *dx, int il, double *src_sca,
                           int scattering) {
-  if (ctl->ip != 1) OFAIL("Only IP=1 scattering (srcfunc_sca_1d) is supported!");
-  o_srcfunc_sca_1d(ctl, atm, aero, sec, x, dx, il, src_sca, scattering);
+  if (ctl->ip == 1)
+    o_srcfunc_sca_1d(ctl, atm, aero, sec, x, dx, il, src_sca, scattering);
+  else
+    o_srcfunc_sca_3d(ctl, atm, aero, sec, x, dx, il, src_sca, scattering);
   if (TSUN > 0) o_srcfunc_sca_sun(ctl, atm, aero, sec, x, dx, il, src_sca);
 }
 
@@ -1847,8 +1904,18 @@ void oracle_formod(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero) {
   oracle_hydrostatic(ctl, atm);
   if (ctl->retnn || ctl->retrr || ctl->retss) oracle_get_opt_prop(ctl, aero);
   o_init_tables(ctl);
-#pragma omp parallel for schedule(dynamic)
+#ifdef _OPENMP
+  /* fewer rays than threads: the spare threads work on the secondary fans */
+  int nth = omp_get_max_threads();
+  int nouter = obs->nr < nth ? (obs->nr > 0 ? obs->nr : 1) : nth;
+  o_inner = nth / nouter;
+  if (o_inner > 1) omp_set_max_active_levels(2);
+#else
+  int nouter = 1;
+#endif
+#pragma omp parallel for schedule(dynamic) num_threads(nouter)
   for (int ir = 0; ir < obs->nr; ir++) oracle_formod_pencil(ctl, atm, obs, aero, ctl->sca_mult, ir);
+  o_inner = 1;
   /* The reference releases its queue after use, leaving capacity 0. */
   if (ctl->queue.capacity > 0) ctl->queue.capacity = 0;
   ctl->queue.state = Queue_Inactive;
@@ -1860,6 +1927,12 @@ void oracle_formod(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero) {
   for (int id = 0; id < ctl->nd; id++)
     for (int ir = 0; ir < obs->nr; ir++)
       if (mask[id][ir]) obs->rad[id][ir] = NAN;
+}
+
+void oracle_srcfunc_sca(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double time, const double *x,
+                        const double *dx, int il, double *src_sca, int scattering) {
+  o_init_tables(ctl);
+  o_srcfunc_sca(ctl, atm, aero, time, x, dx, il, src_sca, scattering);
 }
 
 /* ------------------------------------------------------------------------

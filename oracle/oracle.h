@@ -120,6 +120,12 @@ void oracle_kernel(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, 
 void oracle_kernel_columns(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero, double *k, size_t m,
                            size_t n, size_t ldk, const int *cols, int ncols);
 
+/* The scattering source at Cartesian point x with LOS tangent dx in aerosol
+   layer il (srcfunc_sca, scatter.c:643-662: srcfunc_sca_1d for IP 1,
+   srcfunc_sca_3d otherwise, plus the solar term). */
+void oracle_srcfunc_sca(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double time, const double *x,
+                        const double *dx, int il, double *src_sca, int scattering);
+
 /* Optimal estimation (reference src/retrievalmodel.c:407-706, read_ret
    :710-757) without its files: chisq[] receives every cost-function value
    (at most 1 + 20 conv_itmax), *nchisq their number, err_total[n] the

@@ -53,6 +53,7 @@ def load():
                                          C.POINTER(C.c_int), C.c_int]),
         "oracle_set_fan": (None, [C.c_int]),
         "oracle_read_ret": (None, [C.c_int, P, P, P]),
+        "oracle_srcfunc_sca": (None, [P, P, P, C.c_double, D, D, C.c_int, D, C.c_int]),
         "oracle_cov_apr": (None, [P, P, P, P, D]),
         "oracle_cov_meas": (None, [P, P, P, D, D, D]),
         "oracle_optimal_estimation": (None, [P, P, P, P, P, P, P, P, D, C.POINTER(C.c_int), D]),

@@ -136,3 +136,55 @@ def test_errors(in_examples, keys, msg):
         "lib.formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(J.aero_t()))\n" % (sys.path[:4], keys))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and msg in r.stdout, r.stdout[-500:]
+
+
+def scatter_case(lib, prefix, tmp_path, ip):
+    """One steep ray from 30 km through the aerosol layer of the aerosol
+    workload (two channels, one emitter), single scattering with the direct
+    fans of srcfunc_sca_3d, two profiles 5 degrees apart (the second 6 K
+    warmer); coarse ray steps keep the oracle's 32,400 pencils per point
+    affordable.  For IP 3, CX covers every point a direct pencil reaches
+    (limb pencils run ~1500 km): without a grid point in range the
+    reference's average is 0 / 0 and the NaN ray ends in "Too many LOS
+    points!"."""
+    from pyjurassic import synth
+    tmp_path.mkdir(parents=True, exist_ok=True)
+    w = synth.make("aerosol_limb_8192", tmp_path, nray=2).select(nd=2, ng=1)
+    w.write_ctl(tmp_path / "c.ctl")
+    keys = ["IP", ip, "RAYDZ", 2, "RAYDS", 20] + (["CZ", 3, "CX", 5000] if ip == 3 else [])
+    ctl = J.ctl_t()
+    argc, argv = J.argv(tmp_path / "c.ctl", *keys)
+    getattr(lib, prefix + "read_ctl")(argc, argv, C.byref(ctl))
+    aero = J.aero_t()
+    getattr(lib, prefix + "read_aero")(None, w.aerofile.encode(), C.byref(ctl), C.byref(aero))
+    getattr(lib, prefix + "get_opt_prop")(C.byref(ctl), C.byref(aero))
+    atm = J.atm_t()
+    n = 0
+    for lat, dt in ((0.0, 0.0), (5.0, 6.0)):
+        for l in range(w.nlev):
+            atm.z[n], atm.lon[n], atm.lat[n], atm.p[n], atm.t[n] = w.z[0, l], 0.0, lat, w.p[0, l], w.t[0, l] + dt
+            atm.q[0][n] = w.q[0, 0, l]
+            n += 1
+    atm.np = n
+    obs = J.obs_t()
+    obs.nr = 1
+    obs.obsz[0], obs.obslon[0], obs.obslat[0] = 30.0, 0.0, 2.0
+    obs.vpz[0], obs.vplon[0], obs.vplat[0] = 0.0, 0.0, 2.3
+    getattr(lib, prefix + "formod")(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero))
+    return np.array([obs.rad[i][0] for i in range(2)]), np.array([obs.tau[i][0] for i in range(2)])
+
+
+@pytest.mark.parametrize("ip", [2, 3])
+def test_scattering_3d_vs_oracle(glib, olib, tmp_path, ip):
+    """srcfunc_sca_3d (scatter.c:811-895) on the GPU: every scattering point's
+    32,400 direct pencils traced and integrated as one level of the
+    scattering tree, summed by k_angular3d."""
+    print("case scattering ip %d" % ip, flush=True)
+    g = scatter_case(glib, "", tmp_path / "g", ip)
+    pencils = glib.jur_stat(b"pencils")
+    o = scatter_case(olib, "oracle_", tmp_path / "o", ip)
+    rel = np.max(np.abs(g[0] / o[0] - 1))
+    print("ip %d: %d pencils, rad gpu %s oracle %s, max rel %.2e" % (ip, pencils, g[0], o[0], rel), flush=True)
+    assert pencils > 32400  # the primary, its points' direct fans (and solar pencils)
+    assert rel < 1e-4
+    np.testing.assert_allclose(g[1], o[1], atol=1e-5)

@@ -12,7 +12,8 @@ weight normalisation, a dropped sin(theta) or a misindexed fan pencil breaks
 the equality; with a J of 2 B the radiance would be off by more than 1e-4
 here.  This pins srcfunc_sca_1d at scattering depth 1 and 2 and for the fan
 sizes the oracle supports, which no reference fixture covers beyond depth 1
-with 28 pencils.
+with 28 pencils, and srcfunc_sca_3d (IP 2 / 3) with its normalisation over
+all channels.
 """
 import ctypes as C
 import struct
@@ -22,6 +23,7 @@ import pytest
 
 import oracle_ctypes
 from helpers import read_workload
+from pyjurassic import jtypes as J
 from pyjurassic import synth
 
 T_ISO = 250.0
@@ -94,3 +96,59 @@ def test_isothermal_opaque_radiance_is_planck(olib, tmp_path, workload, zlayer, 
     dev = np.max(np.abs(rad / expect - 1))
     print("max deviation from the closure %.2e" % dev)
     assert dev < 1e-7
+
+
+def _geo_setup(olib, tmp_path, nu1=None):
+    """The aerosol workload reduced to one emitter, opaque tables, isothermal,
+    as two identical profiles 5 degrees apart (IP 2)."""
+    w = synth.make("aerosol_limb_8192", tmp_path, nray=2)
+    w = w.select(nd=2, ng=1)
+    for f in tmp_path.glob("*.bin"):
+        write_opaque_table(f)
+    w.write_ctl(tmp_path / "c.ctl")
+    keys = ["IP", 2] + (["NU[1]", "%.4f" % nu1(w)] if nu1 else [])
+    ctl = J.ctl_t()
+    argc, argv = J.argv(tmp_path / "c.ctl", *keys)
+    olib.oracle_read_ctl(argc, argv, C.byref(ctl))
+    aero = J.aero_t()
+    olib.oracle_read_aero(None, w.aerofile.encode(), C.byref(ctl), C.byref(aero))
+    olib.oracle_get_opt_prop(C.byref(ctl), C.byref(aero))
+    atm = J.atm_t()
+    n = 0
+    for lat in (0.0, 5.0):
+        for l in range(w.nlev):
+            atm.z[n], atm.lon[n], atm.lat[n], atm.p[n], atm.t[n] = w.z[0, l], 0.0, lat, w.p[0, l], T_ISO
+            atm.q[0][n] = w.q[0, 0, l]
+            n += 1
+    atm.np = n
+    return ctl, atm, aero
+
+
+@pytest.mark.parametrize("same_channels", [False, True])
+def test_direct_fan_source(olib, tmp_path, same_channels):
+    """srcfunc_sca_3d (IP 2, scatter.c:811-895) inside the aerosol layer of an
+    isothermal opaque atmosphere: every one of the 32,400 direct pencils sees
+    B(T), so with one channel J = B(T).  The reference normalises by the
+    weights summed over all channels (scatter.c:883, 891): with two channels
+    of the same wavenumber (identical tables and phase functions) each gets
+    exactly J = B(T) / 2."""
+    ctl, atm, aero = _geo_setup(olib, tmp_path, (lambda w: w.nu[0]) if same_channels else None)
+    if not same_channels:
+        ctl.nd = 1
+    x = (C.c_double * 3)(6367.421 + 17.0, 0.0, 0.0)
+    dx = (C.c_double * 3)(0.0, 1.0, 0.0)
+    src, b = np.zeros(ctl.nd), np.zeros(ctl.nd)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    olib.oracle_reset()
+    try:
+        olib.oracle_srcfunc_sca(C.byref(ctl), C.byref(atm), C.byref(aero), 0.0, x, dx, 0, dp(src), 1)
+        olib.oracle_srcfunc_planck(C.byref(ctl), T_ISO, dp(b))
+    finally:
+        olib.oracle_reset()
+    ratio = src / b
+    print("J / B(T):", ratio)
+    if same_channels:
+        assert src[0] == src[1]
+        np.testing.assert_allclose(ratio, 0.5, atol=1e-4)
+    else:
+        assert ctl.nd == 1 and 1 - 1e-4 < ratio[0] <= 1
