@@ -515,6 +515,118 @@ def bench_retrieval(args, lib):
     print(json.dumps(line), flush=True)
 
 
+def scatter3d_inputs(lib, prefix, w, ctlpath):
+    """ctl, aero (with optics), the two-profile atmosphere and 16 steep rays
+    of the direct-fan workload, read with `lib`'s readers."""
+    from pyjurassic import jtypes as J
+    ctl = J.ctl_t()
+    argc, argv = J.argv(ctlpath, "IP", 2)
+    quiet(lambda: getattr(lib, prefix + "read_ctl")(argc, argv, C.byref(ctl)))
+    aero = J.aero_t()
+    quiet(lambda: getattr(lib, prefix + "read_aero")(None, w.aerofile.encode(), C.byref(ctl), C.byref(aero)))
+    quiet(lambda: getattr(lib, prefix + "get_opt_prop")(C.byref(ctl), C.byref(aero)))
+    atm = J.atm_t()
+    n = 0
+    for lat, dt in ((0.0, 0.0), (5.0, 6.0)):
+        for l in range(w.nlev):
+            atm.z[n], atm.lon[n], atm.lat[n], atm.p[n], atm.t[n] = w.z[0, l], 0.0, lat, w.p[0, l], w.t[0, l] + dt
+            for g in range(ctl.ng):
+                atm.q[g][n] = w.q[0, g, l]
+            n += 1
+    atm.np = n
+    obs = J.obs_t()
+    obs.nr = 16
+    for i in range(obs.nr):
+        lat0 = 0.5 + 0.25 * i
+        obs.obsz[i], obs.obslon[i], obs.obslat[i] = 30.0, 0.0, lat0
+        obs.vpz[i], obs.vplon[i], obs.vplat[i] = 0.0, 0.0, lat0 + 0.3
+    return ctl, aero, atm, obs
+
+
+def bench_scatter3d(args, lib):
+    """srcfunc_sca_3d (reference src/scatter.c:811-895), the scattering
+    source for 2-D / 3-D atmospheres: 16 steep rays from 30 km through the
+    15-20 km aerosol layer of the aerosol workload (8 of its channels, its
+    five emitters), single scattering, IP 2 over two profiles 5 degrees
+    apart.  Every scattering point spawns 32,400 direct pencils (a 180 x 180
+    grid of directions), so the work is the pencils: value = pencil x
+    channel radiances per second.  The CPU leg times the oracle's
+    srcfunc_sca_3d at one scattering point of the layer (its 32,400
+    pencils)."""
+    import torch
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from pyjurassic import synth
+    tmp = Path(tempfile.mkdtemp(prefix="jur_s3d_"))
+    w = synth.make("aerosol_limb_8192", tmp, nray=1).select(nd=8)
+    ctlpath = tmp / "s3d.ctl"
+    w.write_ctl(ctlpath)
+    ctl, aero, atm, obs = scatter3d_inputs(lib, "", w, ctlpath)
+    r = C.byref
+    from pyjurassic import jtypes as J
+    o = J.obs_t()
+
+    def step():
+        C.memmove(C.addressof(o), C.addressof(obs), C.sizeof(obs))
+        lib.formod(r(ctl), r(atm), r(o), r(aero))
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    steps = args.steps
+    tot = lambda: np.array([lib.jur_stat(k) for k in (b"path_ms_total", b"integrated_points_total",
+                                                      b"pencils_total")])
+    s0 = tot()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    dt = time.perf_counter() - t0
+    path_ms, ipts, pencils = (tot() - s0) / steps
+    nd, ng = ctl.nd, ctl.ng
+    alg_bytes = ipts * nd * (128 * ng + 8) + pencils * nd * 8
+    achieved = alg_bytes / (path_ms * 1e-3) / 1e9
+    npts = (pencils - obs.nr) / 32400
+    line = {
+        "metric": "direct-fan pencil x channel radiances/s (srcfunc_sca_3d, IP 2)",
+        "value": pencils * nd * steps / dt,
+        "unit": "pencil*channel/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (path integration), f64 (ray geometry, source sums)",
+        "data": "synthetic (seeded generator, reference file formats)",
+        "config": {"workload": "aerosol3d_steep_16", "rays": obs.nr, "channels": nd, "emitters": ng,
+                   "profiles": 2, "levels": w.nlev, "scattering_points": npts, "pencils": pencils,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_path_fast",
+                     "kernel_ms": path_ms, "kernel_share": path_ms / (1e3 * dt / steps),
+                     "alg_bytes_per_call": alg_bytes,
+                     "note": "path kernel time and algorithmic bytes summed over the call's launches"},
+    }
+    if not args.no_cpu:
+        import oracle_ctypes
+        olib = oracle_ctypes.load()
+        octl, oaero, oatm, _ = scatter3d_inputs(olib, "oracle_", w, ctlpath)
+        x = (C.c_double * 3)()
+        dx = (C.c_double * 3)()
+        # a point at 17 km in the layer, LOS tangent slanting down (an
+        # exactly vertical tangent leaves bascoord's basis undefined)
+        xs = np.array([np.cos(np.radians(2.0)), 0.0, np.sin(np.radians(2.0))]) * (6367.421 + 17.0)
+        ds = -xs / np.linalg.norm(xs) + np.array([0.0, 0.0, 0.3])
+        for i in range(3):
+            x[i], dx[i] = xs[i], ds[i]
+        src = np.zeros(nd)
+        c0 = time.perf_counter()
+        quiet(lambda: olib.oracle_srcfunc_sca(r(octl), r(oatm), r(oaero), 0.0, x, dx, 0,
+                                              src.ctypes.data_as(C.POINTER(C.c_double)), 1))
+        cdt = time.perf_counter() - c0
+        line["cpu_baseline"] = {"value": 32400 * nd / cdt, "unit": "pencil*channel/s", "cores": ncores(),
+                                "kind": "port",
+                                "sample": "the FP64 oracle's srcfunc_sca_3d at one scattering point of the layer "
+                                          "(32,400 pencils x %d channels), %.1f s wall" % (nd, cdt)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -531,6 +643,8 @@ def main():
         return bench_jacobian(args, lib)
     if args.workload == "retrieval_limb_17":
         return bench_retrieval(args, lib)
+    if args.workload == "aerosol3d_steep_16":
+        return bench_scatter3d(args, lib)
     w, ctl, aero, scattering, (argc, argv) = setup(lib, args.workload, args.nray)
     aero_p = C.byref(aero) if aero is not None else None
     if args.variant is not None:
