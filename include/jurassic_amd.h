@@ -124,7 +124,7 @@ void optimal_estimation(ret_t *ret, ctl_t *ctl, obs_t *obs_meas, obs_t *obs_i, a
    atm_of_ray [nray] selects a profile per ray; NULL means ray i uses profile
    i when natm == nray, else profile 0.  Outputs: rad, tau [nd][nray]
    (channel-major like obs_t, FP32); tpz/tplon/tplat [nray] and npts [nray]
-   (LOS points) are optional. */
+   (LOS points) are optional.  Vertical profiles only: ctl->ip must be 1. */
 typedef struct {
   int nray, nlev, natm;
   const int *atm_of_ray;
