@@ -74,6 +74,17 @@ def parse():
     return ap.parse_args()
 
 
+_JSON_OUT = None
+
+
+def emit(line):
+    """The one JSON line, on the process's stdout (everything else the
+    library or this script prints goes to stderr, see main)."""
+    out = _JSON_OUT or sys.stdout
+    out.write(json.dumps(line) + "\n")
+    out.flush()
+
+
 def quiet(fn):
     """Run fn with stdout silenced (the control parser echoes every key)."""
     devnull = os.open(os.devnull, os.O_WRONLY)
@@ -83,6 +94,7 @@ def quiet(fn):
         return fn()
     finally:
         os.dup2(saved, 1)
+        os.close(saved)
         os.close(devnull)
 
 
@@ -258,7 +270,7 @@ def bench_surrogate(args, lib):
                       "surrogate_pencils": res[1][2], "max_rel_rad": float(rel.max()),
                       "mean_rel_rad": float(rel.mean()), "training": train},
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 JACOBIAN_TH = np.arange(6.0, 30.01, 1.5)  # 17 limb tangent heights [km]
@@ -385,7 +397,7 @@ def bench_jacobian(args, lib):
             "resolved": "entries whose radiance change exceeds 1e-8 of the radiance; below that the "
                         "finite difference approaches FP64 rounding in the reference too",
             "columns_compared": len(cols)}
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def retrieval_truth(lib, prefix, ctl, atm, obs):
@@ -512,7 +524,7 @@ def bench_retrieval(args, lib):
                             "max_rel_q": float(np.max(np.abs(qg / qo - 1))),
                             "cost_evaluations_gpu": len(costs), "cost_evaluations_oracle": nc.value,
                             "final_chi2_gpu": float(costs[-1][1]), "final_chi2_oracle": float(chisq[nc.value - 1])}
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def scatter3d_inputs(lib, prefix, w, ctlpath):
@@ -624,11 +636,17 @@ def bench_scatter3d(args, lib):
                                 "kind": "port",
                                 "sample": "the FP64 oracle's srcfunc_sca_3d at one scattering point of the layer "
                                           "(32,400 pencils x %d channels), %.1f s wall" % (nd, cdt)}
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def main():
+    global _JSON_OUT
     args = parse()
+    # stdout carries only the JSON line: fd 1 now points at stderr (the
+    # library prints progress messages such as its table initialisation)
+    sys.stdout.flush()
+    _JSON_OUT = os.fdopen(os.dup(1), "w")
+    os.dup2(2, 1)
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has a single MI355X "
               "(run with --gpus 1)", file=sys.stderr)
@@ -742,7 +760,7 @@ def main():
                                           "%.1f s wall" % (ns, nray, nd, cdt)}
         line["accuracy"] = accuracy(rad.cpu().numpy().T[sel], tau.cpu().numpy().T[sel], orad, otau, ctl)
         line["accuracy"]["sample_rays"] = ns
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 if __name__ == "__main__":

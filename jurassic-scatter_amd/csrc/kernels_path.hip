@@ -785,7 +785,7 @@ __device__ inline double planck64(const double* row, double t) {
 }
 
 template <int NGT, bool HINT>
-__global__ void __launch_bounds__(256) k_path64(PathArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_path64(PathArgs A) {
   const int lane = threadIdx.x & 63;
   int chunk, group;
   if (!map_work(A, 1, chunk, group)) return;
