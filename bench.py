@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=None,
                     help="rays of the CPU oracle sample (default: per workload, ~10-30 s of CPU work)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="library option (jur_set_option) before the run; repeatable")
     ap.add_argument("--variant", type=int, default=None,
                     help="path-kernel variant (default: autotuned on the first call)")
     ap.add_argument("--wpb", type=int, default=None, help="waves per block (default: autotuned)")
@@ -655,6 +657,9 @@ def main():
     from pyjurassic import jtypes as J, lib as L
 
     lib = L.load()
+    for kv in args.option:
+        key, val = kv.split("=", 1)
+        lib.jur_set_option(key.encode(), float(val))
     if args.workload == "cirrus_16384":
         return bench_surrogate(args, lib)
     if args.workload == "jacobian_limb_17":

@@ -756,6 +756,7 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   P.out_ld = out_ld;
   P.out_off = out_off;
   P.fp64 = engine_option_int("fp64", 0) != 0;
+  P.fp64_split = engine_option_int("fp64_split", -1);
   P.rad64 = P.fp64 ? rad64 : nullptr;
   P.tau64 = P.fp64 ? tau64 : nullptr;
   // the FP64 records belong to the trace just made of these rays

@@ -60,6 +60,7 @@ struct PathArgs {
   int out_ld, out_off;
   int cpw;                 // kernel variant (channels per wave x occupancy), see launch_path
   int fp64;                // 1: FP64 arithmetic (k_path64); also writes rad64/tau64 if set
+  int fp64_split;          // FP64: emitters on separate lanes (k_path64e): 1 on, 0 off, -1 small calls
   double *rad64, *tau64;   // [nd][out_ld] (optional)
   const double* rec64;     // FP64 records / surface temperatures (optional, fp64)
   const double* tsurf64;

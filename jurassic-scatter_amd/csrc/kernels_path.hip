@@ -689,6 +689,11 @@ struct LaneHints {
   int* base;
   __device__ int& operator[](int k) const { return base[k * 64]; }
 };
+// Hint slots in registers (one emitter per lane: 7 slots).
+struct RegHints {
+  int* h;
+  __device__ int& operator[](int k) const { return h[k]; }
+};
 
 template <bool HINT>
 __device__ inline double corner64(const float2* col, int n, double e_path, double du, int& h) {
@@ -702,9 +707,9 @@ __device__ inline double corner64(const float2* col, int n, double e_path, doubl
 // (forwardmodel.c:433-500).  hint: brackets of the previous point (p, T at
 // both pressures, the four corners' inverse lookups), used when HINT
 // (increasing grids and columns).
-template <bool HINT>
+template <bool HINT, class Hints>
 __device__ inline double ega64(const DevTables& T, int tb, double p, double t, double du, double& tau_path,
-                               LaneHints hint) {
+                               Hints hint) {
   double eps = 0.0;
   const int ntp = T.np[tb];
   if (ntp < 2) {
@@ -784,6 +789,55 @@ __device__ inline double planck64(const double* row, double t) {
   return lin(t0, row[ilo], t1, row[ilo + 1], t);
 }
 
+// Radiance and transmittance update at one LOS point given the gas
+// segment transmittance tau_gas (forwardmodel.c:317-390).
+template <class Rec>
+__device__ __forceinline__ void point64(const PathArgs& A, const ChanConst64& cc, const double* prow, int id,
+                                        int uo, const Rec& rec, int ip, double p, double t, double ds, double af,
+                                        double qh, double tau_gas, int& kj, double& rad, double& tau) {
+  const double uco2 = cc.co2_on ? rec(ip, uo + cc.co2_ig) : 0.0;
+  const double uh2o = cc.h2o_on ? rec(ip, uo + cc.h2o_ig) : 0.0;
+  const double kw = A.nw > 0 ? rec(ip, REC_K + cc.window) : 0.0;
+  const double beta = continua64(cc, p, t, ds, qh, uco2, uh2o, kw);
+  const double B = planck64(prow, t);
+  const bool scpt = A.scattering > 0 && af > 0.0;
+  const int ai = scpt || (A.sca_n > 0 && af != 0.0) ? (int)rec(ip, REC_AI) : 0;
+  if (scpt) {
+    // scattering source (forwardmodel.c:317-360); a zero-length segment
+    // adds nothing (the reference forms -log(1)/0 there)
+    if (tau_gas > 0.0 && ds > 0.0) {
+      const double be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
+      const double J = A.J[(size_t)kj * A.nd + id];
+      const double bt = -log(tau_gas) / ds + beta + af * be;
+      const double eps = 1.0 - exp(-bt * ds);
+      const double src = ((bt - af * bs) * B + af * bs * J) / bt;
+      rad += tau * eps * src;
+      tau *= exp(-bt * ds);
+    }
+    kj++;
+  } else if (tau_gas > 0.0) {
+    // gas, continua and aerosol extinction (forwardmodel.c:363-390)
+    double bx = beta;
+    if (A.sca_n > 0 && af != 0.0) bx += af * (A.ext_beta_e ? A.beta_e[ai * A.nd + id] : A.beta_a[ai * A.nd + id]);
+    const double eps = 1.0 - tau_gas * exp(-bx * ds);
+    rad += B * eps * tau;
+    tau *= 1.0 - eps;
+  }
+}
+// Surface term and outputs of one ray x channel (forwardmodel.c:393-399).
+__device__ __forceinline__ void finish64(const PathArgs& A, const double* prow, int id, int ray, double rad,
+                                         double tau) {
+  const double ts = A.tsurf64 ? A.tsurf64[ray] : (double)A.tsurf[ray];
+  if (ts > 0.0) rad += planck64(prow, ts) * tau;
+  const size_t o = (size_t)id * A.out_ld + A.out_off + ray;
+  if (A.rad64) {
+    A.rad64[o] = rad;
+    A.tau64[o] = tau;
+  }
+  A.rad[o] = (float)rad;
+  A.tau[o] = (float)tau;
+}
+
 template <int NGT, bool HINT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_path64(PathArgs A) {
   const int lane = threadIdx.x & 63;
@@ -825,46 +879,66 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
       if (g < A.ng)
         tau_gas *= 1.0 - ega64<HINT>(A.T, id * A.ng + g, p, t, rec(ip, uo + g), tau_path[g],
                                      LaneHints{hbase + g * 7 * 64});
-    const double uco2 = cc.co2_on ? rec(ip, uo + cc.co2_ig) : 0.0;
-    const double uh2o = cc.h2o_on ? rec(ip, uo + cc.h2o_ig) : 0.0;
-    const double kw = A.nw > 0 ? rec(ip, REC_K + cc.window) : 0.0;
-    const double beta = continua64(cc, p, t, ds, qh, uco2, uh2o, kw);
-    const double B = planck64(prow, t);
-    const bool scpt = A.scattering > 0 && af > 0.0;
-    const int ai = scpt || (A.sca_n > 0 && af != 0.0) ? (int)rec(ip, REC_AI) : 0;
-    if (scpt) {
-      // scattering source (forwardmodel.c:317-360); a zero-length segment
-      // adds nothing (the reference forms -log(1)/0 there)
-      if (tau_gas > 0.0 && ds > 0.0) {
-        const double be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
-        const double J = A.J[(size_t)kj * A.nd + id];
-        const double bt = -log(tau_gas) / ds + beta + af * be;
-        const double eps = 1.0 - exp(-bt * ds);
-        const double src = ((bt - af * bs) * B + af * bs * J) / bt;
-        rad += tau * eps * src;
-        tau *= exp(-bt * ds);
-      }
-      kj++;
-    } else if (tau_gas > 0.0) {
-      // gas, continua and aerosol extinction (forwardmodel.c:363-390)
-      double bx = beta;
-      if (A.sca_n > 0 && af != 0.0) bx += af * (A.ext_beta_e ? A.beta_e[ai * A.nd + id] : A.beta_a[ai * A.nd + id]);
-      const double eps = 1.0 - tau_gas * exp(-bx * ds);
-      rad += B * eps * tau;
-      tau *= 1.0 - eps;
-    }
+    point64(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau);
   }
   if (!valid) return;
-  // surface (forwardmodel.c:393-399)
-  const double ts = A.tsurf64 ? A.tsurf64[ray] : (double)A.tsurf[ray];
-  if (ts > 0.0) rad += planck64(prow, ts) * tau;
-  const size_t o = (size_t)id * A.out_ld + A.out_off + ray;
-  if (A.rad64) {
-    A.rad64[o] = rad;
-    A.tau64[o] = tau;
+  finish64(A, prow, id, ray, rad, tau);
+}
+
+// The same integration with a ray's emitters on separate lanes: 8 lanes per
+// ray (lane g < ng runs emitter g's EGA chain, its hints in registers),
+// 8 rays per wave.  The segment transmittance is the product of the lanes'
+// 1 - eps in emitter order (shuffles), so the arithmetic and its order are
+// k_path64's; the dependent chain per LOS point is one emitter long instead
+// of ng.  For calls too small to fill the GPU with one lane per ray (the
+// few-ray forward models of a retrieval).  Lanes of a ray stay converged:
+// they share its point count.
+template <bool HINT>
+__global__ void __launch_bounds__(256) k_path64e(PathArgs A) {
+  const int lane = threadIdx.x & 63, sub = lane & 7;
+  const int wv = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  const int ngroups = (A.nray + 7) >> 3;
+  const int bpc = (ngroups + wpb - 1) / wpb;
+  const int id = blockIdx.x / bpc;
+  const int group = (blockIdx.x - id * bpc) * wpb + wv;
+  if (id >= A.nd || group >= ngroups) return;
+  const int slot = group * 8 + (lane >> 3);
+  const bool valid = slot < A.nray;
+  const int ray = valid ? A.order[slot] : 0;
+  const int npts = valid ? A.np[ray] : 0;
+  const bool gl = sub < A.ng;
+  const int uo = rec_u(A.nw);
+  const ChanConst64& cc = A.cc64[id];
+  const double* prow = A.planck64 + (size_t)id * JUR_NPLANCK;
+  int hv[7] = {0, 0, 0, 0, 0, 0, 0};
+  double tau_path = 1.0, rad = 0.0, tau = 1.0;
+  int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
+  const size_t rbase = (size_t)ray * A.cap * A.R;
+  auto rec = [&](int ip, int f) -> double {
+    const size_t i = rbase + (size_t)ip * A.R + f;
+    return A.rec64 ? A.rec64[i] : (double)A.rec[i];
+  };
+  const int gbase = lane & ~7;
+  for (int ip = 0; ip < npts; ip++) {
+    const double p = rec(ip, REC_P), t = rec(ip, REC_T), ds = rec(ip, REC_DS), af = rec(ip, REC_AF);
+    const double qh = rec(ip, REC_QH2O);
+    double f = 1.0;
+    if (gl) f = 1.0 - ega64<HINT>(A.T, id * A.ng + sub, p, t, rec(ip, uo + sub), tau_path, RegHints{hv});
+    double tau_gas = 1.0;
+    for (int g = 0; g < A.ng; g++) tau_gas *= __shfl(f, gbase + g, 64);
+    point64(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau);
   }
-  A.rad[o] = (float)rad;
-  A.tau[o] = (float)tau;
+  if (!valid || sub != 0) return;
+  finish64(A, prow, id, ray, rad, tau);
+}
+
+void launch_64e(const PathArgs& A, int wpb, hipStream_t s) {
+  const int ngroups = (A.nray + 7) / 8;
+  const long nb = (long)A.nd * ((ngroups + wpb - 1) / wpb);
+  if (A.T.shared)
+    hipLaunchKernelGGL(k_path64e<true>, dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  else
+    hipLaunchKernelGGL(k_path64e<false>, dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
 }
 
 template <int NGT>
@@ -911,6 +985,13 @@ int path_variants() { return 7; }
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;
   if (A.fp64) {
+    // emitters on separate lanes when one lane per ray leaves the GPU
+    // mostly idle (fewer than 2048 waves: 8 per CU)
+    const long waves = (long)A.nd * ((A.nray + 63) / 64);
+    if (A.ng <= 8 && (A.fp64_split > 0 || (A.fp64_split < 0 && waves < 2048))) {
+      launch_64e(A, 4, s);
+      return;
+    }
     if (A.ng <= 2) launch_64<2>(A, 4, s);
     else if (A.ng <= 5) launch_64<5>(A, 4, s);
     else launch_64<NGMAX>(A, 4, s);

@@ -170,3 +170,24 @@ def test_size_mismatch_exits(in_examples):
         " k.ctypes.data_as(C.POINTER(C.c_double)), 11, 3, 3)\n" % (sys.path[:4],))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "Matrix size" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["clear", "ext1", "aero0"])
+def test_fp64_emitter_lanes_bit_identical(glib, in_examples, case):
+    """k_path64e (a ray's emitters on separate lanes, chosen for small FP64
+    calls) performs k_path64's arithmetic in its order: FP64 radiances and
+    transmittances bit-identical, clear air, extinction and scattering."""
+    out = []
+    glib.jur_set_option(b"fp64", 1)
+    try:
+        for split in (0, 1):
+            glib.jur_set_option(b"fp64_split", split)
+            ctl, obs, atm, aero = read_example(glib, "", case)
+            glib.formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero))
+            out.append(np.array([[obs.rad[id][ir], obs.tau[id][ir]] for id in range(ctl.nd)
+                                 for ir in range(obs.nr)]))
+    finally:
+        glib.jur_set_option(b"fp64_split", -1)
+        glib.jur_set_option(b"fp64", 0)
+    print("%s: %d values, max |diff| %.3g" % (case, out[0].size, np.max(np.abs(out[0] - out[1]))), flush=True)
+    np.testing.assert_array_equal(out[0], out[1])
