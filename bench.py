@@ -47,7 +47,7 @@ MEASURED_TRAFFIC = {
     ("clear_batch_65536", 65536, 6, 4): (1.9911e+11, "profiles/k_path_fast_clear_batch_65536.md"),
 }
 # The same for the FP64 path kernel of kernel() on the Jacobian workload.
-MEASURED_TRAFFIC_JACOBIAN = (6.8150e+10, "profiles/k_path64_jacobian_limb_17.md")
+MEASURED_TRAFFIC_JACOBIAN = (1.0259e+11, "profiles/k_path64_jacobian_limb_17.md")
 
 
 def parse():
