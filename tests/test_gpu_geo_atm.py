@@ -188,3 +188,50 @@ def test_scattering_3d_vs_oracle(glib, olib, tmp_path, ip):
     assert pencils > 32400  # the primary, its points' direct fans (and solar pencils)
     assert rel < 1e-4
     np.testing.assert_allclose(g[1], o[1], atol=1e-5)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_raytrace_los_vs_oracle(glib, olib, in_examples, name):
+    """raytrace() on the transect: the LOS points, their atmosphere (IP 2 / 3
+    interpolation at each point's longitude and latitude) and column
+    densities against the oracle."""
+    ctl, obs, atm = transect(olib, "oracle_", CASES[name])
+    aero = J.aero_t()
+    lo, lg = J.los_t(), J.los_t()
+    olib.oracle_raytrace(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero), C.byref(lo), 4)
+    glib.raytrace(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero), C.byref(lg), 4)
+    n = lo.np
+    assert lg.np == n and n > 50
+    for f in ["z", "lon", "lat", "p", "t", "ds"]:
+        a = np.frombuffer(getattr(lo, f), dtype=np.float64)[:n]
+        b = np.frombuffer(getattr(lg, f), dtype=np.float64)[:n]
+        np.testing.assert_allclose(b, a, rtol=1e-11, atol=1e-10, err_msg=f)
+    uo = np.array([[lo.u[i][g] for g in range(ctl.ng)] for i in range(n)])
+    ug = np.array([[lg.u[i][g] for g in range(ctl.ng)] for i in range(n)])
+    np.testing.assert_allclose(ug, uo, rtol=1e-11)
+    lat = np.frombuffer(lo.lat, dtype=np.float64)[:n]
+    assert lat.max() - lat.min() > 3  # the LOS crosses profiles
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_hydrostatic_profiles_vs_oracle(glib, olib, in_examples, name):
+    """HYDZ rebuilds the pressure of each profile (runs of equal lon, lat)
+    from its own reference level; FP64 formod on the rebalanced transect."""
+    keys = CASES[name] + ("HYDZ", 12)
+    glib.jur_set_option(b"fp64", 1)
+    try:
+        ctl, obs, atm = transect(glib, "", keys)
+        glib.formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(J.aero_t()))
+        g = np.array([obs.rad[0][i] for i in range(obs.nr)]), np.array(atm.p[:atm.np])
+    finally:
+        glib.jur_set_option(b"fp64", 0)
+    ctl, obs, atm = transect(olib, "oracle_", keys)
+    olib.oracle_formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(J.aero_t()))
+    o = np.array([obs.rad[0][i] for i in range(obs.nr)]), np.array(atm.p[:atm.np])
+    np.testing.assert_allclose(g[1], o[1], rtol=1e-12)
+    rel = np.max(np.abs(g[0] / o[0] - 1))
+    print("%s HYDZ fp64: max rel rad %.2e" % (name, rel), flush=True)
+    assert rel <= 1e-10
+    # the four profiles were rebalanced separately: their pressures differ
+    nl = atm.np // len(LATS)
+    assert not np.allclose(o[1][:nl], o[1][3 * nl:], rtol=1e-6)
