@@ -110,6 +110,69 @@ extern "C" void bhmie(double x, double n_real, double n_imag, double* phase, dou
   HIP_OK(hipStreamSynchronize(s));
 }
 
+// opt_prop_external (scatter.c:512-579): the mode's optical properties from
+// a table of rows "nu beta_e beta_s P(0 deg) ... P(180 deg)", for each
+// channel the row nearest its wavenumber (bracket, then the closer of its two
+// rows; a tie keeps the lower).  NN = 0 selects this, NN = 1 stops with the
+// reference's "Implement me!", other values leave the mode at zero.  Lines
+// follow the reference's TOK rules (an unparsable wavenumber or coefficient
+// skips the line, a missing token is an error); an unparsable phase value is
+// an error here (the reference keeps whatever its static table held there).
+// A mode naming the same file as the previous mode reuses the last table; a
+// one-row table serves every channel.
+static void ext_mode(const ctl_t* ctl, const aero_t* aero, int m, double* be, double* bs, double* ph) {
+  static std::vector<double> nu, ne, ns, np_;
+  const int nd = ctl->nd;
+  for (int id = 0; id < nd; id++) {
+    be[id] = bs[id] = 0;
+    for (int ia = 0; ia < NTHETA; ia++) ph[(size_t)id * NTHETA + ia] = 0;
+  }
+  if (aero->nn[m] == 0) {
+    if (m == 0 || strcmp(aero->filepath[m], aero->filepath[m - 1]) != 0) {
+      printf("Read non-spherical optical properties: %s\n", aero->filepath[m]);
+      FILE* in = fopen(aero->filepath[m], "r");
+      if (!in) JUR_FAIL("Cannot open file!");
+      nu.clear();
+      ne.clear();
+      ns.clear();
+      np_.clear();
+      static char line[LEN];
+      double row[3 + NTHETA];
+      while (fgets(line, LEN, in)) {
+        bool skip = false;
+        char* tok = strtok(line, " \t");
+        for (int k = 0; k < 3 + NTHETA && !skip; k++) {
+          if (k > 0) tok = strtok(nullptr, " \t");
+          if (!tok) JUR_FAIL("Error while reading!");
+          if (sscanf(tok, "%lg", &row[k]) != 1) {
+            if (k >= 3) JUR_FAIL("Error while reading!");
+            skip = true;
+          }
+        }
+        if (skip) continue;
+        nu.push_back(row[0]);
+        ne.push_back(row[1]);
+        ns.push_back(row[2]);
+        np_.insert(np_.end(), row + 3, row + 3 + NTHETA);
+        if (nu.size() > (size_t)REFMAX) JUR_FAIL("Too many data points!");
+      }
+      fclose(in);
+      if (nu.empty()) JUR_FAIL("Could not read any data!");
+    }
+    const int n = (int)nu.size();
+    if (n < 1) JUR_FAIL("Could not read any data!");
+    for (int id = 0; id < nd; id++) {
+      const double x = ctl->nu[id];
+      int im = n >= 2 ? locate_mono<double>(nu.data(), n, x) : 0;
+      if (n >= 2 && im != n && fabs(nu[im] - x) > fabs(nu[im + 1] - x)) im = im + 1;
+      be[id] = ne[im];
+      bs[id] = ns[im];
+      for (int ia = 0; ia < NTHETA; ia++) ph[(size_t)id * NTHETA + ia] = np_[(size_t)im * NTHETA + ia];
+    }
+  }
+  if (aero->nn[m] == 1) JUR_FAIL("Implement me!");
+}
+
 extern "C" void get_opt_prop(ctl_t* ctl, aero_t* aero) {
   engine_get_opt_prop(ctl, aero);
 }
@@ -154,11 +217,21 @@ void engine_get_opt_prop(ctl_t* ctl, aero_t* aero) {
   std::vector<int> item_of((size_t)nmode * NRAD * nd, -1);
   std::vector<MieNode> nodes((size_t)nmode * NRAD);
   std::vector<double> nr(nd), ni(nd);
+  // tabulated modes ("Ext"): their properties, host side, into the mode rows below
+  std::vector<char> is_ext(nmode, 0);
+  std::vector<double> ext_be((size_t)nmode * nd), ext_bs((size_t)nmode * nd),
+      ext_p((size_t)nmode * nd * NTHETA);
   for (int m = 0; m < nmode; m++) {
     if (strcasecmp(aero->type[m], "MIE") != 0) {
-      if (strcasecmp(aero->type[m], "Const") == 0) JUR_FAIL("Implement me!");
-      if (strcasecmp(aero->type[m], "Ext") == 0)
-        JUR_FAIL("External optical properties (Ext) are not supported by this build!");
+      if (strcasecmp(aero->type[m], "Ext") == 0) {
+        is_ext[m] = 1;
+        ext_mode(ctl, aero, m, &ext_be[(size_t)m * nd], &ext_bs[(size_t)m * nd], &ext_p[(size_t)m * nd * NTHETA]);
+        continue;
+      }
+      if (strcasecmp(aero->type[m], "Const") == 0) {
+        printf("Using constant extinction [1/km]: %g\n", aero->nn[m]);
+        JUR_FAIL("Implement me!");
+      }
       JUR_FAIL("Please give valid scattering model (MIE, Ext, Const)!");
     }
     // Refractive indices, linear in wavenumber (scatter.c:444-462).
@@ -207,10 +280,20 @@ void engine_get_opt_prop(ctl_t* ctl, aero_t* aero) {
   double* mbe = (double*)b.mbe.need((size_t)nmode * nd * sizeof(double));
   double* mbs = (double*)b.mbs.need((size_t)nmode * nd * sizeof(double));
   double* mp = (double*)b.mp.need((size_t)nmode * nd * NTHETA * sizeof(double));
-  for (int m = 0; m < nmode; m++)
+  for (int m = 0; m < nmode; m++) {
+    if (is_ext[m]) {
+      HIP_OK(hipMemcpyAsync(mbe + (size_t)m * nd, &ext_be[(size_t)m * nd], nd * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(mbs + (size_t)m * nd, &ext_bs[(size_t)m * nd], nd * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(mp + (size_t)m * nd * NTHETA, &ext_p[(size_t)m * nd * NTHETA],
+                            (size_t)nd * NTHETA * sizeof(double), hipMemcpyHostToDevice, s));
+      continue;
+    }
     launch_mie_mode(dn + (size_t)m * NRAD, NRAD, dio + (size_t)m * NRAD * nd, nd, b.qext.as<double>(),
                     b.qsca.as<double>(), b.phase.as<double>(), mbe + (size_t)m * nd, mbs + (size_t)m * nd,
                     mp + (size_t)m * nd * NTHETA, s);
+  }
   std::vector<int> mode0(nl), nmod(nl);
   for (int il = 0, c = 0; il < nl; il++) {
     mode0[il] = c;

@@ -1450,8 +1450,59 @@ static void o_mie_log(ctl_t *ctl, aero_t *aero, int count, double *beta_ext, dou
     for (int jj = 0; jj < NTHETA; ++jj) phase[id][jj] /= beta_sca[id];
 }
 
-/* get_opt_prop (scatter.c:330-418).  Only the "MIE" model type is
-   restated; "Ext" and "Const" stop with an error. */
+/* opt_prop_external (scatter.c:512-579): the mode's optical properties
+   from a table of rows "nu beta_e beta_s P(0 deg) ... P(180 deg)", for each
+   channel the row nearest its wavenumber (bracket by locate, then the
+   closer of the bracket's two rows; a tie keeps the lower).  NN = 0 selects
+   this; NN = 1 (interpolation) stops with the reference's "Implement me!";
+   other values leave the mode's properties at zero.  Lines are read with
+   the reference's TOK rules (a token that does not parse skips the line,
+   or that phase value; a missing token is an error).  DEVIATION: a mode
+   naming the same file as the previous mode reuses its table (the
+   reference skips the read but then searches zero rows, reading before its
+   arrays). */
+static void o_opt_prop_external(ctl_t *ctl, aero_t *aero, int count, double *beta_ext, double *beta_sca,
+                                double phase[NDMAX][NTHETA]) {
+  static double nu[REFMAX], n_ext[REFMAX], n_sca[REFMAX], n_phase[REFMAX][NTHETA];
+  static int npts = 0;
+  if (aero->nn[count] == 0) {
+    if (count == 0 || strcmp(aero->filepath[count], aero->filepath[count - 1]) != 0) {
+      printf("Read non-spherical optical properties: %s\n", aero->filepath[count]);
+      FILE *in = fopen(aero->filepath[count], "r");
+      if (!in) OFAIL("Cannot open file!");
+      static char line[LEN];
+      npts = 0;
+      while (fgets(line, LEN, in)) {
+        char *tok;
+        if (!(tok = strtok(line, " \t"))) OFAIL("Error while reading!");
+        if (sscanf(tok, "%lg", &nu[npts]) != 1) continue;
+        if (!(tok = strtok(NULL, " \t"))) OFAIL("Error while reading!");
+        if (sscanf(tok, "%lg", &n_ext[npts]) != 1) continue;
+        if (!(tok = strtok(NULL, " \t"))) OFAIL("Error while reading!");
+        if (sscanf(tok, "%lg", &n_sca[npts]) != 1) continue;
+        for (int ia = 0; ia < NTHETA; ia++) {
+          if (!(tok = strtok(NULL, " \t"))) OFAIL("Error while reading!");
+          if (sscanf(tok, "%lg", &n_phase[npts][ia]) != 1) continue;
+        }
+        if ((++npts) > REFMAX) OFAIL("Too many data points!");
+      }
+      fclose(in);
+      if (npts < 1) OFAIL("Could not read any data!");
+    }
+    for (int id = 0; id < ctl->nd; id++) {
+      int im = o_locate(nu, npts, ctl->nu[id]);
+      if (im != npts && fabs(nu[im] - ctl->nu[id]) > fabs(nu[im + 1] - ctl->nu[id])) im = im + 1;
+      beta_ext[id] = n_ext[im];
+      beta_sca[id] = n_sca[im];
+      for (int ia = 0; ia < NTHETA; ++ia) phase[id][ia] = n_phase[im][ia];
+    }
+  }
+  if (aero->nn[count] == 1) OFAIL("Implement me!");
+}
+
+/* get_opt_prop (scatter.c:330-418): "MIE" (log-normal Mie modes) and "Ext"
+   (tabulated properties) model types; "Const" stops with the reference's
+   "Implement me!". */
 void oracle_get_opt_prop(ctl_t *ctl, aero_t *aero) {
   int nl = 1, nm = 1, count;
   static double mbe[NDMAX], mbs[NDMAX], mp[NDMAX][NTHETA];
@@ -1486,8 +1537,13 @@ void oracle_get_opt_prop(ctl_t *ctl, aero_t *aero) {
     for (int jj = 0; jj < aero->nmod[ii]; jj++) {
       if (strcasecmp(aero->type[count], "MIE") == 0)
         o_mie_log(ctl, aero, count, mbe, mbs, mp);
-      else
-        OFAIL("Only MIE optical properties are supported!");
+      else if (strcasecmp(aero->type[count], "Ext") == 0)
+        o_opt_prop_external(ctl, aero, count, mbe, mbs, mp);
+      else if (strcasecmp(aero->type[count], "Const") == 0) {
+        printf("Using constant extinction [1/km]: %g\n", aero->nn[count]);
+        OFAIL("Implement me!");
+      } else
+        OFAIL("Please give valid scattering model (MIE, Ext, Const)!");
       for (int id = 0; id < ctl->nd; id++) {
         aero->beta_e[ii][id] += mbe[id];
         aero->beta_s[ii][id] += mbs[id];
