@@ -395,7 +395,11 @@ bool read_table_bin(const char* filename, HostTable& tb) {
 }
 
 // ASCII table "p T u eps" rows; a row is kept only if u and eps both
-// increase within its (p, T) column (forwardmodel.c:658-737).
+// increase within its (p, T) column (forwardmodel.c:658-737): a
+// non-increasing row overwrites the column's last node, a row beyond
+// TBLNUMAX nodes is dropped.  A new pressure level always opens a new
+// temperature column (the reference indexes column -1 when its first
+// temperature equals the previous row's).
 bool read_table_ascii(const char* filename, HostTable& tb) {
   FILE* in = fopen(filename, "r");
   if (!in) return false;
@@ -408,6 +412,7 @@ bool read_table_ascii(const char* filename, HostTable& tb) {
     if (sscanf(line, "%lg %lg %lg %lg", &press, &temp, &u, &eps) != 4) continue;
     if (press != press_old) {
       press_old = press;
+      temp_old = NAN;
       tb.p.push_back(press);
       tb.t.emplace_back();
       tb.u.emplace_back();

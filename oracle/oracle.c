@@ -462,6 +462,10 @@ static int o_read_tbl_ascii(const char *filename, otab_t *tb) {
       press_old = press;
       if ((++tb->np) >= TBLNPMAX) OFAIL("Too many pressure levels!");
       tb->nt[tb->np] = -1;
+      /* DEVIATION: a new pressure level always opens a new temperature
+         column (the reference indexes column -1 when its first temperature
+         equals the previous row's). */
+      temp_old = NAN;
     }
     int ip = tb->np;
     if (temp != temp_old) {

@@ -134,3 +134,72 @@ def test_bhmie_vs_oracle(glib, olib):
         assert qb.value == pytest.approx(qa.value, rel=1e-11)
         assert sb.value == pytest.approx(sa.value, rel=1e-11)
         np.testing.assert_allclose(np.array(pb), np.array(pa), rtol=1e-10)
+
+
+def _bin_table(path):
+    """The reference's binary table layout (forwardmodel.c:636-651)."""
+    b = path.read_bytes()
+    o = 0
+
+    def take(fmt, n):
+        nonlocal o
+        a = np.frombuffer(b, dtype=fmt, count=n, offset=o)
+        o += a.nbytes
+        return a
+    np_ = int(take("<i4", 1)[0])
+    p = take("<f8", np_)
+    nt = take("<i4", np_)
+    cols = []
+    for ip in range(np_):
+        t = take("<f8", nt[ip])
+        nu = take("<i4", nt[ip])
+        for it in range(nt[ip]):
+            u = take("<f4", nu[it])
+            e = take("<f4", nu[it])
+            cols.append((p[ip], t[it], u, e))
+    return cols
+
+
+def test_ascii_tables_vs_oracle(glib, olib, in_examples, tmp_path):
+    """The ASCII table reader (forwardmodel.c:658-737): the clear example's
+    binary tables written as text, each (p, T) column with one injected
+    non-increasing row (which overwrites the column's last node, as in the
+    reference).  Both libraries read the text; formod agrees with the oracle,
+    and without the injected rows the text tables give the binary tables'
+    radiances exactly."""
+    import shutil
+    out = {}
+    for inject in (False, True):
+        d = tmp_path / ("inj" if inject else "plain")
+        d.mkdir()
+        for g in ("CO2", "H2O", "O3"):
+            lines = []
+            for p, t, u, e in _bin_table(EXAMPLES / "tab" / ("boxcar_792.0000_%s.bin" % g)):
+                for k in range(len(u)):
+                    lines.append("%.9g %.9g %.9g %.9g" % (p, t, u[k], e[k]))
+                    if inject and k == len(u) // 2:
+                        lines.append("%.9g %.9g %.9g %.9g" % (p, t, u[k] * 0.5, e[k] * 0.5))
+            (d / ("boxcar_792.0000_%s.tab" % g)).write_text("\n".join(lines) + "\n")
+        shutil.copy(EXAMPLES / "tab" / "boxcar_792.0000.filt", d / "boxcar_792.0000.filt")
+        keys = ("TBLBASE", str(d / "boxcar"))
+        for lib, prefix in ((olib, "oracle_"), (glib, "")):
+            ctl, obs, atm = J.ctl_t(), J.obs_t(), J.atm_t()
+            argc, argv = J.argv("clear-air.ctl", *keys)
+            getattr(lib, prefix + "read_ctl")(argc, argv, C.byref(ctl))
+            getattr(lib, prefix + "read_obs")(None, b"org/obs.tab", C.byref(ctl), C.byref(obs))
+            getattr(lib, prefix + "read_atm")(None, b"atm_pwin_0_80.tab", C.byref(ctl), C.byref(atm))
+            getattr(lib, prefix + "formod")(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(J.aero_t()))
+            out[(inject, prefix)] = np.array([obs.rad[0][i] for i in range(obs.nr)])
+    ctl, obs, atm = J.ctl_t(), J.obs_t(), J.atm_t()
+    argc, argv = J.argv("clear-air.ctl")
+    glib.read_ctl(argc, argv, C.byref(ctl))
+    glib.read_obs(None, b"org/obs.tab", C.byref(ctl), C.byref(obs))
+    glib.read_atm(None, b"atm_pwin_0_80.tab", C.byref(ctl), C.byref(atm))
+    glib.formod(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(J.aero_t()))
+    binary = np.array([obs.rad[0][i] for i in range(obs.nr)])
+    np.testing.assert_array_equal(out[(False, "")], binary)
+    for inject in (False, True):
+        rel = np.max(np.abs(out[(inject, "")] / out[(inject, "oracle_")] - 1))
+        print("ascii tables (injected rows %s): max rel rad vs oracle %.2e" % (inject, rel), flush=True)
+        assert rel < 1e-4
+    assert np.max(np.abs(out[(True, "oracle_")] / out[(False, "oracle_")] - 1)) > 1e-6  # the overwrite matters
