@@ -172,7 +172,10 @@ int jur_init(int device);
    FP32 with a compensated radiance sum), "kernel_fp64" (default 1: kernel
    and jur_kernel integrate the perturbed paths in FP64, since a 1 %
    perturbation of one level can change a radiance by less than FP32
-   resolves; optimal_estimation's forward calls follow the same setting).
+   resolves; optimal_estimation's forward calls follow the same setting),
+   "fp64_split" (FP64 path with a ray's emitters on separate lanes: -1, the
+   default, for calls too small to fill the GPU with one lane per ray; 1
+   always; 0 never; results are identical).
    Returns 0. */
 int jur_set_option(const char *key, double value);
 /* Statistics of the last call: "path_ms" (path-integration kernel time,
