@@ -113,7 +113,8 @@ void read_ret(int argc, char *argv[], ctl_t *ctl, ret_t *ret);
    src/retrievalmodel.c:407-706): Levenberg-Marquardt retrieval of atm_i /
    aero_i from obs_meas with a priori atm_apr / aero_apr; writes costs.tab,
    obs_final.tab, atm_final.tab, aero_final.tab and (ERR_ANA) the error
-   analysis files into ret->dir, the matrices with WRITE_MATRIX. */
+   analysis files into ret->dir, the matrices with WRITE_MATRIX.  aero_apr
+   and aero_i may be NULL for clear air. */
 void optimal_estimation(ret_t *ret, ctl_t *ctl, obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr, atm_t *atm_i,
                         aero_t *aero_apr, aero_t *aero_i);
 

@@ -512,6 +512,10 @@ extern "C" void read_ret(int argc, char* argv[], ctl_t* ctl, ret_t* ret) {
 extern "C" void optimal_estimation(ret_t* ret, ctl_t* ctl, obs_t* obs_meas, obs_t* obs_i, atm_t* atm_apr,
                                    atm_t* atm_i, aero_t* aero_apr, aero_t* aero_i) {
   Fp64Scope fp64;
+  // clear air: the aerosol arguments may be NULL
+  static aero_t none_apr, none_i;
+  if (!aero_apr) aero_apr = &none_apr;
+  if (!aero_i) aero_i = &none_i;
   StateVec sv;
   state_vector(ctl, atm_apr, aero_apr, sv);
   const size_t m = meas_count(ctl, obs_meas), n = sv.n;
