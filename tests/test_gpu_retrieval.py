@@ -36,7 +36,11 @@ CASES = {
     # temperature and H2O (correlated a priori), matrices written
     "t_h2o_matrices": (("RETQ_ZMIN[1]", 5, "RETQ_ZMAX[1]", 20, "ERR_Q[1]", 30, "ERR_Q_CZ[1]", 2,
                         "ERR_Q_CH[1]", 300, "WRITE_MATRIX", 1), 3.0),
+    # extinction example: temperature and the aerosol layer's particle
+    # concentration (Mie optics refreshed for every state), truth 1.5 N
+    "t_aerosol_nn": (("RETNN", 1, "ERR_NN", 1.0, "CONV_ITMAX", 3), 3.0),
 }
+CASE_OF = {"t_aerosol_nn": "ext1"}
 
 
 @pytest.fixture(scope="module")
@@ -59,19 +63,22 @@ def _copy(s):
     return d
 
 
-def setup(lib, prefix, keys):
-    ctl, obs, atm, aero = read_example(lib, prefix, "clear", keys)
+def setup(lib, prefix, keys, case="clear"):
+    ctl, obs, atm, aero = read_example(lib, prefix, case, keys)
     ret = J.ret_t()
     argc, argv = J.argv("clear-air.ctl", *keys)
     getattr(lib, prefix + "read_ret")(argc, argv, C.byref(ctl), C.byref(ret))
     return ctl, ret, obs, atm, aero
 
 
-def measurements(olib, keys, bump):
-    ctl, _, obs, atm, aero = setup(olib, "oracle_", keys)
+def measurements(olib, keys, bump, case="clear"):
+    ctl, _, obs, atm, aero = setup(olib, "oracle_", keys, case)
     tru = _copy(atm)
     for ip in range(atm.np):
         tru.t[ip] += bump * np.exp(-((atm.z[ip] - 18) / 5) ** 2)
+    if case != "clear":
+        aero.nn[0] *= 1.5
+        olib.oracle_get_opt_prop(C.byref(ctl), C.byref(aero))
     olib.oracle_formod(C.byref(ctl), C.byref(tru), C.byref(obs), C.byref(aero))
     return obs
 
@@ -84,11 +91,12 @@ def rows(path):
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_retrieval_vs_oracle(glib, olib, in_examples, tmp_path, name):
     extra, bump = CASES[name]
+    case = CASE_OF.get(name, "clear")
     keys = BASE + extra
     print("case %s" % name, flush=True)
-    meas = measurements(olib, keys, bump)
+    meas = measurements(olib, keys, bump, case)
     # oracle
-    ctl, ret, _, atm, aero = setup(olib, "oracle_", keys)
+    ctl, ret, _, atm, aero = setup(olib, "oracle_", keys, case)
     n = olib.oracle_atm2x(C.byref(ctl), C.byref(atm), C.byref(aero), None, None, None)
     chisq, nc, err = np.zeros(1 + 20 * ret.conv_itmax), C.c_int(0), np.zeros(n)
     o_obs, o_atm, o_aero = J.obs_t(), J.atm_t(), J.aero_t()
@@ -104,7 +112,7 @@ def test_retrieval_vs_oracle(glib, olib, in_examples, tmp_path, name):
         old = min(old, v)
     assert min(gaps) > 1e-9, min(gaps)
     # GPU library
-    ctl, ret, _, atm, aero = setup(glib, "", keys)
+    ctl, ret, _, atm, aero = setup(glib, "", keys, case)
     ret.dir = str(tmp_path).encode()
     g_obs, g_atm, g_aero = J.obs_t(), J.atm_t(), J.aero_t()
     glib.optimal_estimation(C.byref(ret), C.byref(ctl), C.byref(_copy(meas)), C.byref(g_obs), C.byref(atm),
@@ -121,6 +129,11 @@ def test_retrieval_vs_oracle(glib, olib, in_examples, tmp_path, name):
     assert dt < 1e-5
     qg, qo = np.array(g_atm.q[1][:g_atm.np]), np.array(o_atm.q[1][:o_atm.np])
     np.testing.assert_allclose(qg, qo, rtol=1e-7)
+    if "RETNN" in extra:
+        print("%s: N a priori %g, retrieved gpu %.9g oracle %.9g" % (name, aero.nn[0], g_aero.nn[0], o_aero.nn[0]),
+              flush=True)
+        assert g_aero.nn[0] == pytest.approx(o_aero.nn[0], rel=1e-7)
+        assert o_aero.nn[0] / aero.nn[0] - 1 > 0.005  # the concentration moved towards the truth
     rg = np.array(g_obs.rad[0][:g_obs.nr])
     ro = np.array(o_obs.rad[0][:o_obs.nr])
     np.testing.assert_allclose(rg, ro, rtol=1e-8)
