@@ -109,6 +109,12 @@ size_t jur_meas_size(ctl_t *ctl, obs_t *obs);
    RESMAX, ERR_ANA, ERR_FORMOD[], ERR_NOISE[], ERR_PRESS*, ERR_TEMP*, ERR_Q*,
    ERR_K*, ERR_NN/RR/SS). */
 void read_ret(int argc, char *argv[], ctl_t *ctl, ret_t *ret);
+/* write_matrix (reference src/retrievalmodel.h, src/retrievalmodel.c:946-1099):
+   the non-zero elements of a matrix in measurement ("y") or state ("x")
+   space along rows and columns, sorted by row ("r") or column ("c"); only
+   with WRITE_MATRIX = 1, like the reference. */
+void write_matrix(const char *dirname, const char *filename, ctl_t *ctl, jur_gsl_matrix *matrix, atm_t *atm,
+                  aero_t *aero, obs_t *obs, const char *rowspace, const char *colspace, const char *sort);
 /* optimal_estimation (reference src/retrievalmodel.h,
    src/retrievalmodel.c:407-706): Levenberg-Marquardt retrieval of atm_i /
    aero_i from obs_meas with a priori atm_apr / aero_apr; writes costs.tab,

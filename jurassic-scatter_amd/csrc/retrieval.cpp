@@ -477,6 +477,19 @@ void kernel_matrix(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, std::vector
 
 }  // namespace
 
+// write_matrix for callers with a GSL-layout matrix (the kernel command)
+extern "C" void write_matrix(const char* dirname, const char* filename, ctl_t* ctl, jur_gsl_matrix* matrix,
+                             atm_t* atm, aero_t* aero, obs_t* obs, const char* rowspace, const char* colspace,
+                             const char* sort) {
+  static aero_t none;
+  if (!aero) aero = &none;
+  const size_t r = matrix->size1, c = matrix->size2;
+  std::vector<double> m(r * c);
+  for (size_t i = 0; i < r; i++)
+    for (size_t j = 0; j < c; j++) m[i * c + j] = matrix->data[i * matrix->tda + j];
+  write_matrix(dirname, filename, ctl, m, atm, aero, obs, rowspace, colspace, sort);
+}
+
 // read_ret (retrievalmodel.c:710-757)
 extern "C" void read_ret(int argc, char* argv[], ctl_t* ctl, ret_t* ret) {
   auto num = [&](const char* key, int idx, const char* dflt) { return scan_ctl(argc, argv, key, idx, dflt, nullptr); };

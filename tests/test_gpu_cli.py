@@ -149,3 +149,53 @@ def test_cli_retrieval_with_screening(tmp_path, monkeypatch):
     assert np.isnan(final[3, 10]) and np.all(np.isfinite(np.delete(final[:, 10], 3)))
     for f in ("costs.tab", "atm_err_total.tab", "atm_cont.tab", "atm_res.tab"):
         assert (d / f).stat().st_size > 0, f
+
+
+KERNEL = ROOT / "jurassic-scatter_amd" / "bin" / "kernel"
+
+
+def test_cli_kernel_vs_oracle(tmp_path):
+    """The kernel command (reference src/kernel.c): the Jacobian of the clear
+    example for temperature and H2O written by write_matrix (non-zero
+    elements, measurement space along rows), against the oracle's kernel;
+    without WRITE_MATRIX no file is written, as in the reference."""
+    keys = ["RETT_ZMIN", 0, "RETT_ZMAX", 40, "RETQ_ZMIN[1]", 0, "RETQ_ZMAX[1]", 20]
+    out = tmp_path / "kernel.tab"
+    r = subprocess.run([str(KERNEL), "clear-air.ctl", "org/obs.tab", "atm_pwin_0_80.tab", "-", str(out)] +
+                       [str(k) for k in keys], cwd=EXAMPLES, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and not out.exists(), r.stdout[-500:]
+    r = subprocess.run([str(KERNEL), "clear-air.ctl", "org/obs.tab", "atm_pwin_0_80.tab", "-", str(out),
+                        "WRITE_MATRIX", "1"] + [str(k) for k in keys], cwd=EXAMPLES, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-500:] + r.stderr[-500:]
+    got = {}
+    names = set()
+    for ln in out.read_text().splitlines():
+        if ln.strip() and not ln.startswith("#"):
+            v = ln.split()
+            got[(int(v[0]), int(v[6]))] = float(v[-1])
+            names.add(v[7])
+    assert names == {"TEMPERATURE", "H2O"}
+    olib = oracle_ctypes.load()
+    import os
+    cwd = os.getcwd()
+    os.chdir(EXAMPLES)
+    try:
+        ctl, obs, atm, aero = J.ctl_t(), J.obs_t(), J.atm_t(), J.aero_t()
+        argc, argv = J.argv("clear-air.ctl", *keys)
+        olib.oracle_read_ctl(argc, argv, C.byref(ctl))
+        olib.oracle_read_obs(None, b"org/obs.tab", C.byref(ctl), C.byref(obs))
+        olib.oracle_read_atm(None, b"atm_pwin_0_80.tab", C.byref(ctl), C.byref(atm))
+        n = olib.oracle_atm2x(C.byref(ctl), C.byref(atm), C.byref(aero), None, None, None)
+        m = olib.oracle_obs2y(C.byref(ctl), C.byref(obs), None, None, None)
+        k = np.zeros((m, n))
+        olib.oracle_kernel(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero),
+                           k.ctypes.data_as(C.POINTER(C.c_double)), m, n, n)
+    finally:
+        os.chdir(cwd)
+    kg = np.zeros((m, n))
+    for (i, j), v in got.items():
+        kg[i, j] = v
+    assert set(got) == {(i, j) for i in range(m) for j in range(n) if k[i, j] != 0}
+    # %g prints 6 significant digits
+    np.testing.assert_allclose(kg, k, rtol=1e-5, atol=1e-9 * np.abs(k).max())
