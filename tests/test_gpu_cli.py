@@ -199,3 +199,33 @@ def test_cli_kernel_vs_oracle(tmp_path):
     assert set(got) == {(i, j) for i in range(m) for j in range(n) if k[i, j] != 0}
     # %g prints 6 significant digits
     np.testing.assert_allclose(kg, k, rtol=1e-5, atol=1e-9 * np.abs(k).max())
+
+
+RAYTRACE = ROOT / "jurassic-scatter_amd" / "bin" / "raytrace"
+
+
+def test_cli_raytrace_golden(tmp_path):
+    """raytrace.sh: the raytrace command's los.0 on the example's limb
+    geometry against the reference's org/los.0 (the script's `diff -q`).
+    Lines must be identical text, except numbers within 1e-5 relative /
+    1e-9 absolute: a longitude of -4.2e-8 deg (atan2 of a rounding-level
+    Cartesian component) prints its 6th digit from the last FP64 bits, which
+    the device's atan2 need not share with the C library's."""
+    import shutil
+    for f in ("clear-air.ctl", "atm_pwin_0_80.tab"):
+        shutil.copy(EXAMPLES / f, tmp_path / f)
+    (tmp_path / "org").mkdir()
+    shutil.copy(EXAMPLES / "org" / "obs_raytrace.tab", tmp_path / "org" / "obs_raytrace.tab")
+    r = subprocess.run([str(RAYTRACE), "clear-air.ctl", "org/obs_raytrace.tab", "atm_pwin_0_80.tab",
+                        "TBLBASE", str(EXAMPLES / "tab" / "boxcar")],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-500:] + r.stderr[-500:]
+    got = (tmp_path / "los.0").read_text().splitlines()
+    want = (EXAMPLES / "org" / "los.0").read_text().splitlines()
+    assert len(got) == len(want)
+    differ = [(g, w) for g, w in zip(got, want) if g != w]
+    print("los.0: %d of %d lines differ in text" % (len(differ), len(want)), flush=True)
+    assert len(differ) <= 3
+    for g, w in differ:
+        a, b = np.array(g.split(), dtype=float), np.array(w.split(), dtype=float)
+        np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-9)
