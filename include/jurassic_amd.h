@@ -158,6 +158,12 @@ int jur_formod_batch_device(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero
 /* Wait for the library stream and check device-side error flags. */
 int jur_sync(void);
 
+/* Every emissivity table of ctl (binary if present, else ASCII) written in
+   the binary layout as TBLBASE_<nu>_<emitter>.bin: what the reference's
+   tab2bin tool does (src/tab2bin.c, through its padded tbl_t); the tab2bin
+   command calls it.  Host only.  Returns 0. */
+int jur_tab2bin(ctl_t *ctl);
+
 /* Select the GPU (call before anything else; default device 0). */
 int jur_init(int device);
 

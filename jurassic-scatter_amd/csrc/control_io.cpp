@@ -455,3 +455,45 @@ bool read_table_ascii(const char* filename, HostTable& tb) {
   }
   return true;
 }
+
+// tab2bin (reference src/tab2bin.c): every table of ctl -- binary if present,
+// else ASCII -- written in the binary layout (forwardmodel.c:636-651) as
+// TBLBASE_<nu>_<emitter>.bin.  A missing table is reported and skipped (the
+// reference would write one with np = -1).
+extern "C" int jur_tab2bin(ctl_t* ctl) {
+  static char fn[2 * LEN + 64];
+  for (int ig = 0; ig < ctl->ng; ig++)
+    for (int id = 0; id < ctl->nd; id++) {
+      HostTable tb;
+      snprintf(fn, sizeof fn, "%s_%.4f_%s.bin", ctl->tblbase, ctl->nu[id], ctl->emitter[ig]);
+      bool ok = read_table_bin(fn, tb);
+      if (!ok) {
+        snprintf(fn, sizeof fn, "%s_%.4f_%s.tab", ctl->tblbase, ctl->nu[id], ctl->emitter[ig]);
+        ok = read_table_ascii(fn, tb);
+      }
+      if (!ok) {
+        printf("Missing emissivity table: %s\n", fn);
+        continue;
+      }
+      snprintf(fn, sizeof fn, "%s_%.4f_%s.bin", ctl->tblbase, ctl->nu[id], ctl->emitter[ig]);
+      printf("Write binary table: %s\n", fn);
+      FILE* out = fopen(fn, "wb");
+      if (!out) JUR_FAIL("Cannot create file!");
+      auto put = [&](const void* d, size_t size, size_t n) {
+        if (n && fwrite(d, size, n, out) != n) JUR_FAIL("Error while writing!");
+      };
+      put(&tb.np, sizeof(int), 1);
+      put(tb.p.data(), sizeof(double), tb.np);
+      put(tb.nt.data(), sizeof(int), tb.np);
+      for (int ip = 0; ip < tb.np; ip++) {
+        put(tb.t[ip].data(), sizeof(double), tb.nt[ip]);
+        put(tb.nu[ip].data(), sizeof(int), tb.nt[ip]);
+        for (int it = 0; it < tb.nt[ip]; it++) {
+          put(tb.u[ip][it].data(), sizeof(float), tb.nu[ip][it]);
+          put(tb.eps[ip][it].data(), sizeof(float), tb.nu[ip][it]);
+        }
+      }
+      fclose(out);
+    }
+  return 0;
+}

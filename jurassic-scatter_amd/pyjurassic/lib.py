@@ -37,6 +37,7 @@ SIGNATURES = {
     "jur_meas_size": (C.c_size_t, [P, P]),
     "read_ret": (None, [C.c_int, P, P, P]),
     "optimal_estimation": (None, [P, P, P, P, P, P, P, P]),
+    "jur_tab2bin": (C.c_int, [P]),
     "write_matrix": (None, [C.c_char_p, C.c_char_p, P, P, P, P, P, C.c_char_p, C.c_char_p, C.c_char_p]),
     "jur_formod_batch": (C.c_int, [P, P, P, C.c_int]),
     "jur_formod_batch_device": (C.c_int, [P, P, P, C.c_int, P]),

@@ -92,3 +92,24 @@ def test_copy_aero(glib, in_examples):
     for im in range(dst.nm):
         assert (dst.top_mod[im], dst.nn[im], dst.rr[im], dst.ss[im]) == (0, 0, 0, 1)
     assert dst.beta_e[0][0] == 0 and dst.p[1][0][7] == 0 and dst.nmod[0] == 0
+
+
+def test_tab2bin_round_trip(tmp_path):
+    """The tab2bin command (reference src/tab2bin.c): the example's binary
+    tables written as ASCII rows ("p T u eps", every digit kept) and
+    converted back give the original .bin files byte for byte; run on the
+    host (no GPU call)."""
+    import subprocess
+    from conftest import EXAMPLES, ROOT
+    from test_gpu_formod import _bin_table
+    for g in ("CO2", "H2O", "O3"):
+        lines = ["# p T u eps"]
+        for p, t, u, e in _bin_table(EXAMPLES / "tab" / ("boxcar_792.0000_%s.bin" % g)):
+            lines += ["%.17g %.17g %.9g %.9g" % (p, t, u[k], e[k]) for k in range(len(u))]
+        (tmp_path / ("boxcar_792.0000_%s.tab" % g)).write_text("\n".join(lines) + "\n")
+    r = subprocess.run([str(ROOT / "jurassic-scatter_amd" / "bin" / "tab2bin"), str(EXAMPLES / "clear-air.ctl"),
+                        "TBLBASE", str(tmp_path / "boxcar")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout[-500:] + r.stderr[-500:]
+    for g in ("CO2", "H2O", "O3"):
+        name = "boxcar_792.0000_%s.bin" % g
+        assert (tmp_path / name).read_bytes() == (EXAMPLES / "tab" / name).read_bytes(), name
