@@ -35,6 +35,23 @@
  *       reference ships no kernel output); its columns are checked against
  *       formod on independently perturbed atmospheres, first-order
  *       linearity, and the zero pattern of levels a ray never reaches.
+ *   optimal_estimation, cov_apr, cov_meas, read_ret: parity unpinned by
+ *       fixtures (no retrieval output ships); checked by closed forms with
+ *       numpy: covariance elements (vertical and surface-chord
+ *       correlations), the first damped step, the cost function, the error
+ *       analysis against a numpy inverse, a vanishing gradient at
+ *       convergence.
+ *   intpol_atm_2d / 3d (IP 2, 3): parity unpinned by fixtures (every
+ *       example uses IP 1); checked by projection weights computed with a dot
+ *       product, clamping, IP 2 = IP 1 for identical profiles, a symmetric
+ *       IP 3 window reproducing a linear profile, chord-distance weights.
+ *   srcfunc_sca_3d: parity unpinned by fixtures; checked by the isothermal
+ *       opaque closure (J = B(T) with one channel) and the reference's
+ *       normalisation over all channels (J = B(T) / 2 for two identical
+ *       channels).
+ *   opt_prop_external (Ext aerosol type): parity unpinned by fixtures (no
+ *       table ships); checked by the nearest-row rule with its tie, the
+ *       mode sums and the NN / Const exits.
  */
 #ifndef JURASSIC_ORACLE_H
 #define JURASSIC_ORACLE_H
