@@ -838,7 +838,7 @@ __device__ __forceinline__ void finish64(const PathArgs& A, const double* prow, 
   A.tau[o] = (float)tau;
 }
 
-template <int NGT, bool HINT>
+template <int NGT, bool HINT, bool R64>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_path64(PathArgs A) {
   const int lane = threadIdx.x & 63;
   int chunk, group;
@@ -864,11 +864,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
   }
   double rad = 0.0, tau = 1.0;
   int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
-  // point records: the FP64 copy when the trace wrote one
+  // point records: the FP64 copy when the trace wrote one (R64)
   const size_t rbase = (size_t)ray * A.cap * A.R;
   auto rec = [&](int ip, int f) -> double {
     const size_t i = rbase + (size_t)ip * A.R + f;
-    return A.rec64 ? A.rec64[i] : (double)A.rec[i];
+    return R64 ? A.rec64[i] : (double)A.rec[i];
   };
   for (int ip = 0; ip < npts; ip++) {
     const double p = rec(ip, REC_P), t = rec(ip, REC_T), ds = rec(ip, REC_DS), af = rec(ip, REC_AF);
@@ -946,10 +946,18 @@ void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
   const int ngroups = (A.nray + 63) / 64;
   long nb = (long)A.nd * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;
-  if (A.T.shared && NGT <= 8)
-    hipLaunchKernelGGL((k_path64<NGT, NGT <= 8>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
-  else
-    hipLaunchKernelGGL((k_path64<NGT, false>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  const bool r64 = A.rec64 != nullptr;
+  if (A.T.shared && NGT <= 8) {
+    if (r64)
+      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, true>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, false>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  } else {
+    if (r64)
+      hipLaunchKernelGGL((k_path64<NGT, false, true>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_path64<NGT, false, false>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  }
 }
 
 template <int NGT, int CPW, int OCC>
