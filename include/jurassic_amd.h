@@ -188,7 +188,12 @@ int jur_init(int device);
    resolves; optimal_estimation's forward calls follow the same setting),
    "fp64_split" (FP64 path with a ray's emitters on separate lanes: -1, the
    default, for calls too small to fill the GPU with one lane per ray; 1
-   always; 0 never; results are identical).
+   always; 0 never; results are identical), "kernel_prefix" (1: kernel
+   integrates each perturbed copy's rays from their first LOS point whose
+   records differ from the unperturbed ray's, starting from its path state
+   there -- bit-identical, 30-45 % fewer points on the reference examples,
+   but slower on the GPU as measured (the wave's lanes no longer walk the
+   same points of copies of one ray); default 0).
    Returns 0. */
 int jur_set_option(const char *key, double value);
 /* Statistics of the last call: "path_ms" (path-integration kernel time,

@@ -16,6 +16,8 @@
 namespace {
 
 struct Engine {
+  int prefix_nr = 0;          // FP64 prefix reuse: rays are copies of the first prefix_nr
+  DBuf pstart, prem, pids, snap;
   bool ready = false;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -714,6 +716,20 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
                       float* tau, int out_ld, int out_off, hipStream_t s, double* rad64, double* tau64) {
   Engine& e = E();
   int* order = (int*)e.order.need(nray * sizeof(int));
+  // FP64 prefix reuse (engine_set_prefix): clear air, FP64 records of this
+  // trace, rays = copies of the first pnr, snapshot buffer of modest size
+  const int R_ = rec_len(ctl->ng, ctl->nw);
+  const bool fp64 = engine_option_int("fp64", 0) != 0;
+  const int pnr = e.prefix_nr;
+  const size_t snap_bytes = (size_t)(pnr > 0 ? pnr : 0) * ctl->nd * (size_t)(cap + 1) * (ctl->ng + 2) * sizeof(double);
+  const bool prefix = fp64 && scattering == 0 && pnr > 0 && nray > pnr && nray % pnr == 0 && !e.capturing &&
+                      e.last_rec64 && e.last_nray == nray && e.last_cap == cap && snap_bytes < ((size_t)1 << 30);
+  int *pstart = nullptr, *prem = nullptr;
+  if (prefix) {
+    pstart = (int*)e.pstart.need(nray * sizeof(int));
+    prem = (int*)e.prem.need(nray * sizeof(int));
+    launch_prefix_start(nray, pnr, R_, cap, e.last_rec64, npts, pstart, prem, s);
+  }
   if (e.capturing) {
     // counting sort: kernels only (no library temporaries or memsets)
     launch_count_sort(npts, nray, cap + 1, (int*)e.hist.need((size_t)(cap + 1) * sizeof(int)), order, s);
@@ -723,7 +739,8 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     size_t tb = sort_temp_bytes(nray);
     void* tmp = e.sorttmp.need(tb);
     launch_iota(ids, nray, s);
-    launch_sort_rays(npts, nullptr, keys, ids, order, tmp, tb, nray, s);
+    // by the points each ray integrates (all, or those after its prefix)
+    launch_sort_rays(prefix ? prem : npts, nullptr, keys, ids, order, tmp, tb, nray, s);
   }
   PathArgs P;
   memset(&P, 0, sizeof P);
@@ -763,6 +780,12 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   P.rec64 = (P.fp64 && e.last_nray == nray && e.last_cap == cap) ? e.last_rec64 : nullptr;
   P.tsurf64 = P.rec64 ? e.last_tsurf64 : nullptr;
   int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
+  if (P.fp64) {
+    // the FP64 kernels have one configuration (variant and waves per block
+    // apply to the FP32 kernels only)
+    if (variant < 0) variant = 2;
+    if (wpb < 0) wpb = 4;
+  }
   if (variant < 0 || wpb < 0) {
     // Autotuner: the first large batch of a (emitters, channels, size class)
     // shape times each kernel variant (channels per wave x occupancy) and
@@ -806,7 +829,15 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     if (wpb < 0) wpb = best % 8;
   }
   P.cpw = variant;
-  launch_sum_points(npts, nray, (unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
+  // LOS points integrated: with prefix reuse the base rays' and every ray's
+  // remainder
+  unsigned long long* acc = (unsigned long long*)e.pts_acc.need(sizeof(unsigned long long));
+  if (prefix) {
+    launch_sum_points(prem, nray, acc, s);
+    launch_sum_points(npts, pnr, acc, s);
+  } else {
+    launch_sum_points(npts, nray, acc, s);
+  }
   if (e.capturing) {
     launch_path(P, wpb, s);
     return;
@@ -819,9 +850,28 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   }
   auto& ev = e.path_ev[e.path_ev_used++];
   HIP_OK(hipEventRecord(ev.first, s));
+  if (prefix) {
+    // the base rays first, writing their path state at every point; then
+    // every ray from its first differing point (base rays: their end)
+    PathArgs B = P;
+    int* pids = (int*)e.pids.need(pnr * sizeof(int));
+    launch_iota(pids, pnr, s);
+    B.nray = pnr;
+    B.order = pids;
+    B.snap_mode = 1;
+    B.pnr = pnr;
+    B.snap = (double*)e.snap.need(snap_bytes);
+    launch_path(B, wpb, s);
+    P.snap_mode = 2;
+    P.pnr = pnr;
+    P.pstart = pstart;
+    P.snap = B.snap;
+  }
   launch_path(P, wpb, s);
   HIP_OK(hipEventRecord(ev.second, s));
 }
+
+void engine_set_prefix(int nr) { E().prefix_nr = nr; }
 
 extern "C" int jur_set_option(const char* key, double value) {
   Engine& e = E();

@@ -139,7 +139,12 @@ void run_copies(ctl_t* ctl, const atm_t* atm, const obs_t* obs, const aero_t* ae
   HIP_OK(hipStreamSynchronize(s));
   engine_prepare(ctl);
   engine_set_aero(ctl, ctl->sca_n > 0 ? aero : nullptr);
+  // the copies' rays repeat copy 0's up to the first point a perturbation
+  // reaches (option "kernel_prefix", default 0: measured slower, see the
+  // option's description in jurassic_amd.h)
+  engine_set_prefix(C > 1 && engine_option_int("kernel_prefix", 0) ? nr : 0);
   engine_forward(ctl, A, R, ctl->sca_mult, O, s);
+  engine_set_prefix(0);
   double *rad, *tau;
   formod_epilogue(ctl, (int)N, nr, R.time, O, &rad, &tau, s);
   y.resize(nd * N);

@@ -103,6 +103,10 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
                     int scattering, const OutDev& out, hipStream_t s);
 hipStream_t engine_stream();
 int engine_option_int(const char* key, int dflt);
+// FP64 clear-air calls whose rays are copies of the first nr rays (the
+// Jacobian's perturbed states): each ray is integrated from its first LOS
+// point that differs from its base ray (0: off).
+void engine_set_prefix(int nr);
 double engine_stat(const char* key);
 void engine_set_stat(const char* key, double v);
 

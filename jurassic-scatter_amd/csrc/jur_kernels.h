@@ -61,6 +61,14 @@ struct PathArgs {
   int cpw;                 // kernel variant (channels per wave x occupancy), see launch_path
   int fp64;                // 1: FP64 arithmetic (k_path64); also writes rad64/tau64 if set
   int fp64_split;          // FP64: emitters on separate lanes (k_path64e): 1 on, 0 off, -1 small calls
+  // FP64 prefix reuse (clear air): rays [0, pnr) are bases, ray i >= pnr
+  // repeats base i % pnr up to its first differing LOS point.  snap_mode 1:
+  // the (base) rays write their path state before every point and after the
+  // last into snap [pnr][nd][cap + 1][ng + 2] (tau_path per emitter, rad,
+  // tau); 2: every ray starts at pstart[ray] from its base's state there.
+  int snap_mode, pnr;
+  const int* pstart;
+  double* snap;
   double *rad64, *tau64;   // [nd][out_ld] (optional)
   const double* rec64;     // FP64 records / surface temperatures (optional, fp64)
   const double* tsurf64;
@@ -81,6 +89,11 @@ size_t sort_temp_bytes(int n);
 void launch_sort_rays(const int* np, int* keys_tmp, int* keys_out, int* ids_in, int* ids_out,
                       void* temp, size_t temp_bytes, int n, hipStream_t s);
 void launch_iota(int* ids, int n, hipStream_t s);
+// First LOS point at which ray i >= pnr differs from ray i % pnr (FP64
+// records, point counts); start[i] (np[i] for the bases) and the points left,
+// rem[i] = np[i] - start[i].
+void launch_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,
+                         int* rem, hipStream_t s);
 void launch_zero_int(int* p, int n, hipStream_t s);
 // Rays by descending point count into order (hist: nbins ints of scratch).
 void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, hipStream_t s);

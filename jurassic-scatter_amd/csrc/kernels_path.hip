@@ -864,13 +864,38 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
   }
   double rad = 0.0, tau = 1.0;
   int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
+  // prefix reuse: the state of the base ray before point ip0
+  const int snw = A.ng + 2;
+  double* sp = nullptr;
+  int ip0 = 0;
+  if (A.snap_mode && valid) {
+    sp = A.snap + ((size_t)(ray % A.pnr) * A.nd + id) * (size_t)(A.cap + 1) * snw;
+    if (A.snap_mode == 2) {
+      ip0 = A.pstart[ray];
+      const double* st = sp + (size_t)ip0 * snw;
+#pragma unroll
+      for (int g = 0; g < NGT; g++)
+        if (g < A.ng) tau_path[g] = st[g];
+      rad = st[A.ng];
+      tau = st[A.ng + 1];
+    }
+  }
+  auto save = [&](int ip) {
+    double* st = sp + (size_t)ip * snw;
+#pragma unroll
+    for (int g = 0; g < NGT; g++)
+      if (g < A.ng) st[g] = tau_path[g];
+    st[A.ng] = rad;
+    st[A.ng + 1] = tau;
+  };
   // point records: the FP64 copy when the trace wrote one (R64)
   const size_t rbase = (size_t)ray * A.cap * A.R;
   auto rec = [&](int ip, int f) -> double {
     const size_t i = rbase + (size_t)ip * A.R + f;
     return R64 ? A.rec64[i] : (double)A.rec[i];
   };
-  for (int ip = 0; ip < npts; ip++) {
+  for (int ip = ip0; ip < npts; ip++) {
+    if (A.snap_mode == 1) save(ip);
     const double p = rec(ip, REC_P), t = rec(ip, REC_T), ds = rec(ip, REC_DS), af = rec(ip, REC_AF);
     const double qh = rec(ip, REC_QH2O);
     double tau_gas = 1.0;
@@ -882,6 +907,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     point64(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau);
   }
   if (!valid) return;
+  if (A.snap_mode == 1) save(npts);
   finish64(A, prow, id, ray, rad, tau);
 }
 
@@ -930,6 +956,28 @@ __global__ void __launch_bounds__(256) k_path64e(PathArgs A) {
   }
   if (!valid || sub != 0) return;
   finish64(A, prow, id, ray, rad, tau);
+}
+
+__global__ void k_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,
+                               int* rem) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nray) return;
+  const int n = np[i];
+  int f = n;
+  if (i >= pnr) {
+    const int b = i % pnr, nb = np[b], nmin = n < nb ? n : nb;
+    const double* ri = rec64 + (size_t)i * cap * R;
+    const double* rb = rec64 + (size_t)b * cap * R;
+    f = 0;
+    for (; f < nmin; f++) {
+      bool same = true;
+      for (int k = 0; k < R && same; k++)
+        same = __double_as_longlong(ri[(size_t)f * R + k]) == __double_as_longlong(rb[(size_t)f * R + k]);
+      if (!same) break;
+    }
+  }
+  start[i] = f;
+  rem[i] = n - f;
 }
 
 void launch_64e(const PathArgs& A, int wpb, hipStream_t s) {
@@ -996,7 +1044,7 @@ void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
     // emitters on separate lanes when one lane per ray leaves the GPU
     // mostly idle (fewer than 2048 waves: 8 per CU)
     const long waves = (long)A.nd * ((A.nray + 63) / 64);
-    if (A.ng <= 8 && (A.fp64_split > 0 || (A.fp64_split < 0 && waves < 2048))) {
+    if (A.ng <= 8 && A.snap_mode == 0 && (A.fp64_split > 0 || (A.fp64_split < 0 && waves < 2048))) {
       launch_64e(A, 4, s);
       return;
     }
@@ -1105,4 +1153,11 @@ void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, h
 
 void launch_zero_int(int* p, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+void launch_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,
+                         int* rem, hipStream_t s) {
+  if (nray > 0)
+    hipLaunchKernelGGL(k_prefix_start, dim3((nray + 127) / 128), dim3(128), 0, s, nray, pnr, R, cap, rec64, np,
+                       start, rem);
 }

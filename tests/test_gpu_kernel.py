@@ -191,3 +191,26 @@ def test_fp64_emitter_lanes_bit_identical(glib, in_examples, case):
         glib.jur_set_option(b"fp64", 0)
     print("%s: %d values, max |diff| %.3g" % (case, out[0].size, np.max(np.abs(out[0] - out[1]))), flush=True)
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("name", ["clear_t_p_q", "clear_hydz_bbt", "ext1_aerosol"])
+def test_prefix_reuse_bit_identical(glib, in_examples, name):
+    """A perturbed copy's ray repeats copy 0's up to its first differing LOS
+    point (FP64 records compared bit for bit) and is integrated from there,
+    starting from copy 0's path state: the Jacobian is bit-identical to the
+    one integrated from the observer (option kernel_prefix 0)."""
+    case, keys, _ = CASES[name]
+    ks = []
+    for pre in (0, 1):
+        glib.jur_set_option(b"kernel_prefix", pre)
+        try:
+            k, _, _ = jacobian(glib, "", case, keys)
+            pts = glib.jur_stat(b"integrated_points")
+        finally:
+            glib.jur_set_option(b"kernel_prefix", 0)
+        ks.append((k, pts))
+    print("%s: LOS points integrated %d (prefix reuse) vs %d" % (name, ks[1][1], ks[0][1]), flush=True)
+    np.testing.assert_array_equal(ks[1][0], ks[0][0])
+    # the statistic is of the last batched call: an aerosol column (its own
+    # call, no copies to reuse) in ext1_aerosol
+    assert ks[1][1] < ks[0][1] or name == "ext1_aerosol"
