@@ -12,6 +12,14 @@ integration), timed with HIP events recorded by the library on its own
 stream; `cpu_baseline` times the FP64 oracle on a bounded sample of the same
 rays on this host's cores, and `accuracy` compares the GPU radiances of that
 sample with the oracle's.
+
+Other workloads (--workload): the synthetic generator's other cases
+(clear_limb_17, aerosol_limb_8192, cirrus_4096: scattering), cirrus_16384
+(the learned scattering source against the plain method), jacobian_limb_17
+(kernel(): the finite-difference Jacobian), retrieval_limb_17
+(optimal_estimation) and aerosol3d_steep_16 (srcfunc_sca_3d's direct fans
+over a 2-D atmosphere); each prints its own JSON line of the same form.
+Library options for experiments: --option KEY=VALUE (jur_set_option).
 """
 import argparse
 import ctypes as C
