@@ -291,18 +291,6 @@ void write_matrix(const char* dirname, const char* filename, const ctl_t* ctl, c
   fclose(out);
 }
 
-// corr_function (retrievalmodel.c:91-108): exp(-d_h/ch - |dz|/cz) with the
-// horizontal distance of the two points at the surface.
-double corr_function(double z0, double lon0, double lat0, double z1, double lon1, double lat1, double cz,
-                     double ch) {
-  double x0[3], x1[3];
-  geo2cart(0, lon0, lat0, x0);
-  geo2cart(0, lon1, lat1, x1);
-  const double d = std::sqrt((x0[0] - x1[0]) * (x0[0] - x1[0]) + (x0[1] - x1[1]) * (x0[1] - x1[1]) +
-                             (x0[2] - x1[2]) * (x0[2] - x1[2]));
-  return std::exp(-d / ch - std::fabs(z0 - z1) / cz);
-}
-
 // set_cov_apr (retrievalmodel.c:761-857)
 void set_cov_apr(const ret_t* ret, const ctl_t* ctl, const atm_t* atm, const aero_t* aero, const StateVec& sv,
                  std::vector<double>& s_a) {
@@ -324,6 +312,8 @@ void set_cov_apr(const ret_t* ret, const ctl_t* ctl, const atm_t* atm, const aer
   }
   s_a.assign(n * n, 0.0);
   for (size_t i = 0; i < n; i++) s_a[i * n + i] = x[i] * x[i];
+  std::vector<double> xs(3 * (size_t)(atm->np > 0 ? atm->np : 1));
+  for (int ip = 0; ip < atm->np; ip++) geo2cart(0, atm->lon[ip], atm->lat[ip], &xs[3 * (size_t)ip]);
   for (size_t i = 0; i < n; i++)
     for (size_t j = 0; j < n; j++)
       if (i != j && sv.iqa[i] == sv.iqa[j]) {
@@ -348,10 +338,14 @@ void set_cov_apr(const ret_t* ret, const ctl_t* ctl, const atm_t* atm, const aer
             ch = ret->err_k_ch[iw];
           }
         if (cz > 0 && ch > 0) {
+          // corr_function (retrievalmodel.c:91-108): exp(-d_h/ch - |dz|/cz),
+          // d_h the chord between the surface points (computed once per level)
           const int a = sv.ipa[i], b = sv.ipa[j];
-          s_a[i * n + j] = x[i] * x[j] *
-                           corr_function(atm->z[a], atm->lon[a], atm->lat[a], atm->z[b], atm->lon[b], atm->lat[b],
-                                         cz, ch);
+          const double* xa = &xs[3 * (size_t)a];
+          const double* xb = &xs[3 * (size_t)b];
+          const double d = std::sqrt((xa[0] - xb[0]) * (xa[0] - xb[0]) + (xa[1] - xb[1]) * (xa[1] - xb[1]) +
+                                     (xa[2] - xb[2]) * (xa[2] - xb[2]));
+          s_a[i * n + j] = x[i] * x[j] * std::exp(-d / ch - std::fabs(atm->z[a] - atm->z[b]) / cz);
         }
       }
 }
