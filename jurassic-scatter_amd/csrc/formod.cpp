@@ -246,14 +246,20 @@ extern "C" void formod(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero) {
   ctl->queue.state = Queue_Inactive;
   double *radd, *taud;
   formod_epilogue(ctl, nr, nr, R.time, O, &radd, &taud, s);
-  for (int id = 0; id < nd; id++) {
-    HIP_OK(hipMemcpyAsync(obs->rad[id], radd + (size_t)id * nr, nr * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(obs->tau[id], taud + (size_t)id * nr, nr * sizeof(double), hipMemcpyDeviceToHost, s));
-  }
+  // one copy per [nd][nr] device array, then a host scatter into the
+  // [NDMAX][NRMAX] rows (a copy per channel costs a staging blit and a wait)
+  std::vector<double> h((size_t)2 * nd * nr);
+  HIP_OK(hipMemcpyAsync(h.data(), radd, (size_t)nd * nr * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipMemcpyAsync(h.data() + (size_t)nd * nr, taud, (size_t)nd * nr * sizeof(double), hipMemcpyDeviceToHost,
+                        s));
   HIP_OK(hipMemcpyAsync(obs->tpz, O.tpz, nr * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_OK(hipMemcpyAsync(obs->tplon, O.tplon, nr * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_OK(hipMemcpyAsync(obs->tplat, O.tplat, nr * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
+  for (int id = 0; id < nd; id++) {
+    std::memcpy(obs->rad[id], &h[(size_t)id * nr], nr * sizeof(double));
+    std::memcpy(obs->tau[id], &h[(size_t)(nd + id) * nr], nr * sizeof(double));
+  }
   for (int id = 0; id < nd; id++)
     for (int ir = 0; ir < nr; ir++)
       if (mask[id][ir]) obs->rad[id][ir] = NAN;

@@ -152,8 +152,8 @@ void run_copies(ctl_t* ctl, const atm_t* atm, const obs_t* obs, const aero_t* ae
   if (c0) {
     c0->tau.resize((size_t)nd * nr);
     c0->tp.resize((size_t)3 * nr);
-    for (int id = 0; id < nd; id++)
-      HIP_OK(hipMemcpyAsync(&c0->tau[(size_t)id * nr], tau + (size_t)id * N, nr * sizeof(double),
+    // copy 0's transmittances: the first nr of each channel's N, one strided copy
+    HIP_OK(hipMemcpy2DAsync(c0->tau.data(), nr * sizeof(double), tau, N * sizeof(double), nr * sizeof(double), nd,
                             hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&c0->tp[0], O.tpz, nr * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&c0->tp[nr], O.tplon, nr * sizeof(double), hipMemcpyDeviceToHost, s));
