@@ -215,7 +215,8 @@ def bench_surrogate(args, lib):
     network (trained on the plain method over cirrus_train_8192 by
     pyjurassic.surrogate) gives J at the primaries' scattering points, the
     fused matrix-core kernel evaluates it; the plain method on the first
-    --plain-rays rays judges it (radiance difference, speedup of the call)."""
+    --plain-rays rays is the reference it is compared with (radiance difference,
+    speedup of the call)."""
     import torch
     from pyjurassic import surrogate as S
     weights = Path(args.weights) if args.weights else SURROGATE_WEIGHTS

@@ -1,7 +1,7 @@
 """The learned scattering source in the forward model: training pairs from
 the plain secondary-pencil method, a network trained on them on the GPU,
 and the forward model with the network in place of the plain angular
-integral at the primaries' scattering points, judged against the plain
+integral at the primaries' scattering points, compared with the plain
 method on rays the network has not seen; the fallback path (every point
 outside the training hull) reproduces the plain method exactly."""
 import ctypes as C
