@@ -512,6 +512,9 @@ def bench_retrieval(args, lib):
                      "alg_bytes_per_retrieval": alg_bytes,
                      "note": "path kernel time and algorithmic bytes summed over the retrieval's calls"},
     }
+    # wall time of the last timed retrieval's phases (library timers)
+    fm, km, tm = (lib.jur_stat(k) for k in (b"oe_formod_ms", b"oe_kernel_ms", b"oe_total_ms"))
+    line["phases_ms"] = {"forward_models": fm, "jacobians": km, "algebra_and_files": tm - fm - km, "total": tm}
     if not args.no_cpu:
         import oracle_ctypes
         olib = oracle_ctypes.load()

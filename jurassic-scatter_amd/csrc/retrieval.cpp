@@ -5,6 +5,7 @@
 // jur_kernel); the dense linear algebra (K^T S K products, gain and
 // averaging-kernel matrices, Cholesky solves and inverses) runs on the GPU
 // with rocBLAS / rocSOLVER on host-resident row-major matrices.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -464,7 +465,20 @@ struct Fp64Scope {
   ~Fp64Scope() { jur_set_option("fp64", saved); }
 };
 
+// wall-clock time of one retrieval's phases, reported through jur_stat as
+// oe_formod_ms, oe_kernel_ms and oe_total_ms (the rest is host-side algebra,
+// rocBLAS / rocSOLVER calls and output files)
+double oe_formod_ms, oe_kernel_ms;
+
+struct PhaseClock {
+  double& acc;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit PhaseClock(double& a) : acc(a) {}
+  ~PhaseClock() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
 void kernel_matrix(ctl_t* ctl, atm_t* atm, obs_t* obs, aero_t* aero, std::vector<double>& k, size_t m, size_t n) {
+  PhaseClock clock(oe_kernel_ms);
   k.assign(m * n, 0.0);
   jur_kernel(ctl, atm, obs, aero, k.data(), m, n, n);
 }
@@ -519,6 +533,8 @@ extern "C" void read_ret(int argc, char* argv[], ctl_t* ctl, ret_t* ret) {
 extern "C" void optimal_estimation(ret_t* ret, ctl_t* ctl, obs_t* obs_meas, obs_t* obs_i, atm_t* atm_apr,
                                    atm_t* atm_i, aero_t* aero_apr, aero_t* aero_i) {
   Fp64Scope fp64;
+  const auto t_start = std::chrono::steady_clock::now();
+  oe_formod_ms = oe_kernel_ms = 0;
   // clear air: the aerosol arguments may be NULL
   static aero_t none_apr, none_i;
   if (!aero_apr) aero_apr = &none_apr;
@@ -539,7 +555,10 @@ extern "C" void optimal_estimation(ret_t* ret, ctl_t* ctl, obs_t* obs_meas, obs_
   copy_atm(ctl, atm_i, atm_apr, 0);
   copy_obs(ctl, obs_i, obs_meas, 0);
   copy_aero(ctl, aero_i, aero_apr, 0);
-  formod(ctl, atm_i, obs_i, aero_i);
+  {
+    PhaseClock clock(oe_formod_ms);
+    formod(ctl, atm_i, obs_i, aero_i);
+  }
   {
     StateVec s;
     state_vector(ctl, atm_apr, aero_apr, s);
@@ -608,7 +627,10 @@ extern "C" void optimal_estimation(ret_t* ret, ctl_t* ctl, obs_t* obs_meas, obs_
         aero_i->rr[im] = std::fmin(std::fmax(aero_i->rr[im], 1e-2), 100.0);
         aero_i->ss[im] = std::fmin(std::fmax(aero_i->ss[im], 1.1), 3.0);
       }
-      formod(ctl, atm_i, obs_i, aero_i);
+      {
+        PhaseClock clock(oe_formod_ms);
+        formod(ctl, atm_i, obs_i, aero_i);
+      }
       meas_vector(ctl, obs_i, y_i.data(), nullptr, nullptr);
       for (size_t i = 0; i < n; i++) dx[i] = x_i[i] - x_a[i];
       for (size_t i = 0; i < m; i++) dy[i] = y_m[i] - y_i[i];
@@ -655,4 +677,8 @@ extern "C" void optimal_estimation(ret_t* ret, ctl_t* ctl, obs_t* obs_meas, obs_
     write_matrix(ret->dir, "matrix_avk.tab", ctl, a, atm_i, aero_i, obs_i, "x", "x", "r");
     analyze_avk(ret, ctl, atm_i, aero_i, sv, a);
   }
+  engine_set_stat("oe_formod_ms", oe_formod_ms);
+  engine_set_stat("oe_kernel_ms", oe_kernel_ms);
+  engine_set_stat("oe_total_ms",
+                  std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count());
 }
