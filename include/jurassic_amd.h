@@ -206,7 +206,9 @@ int jur_set_option(const char *key, double value);
    "fallback_points", "mlp_ms", "mlp_samples", "train_ms",
    "shared_grids" (1: the fast table layout is in use), "mie_spheres"
    (spheres of the last get_opt_prop), "kernel_ms" (wall time of the last
-   kernel call), "kernel_columns"; running totals over all calls since
+   kernel call), "kernel_columns", "oe_formod_ms", "oe_kernel_ms",
+   "oe_total_ms" (wall time of the last optimal_estimation call: its
+   forward models, its Jacobians, in total); running totals over all calls since
    the library loaded: "calls_total", "path_ms_total",
    "integrated_points_total", "pencils_total".  Unknown keys give NaN. */
 double jur_stat(const char *key);
