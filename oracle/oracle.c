@@ -1814,6 +1814,14 @@ static void o_srcfunc_sca_3d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, d
   free(tau2);
 }
 
+/* The solar term alone, added to src_sca (tables and filters of ctl loaded
+   on demand), for the tests that pin it. */
+void oracle_srcfunc_sca_sun(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double sec, const double *x,
+                            const double *dx, int il, double *src_sca) {
+  o_init_tables(ctl);
+  o_srcfunc_sca_sun(ctl, atm, aero, sec, x, dx, il, src_sca);
+}
+
 /* srcfunc_sca (scatter.c:643-662) */
 static void o_srcfunc_sca(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double sec,
                           const double *x, const double *dx, int il, double *src_sca,
@@ -1955,6 +1963,9 @@ static void o_formod_fov(ctl_t *ctl, obs_t *obs) {
   free(tt);
   free(obs2);
 }
+
+/* The FOV convolution alone, for the tests that pin it. */
+void oracle_formod_fov(ctl_t *ctl, obs_t *obs) { o_formod_fov(ctl, obs); }
 
 /* formod (forwardmodel.c:15-116), non-queue execution order. */
 void oracle_formod(ctl_t *ctl, atm_t *atm, obs_t *obs, aero_t *aero) {

@@ -18,15 +18,26 @@
  *       examples/org/rad_ext1.tab; bhmie also against Rayleigh limit,
  *       k=0 energy conservation, phase normalisation; gauher against exact
  *       Gaussian moments.
- *   srcfunc_sca_1d, srcfunc_sca_sun, scattering radiance (depth 1):
+ *   srcfunc_sca_1d, srcfunc_sca_sun at night, scattering radiance (depth 1):
  *       examples/org/rad_aero0.tab, rad_aero1.tab; isotropic-field closure.
  *   suncoord: solstice/equinox declination and local-noon checks.
  *   brightness: closed-form inverse of planck (no reference fixture uses it).
  *   hydrostatic: closed form for an isothermal dry column.
  *   ctmn2, ctmo2: parity unpinned by fixtures (no fixture channel lies in
  *       their bands); checked against direct evaluation at table nodes.
- *   formod_fov: parity unpinned by fixtures; checked against a delta-shaped
- *       field of view reproducing the unconvolved radiance.
+ *   formod_fov: no reference fixture uses a FOV; pinned by closed forms
+ *       (tests/test_oracle_fov_sun.py): a delta-shaped FOV reproduces the
+ *       unconvolved radiance, radiances linear in tangent height give the
+ *       value at the weight-averaged offset (inside and beyond the profile,
+ *       un-normalised weights), other shapes the weighted mean of numpy's
+ *       linear interpolation of the rays with the same time only.
+ *   srcfunc_sca_sun by day: no reference fixture is by day; pinned in a
+ *       transparent atmosphere with a phase function linear in the
+ *       scattering angle: J = 6.764e-5 P(theta) B(5780 K) for LOS tangents at
+ *       known angles from the sun direction (built from suncoord's zenith
+ *       and azimuth in a textbook east-north-up frame, the same Earth-fixed
+ *       vector at every point), 0 when the sun ray meets the ground, the
+ *       full term when it grazes past the horizon below 90 deg + dip.
  *   Scattering depth >= 2 and fan sizes other than 28: parity unpinned by
  *       fixtures (no reference test uses SCA_MULT >= 2); checked by the
  *       closure of an isothermal opaque atmosphere (the source J equals the
@@ -156,6 +167,14 @@ void oracle_cov_meas(const ret_t *ret, ctl_t *ctl, obs_t *obs_meas, double *sig_
 void oracle_optimal_estimation(const ret_t *ret, ctl_t *ctl, obs_t *obs_meas, obs_t *obs_i, atm_t *atm_apr,
                                atm_t *atm_i, aero_t *aero_apr, aero_t *aero_i, double *chisq, int *nchisq,
                                double *err_total);
+
+/* The solar term of srcfunc_sca alone (srcfunc_sca_sun, scatter.c:899-995),
+   added to src_sca[nd]. */
+void oracle_srcfunc_sca_sun(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double sec, const double *x,
+                            const double *dx, int il, double *src_sca);
+/* The FOV convolution of obs's radiances and transmittances alone
+   (formod_fov, forwardmodel.c:156-220; FOV shape file ctl->fov). */
+void oracle_formod_fov(ctl_t *ctl, obs_t *obs);
 
 /* Secondary pencils per scattering point (reference: 28); even, 10-28. */
 void oracle_set_fan(int nalpha);

@@ -57,6 +57,8 @@ def load():
         "oracle_cov_apr": (None, [P, P, P, P, D]),
         "oracle_cov_meas": (None, [P, P, P, D, D, D]),
         "oracle_optimal_estimation": (None, [P, P, P, P, P, P, P, P, D, C.POINTER(C.c_int), D]),
+        "oracle_srcfunc_sca_sun": (None, [P, P, P, C.c_double, D, D, C.c_int, D]),
+        "oracle_formod_fov": (None, [P, P]),
         "oracle_reset": (None, []),
     }
     for name, (res, args) in sig.items():
