@@ -130,8 +130,11 @@ void optimal_estimation(ret_t *ret, ctl_t *ctl, obs_t *obs_meas, obs_t *obs_i, a
    q [natm][ng][nlev]; k [natm][nw][nlev] or NULL for zero extinction.
    atm_of_ray [nray] selects a profile per ray; NULL means ray i uses profile
    i when natm == nray, else profile 0.  Outputs: rad, tau [nd][nray]
-   (channel-major like obs_t, FP32); tpz/tplon/tplat [nray] and npts [nray]
-   (LOS points) are optional.  Vertical profiles only: ctl->ip must be 1. */
+   (channel-major like obs_t, FP32) with leading dimension out_ld (0: nray;
+   otherwise >= nray, so that consecutive calls can fill column blocks of
+   one larger array through rad + offset); tpz/tplon/tplat [nray] and npts
+   [nray] (LOS points) are optional.  Vertical profiles only: ctl->ip must
+   be 1. */
 typedef struct {
   int nray, nlev, natm;
   const int *atm_of_ray;
@@ -140,6 +143,7 @@ typedef struct {
   float *rad, *tau;
   double *tpz, *tplon, *tplat;
   int *npts;
+  int out_ld;
 } jur_batch_t;
 
 /* All pointers of b are host pointers.  aero may be NULL when ctl->sca_n is 0;
@@ -181,8 +185,10 @@ int jur_init(int device);
    reference's 28), "kernel_rays" (rays per batched call of kernel/jur_kernel, default
    131072: the perturbed atmospheres are grouped into calls of about this
    many rays), "fp64" (1: path integration in FP64 arithmetic, the
-   reference's operation order, with FP64 radiances for formod; default 0:
-   FP32 with a compensated radiance sum), "kernel_fp64" (default 1: kernel
+   reference's operation order, on FP64 tables grids, Planck table,
+   continuum data and layer optics, with FP64 radiances for formod; the
+   scattering source J of scattering calls is still formed in FP32 by the
+   angular integral; default 0: FP32 with a compensated radiance sum), "kernel_fp64" (default 1: kernel
    and jur_kernel integrate the perturbed paths in FP64, since a 1 %
    perturbation of one level can change a radiance by less than FP32
    resolves; optimal_estimation's forward calls follow the same setting),
@@ -209,8 +215,9 @@ int jur_set_option(const char *key, double value);
    kernel call), "kernel_columns", "oe_formod_ms", "oe_kernel_ms",
    "oe_total_ms" (wall time of the last optimal_estimation call: its
    forward models, its Jacobians, in total); running totals over all calls since
-   the library loaded: "calls_total", "path_ms_total",
-   "integrated_points_total", "pencils_total".  Unknown keys give NaN. */
+   the library loaded: "calls_total", "path_ms_total" (a replayed graph
+   adds its whole time), "integrated_points_total", "pencils_total".
+   Unknown keys give NaN. */
 double jur_stat(const char *key);
 
 /* ---- Scattering-source surrogate (no reference counterpart) ---------- */

@@ -31,6 +31,7 @@ struct Engine {
   unsigned ue_bytes = 0, meta_bytes = 0;
   // aerosol layers
   DBuf beta_e, beta_s, beta_a, alti;
+  DBuf beta_e64, beta_s64, beta_a64;  // FP64 copies for the FP64 path
   int nalti = 0;
   double altimax = 0, altimin = 0;
   int nl = 0;
@@ -216,11 +217,9 @@ double engine_stat(const char* key) {
 }
 void engine_set_stat(const char* key, double v) { E().stats[key] = v; }
 
-void engine_prepare(const ctl_t* ctl) {
-  Engine& e = E();
-  hipStream_t s = e.stream;
-  // Tables and source-function tables: loaded once per table set (the
-  // reference loads them at its first call, forwardmodel.c:286-292).
+// Keys of the table set (table base, emitters, wavenumbers) and of the
+// channel constants (plus the continua switches and windows) of ctl.
+static std::string table_key(const ctl_t* ctl) {
   std::string key = std::string(ctl->tblbase) + "|";
   for (int ig = 0; ig < ctl->ng; ig++) key += std::string(ctl->emitter[ig]) + ",";
   char buf[64];
@@ -228,6 +227,23 @@ void engine_prepare(const ctl_t* ctl) {
     snprintf(buf, sizeof buf, "%.17g,", ctl->nu[id]);
     key += buf;
   }
+  return key;
+}
+static std::string channel_key(const ctl_t* ctl, const std::string& key) {
+  std::string ck = key;
+  char buf[64];
+  snprintf(buf, sizeof buf, "|%d%d%d%d", ctl->ctm_co2, ctl->ctm_h2o, ctl->ctm_n2, ctl->ctm_o2);
+  ck += buf;
+  for (int id = 0; id < ctl->nd; id++) ck += std::to_string(ctl->window[id]) + ",";
+  return ck;
+}
+
+void engine_prepare(const ctl_t* ctl) {
+  Engine& e = E();
+  hipStream_t s = e.stream;
+  // Tables and source-function tables: loaded once per table set (the
+  // reference loads them at its first call, forwardmodel.c:286-292).
+  const std::string key = table_key(ctl);
   if (key != e.tbl_key) {
     const int ng = ctl->ng, nd = ctl->nd, ntbl = ng * nd;
     std::vector<int> np(ntbl, -1), nt((size_t)ntbl * TBLNPMAX, 0), ncol((size_t)ntbl * TBLNPMAX * TBLNTMAX, 0),
@@ -355,10 +371,7 @@ void engine_prepare(const ctl_t* ctl) {
     printf("Initialize source function table...\n");
   }
   // Channel constants (depend on the continua switches and windows too).
-  std::string ck = key;
-  snprintf(buf, sizeof buf, "|%d%d%d%d", ctl->ctm_co2, ctl->ctm_h2o, ctl->ctm_n2, ctl->ctm_o2);
-  ck += buf;
-  for (int id = 0; id < ctl->nd; id++) ck += std::to_string(ctl->window[id]) + ",";
+  const std::string ck = channel_key(ctl, key);
   if (ck != e.chan_key) {
     std::vector<ChanConst> cc(ctl->nd);
     std::vector<ChanConst64> cc64(ctl->nd);
@@ -376,15 +389,16 @@ void engine_prepare(const ctl_t* ctl) {
   }
 }
 
+namespace {
+void drop_forward_graph();
+}
+
 // Aerosol layers for the ray tracer (layer-edge altitudes of
 // add_aerosol_layers, lineofsight.c:211-244) and layer optics for the path
 // integration.
-void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
-  Engine& e = E();
-  hipStream_t s = e.stream;
+// Every input engine_set_aero reads, as bytes.
+static std::vector<char> aero_snapshot(const ctl_t* ctl, const aero_t* aero) {
   const int nd = ctl->nd;
-  // Layer optics and phase functions are uploaded only when they change:
-  // the snapshot holds every input this function reads.
   std::vector<char> snap;
   auto add = [&snap](const void* p, size_t n) {
     snap.insert(snap.end(), (const char*)p, (const char*)p + n);
@@ -407,10 +421,23 @@ void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
       for (int id = 0; id < nd; id++) add(aero->p[il][id], sizeof(double) * NTHETA);
     }
   }
+  return snap;
+}
+
+void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
+  Engine& e = E();
+  hipStream_t s = e.stream;
+  const int nd = ctl->nd;
+  // Layer optics and phase functions are uploaded only when they change:
+  // the snapshot holds every input this function reads.
+  std::vector<char> snap = aero_snapshot(ctl, aero);
   if (e.aero_ready && snap == e.aero_snap) return;
+  // a recorded forward call holds the old layer geometry and buffers
+  drop_forward_graph();
   e.aero_snap.swap(snap);
   e.aero_ready = true;
   std::vector<float> be((size_t)NLMAX * nd, 0.f), bs((size_t)NLMAX * nd, 0.f), ba((size_t)NLMAX * nd, 0.f);
+  std::vector<double> be64((size_t)NLMAX * nd, 0.0), bs64((size_t)NLMAX * nd, 0.0), ba64((size_t)NLMAX * nd, 0.0);
   std::vector<double> alti;
   e.nl = 0;
   if (aero && ctl->sca_n > 0) {
@@ -424,6 +451,9 @@ void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
         be[(size_t)il * nd + id] = (float)aero->beta_e[il][id];
         bs[(size_t)il * nd + id] = (float)aero->beta_s[il][id];
         ba[(size_t)il * nd + id] = (float)aero->beta_a[il][id];
+        be64[(size_t)il * nd + id] = aero->beta_e[il][id];
+        bs64[(size_t)il * nd + id] = aero->beta_s[il][id];
+        ba64[(size_t)il * nd + id] = aero->beta_a[il][id];
       }
       alti.push_back(aero->top[il] + eps);
       alti.push_back(aero->top[il] - eps);
@@ -458,74 +488,81 @@ void engine_set_aero(const ctl_t* ctl, const aero_t* aero) {
   upload(e.beta_e, be, s);
   upload(e.beta_s, bs, s);
   upload(e.beta_a, ba, s);
+  upload(e.beta_e64, be64, s);
+  upload(e.beta_s64, bs64, s);
+  upload(e.beta_a64, ba64, s);
   upload(e.alti, alti, s);
   scatter_set_phase(ctl, aero, s);
   HIP_OK(hipStreamSynchronize(s));
 }
 
 namespace {
+// Trace arguments of a ray set (everything but the buffers).
+TraceArgs trace_args(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on, int cap) {
+  Engine& e = E();
+  TraceArgs A;
+  memset(&A, 0, sizeof A);
+  A.nray = rays.nray;
+  A.obsz = rays.obsz;
+  A.obslon = rays.obslon;
+  A.obslat = rays.obslat;
+  A.vpz = rays.vpz;
+  A.vplon = rays.vplon;
+  A.vplat = rays.vplat;
+  A.atm_idx = rays.atm_idx;
+  A.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw, atm.geo};
+  A.rayds = ctl->rayds;
+  A.raydz = ctl->raydz;
+  A.transs = ctl->transs;
+  A.refrac = ctl->refrac;
+  A.sca_n = sca_on ? ctl->sca_n : 0;
+  A.alti = e.alti.as<double>();
+  A.nalti = e.nalti;
+  A.altimax = e.altimax;
+  A.altimin = e.altimin;
+  A.nl = e.nl;
+  for (int il = 0; il < NLMAX; il++) {
+    A.top[il] = e.top[il];
+    A.bottom[il] = e.bottom[il];
+    A.trans[il] = e.trans[il];
+  }
+  A.nw = ctl->nw;
+  A.ig_h2o = find_emitter(ctl, "H2O");
+  A.R = rec_len(ctl->ng, ctl->nw);
+  A.cap = cap;
+  return A;
+}
+
 // Enqueue one ray trace of a ray set with per-ray capacity cap (status
 // flags cleared first); no host synchronisation.
 void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on, TraceBufs& tb,
                    int cap, double* tpz, double* tplon, double* tplat, hipStream_t s) {
   Engine& e = E();
-  const int R = rec_len(ctl->ng, ctl->nw);
-  const int ig_h2o = find_emitter(ctl, "H2O");
-  {
-    const size_t npt = (size_t)rays.nray * cap;
-    TraceArgs A;
-    memset(&A, 0, sizeof A);
-    A.nray = rays.nray;
-    A.obsz = rays.obsz;
-    A.obslon = rays.obslon;
-    A.obslat = rays.obslat;
-    A.vpz = rays.vpz;
-    A.vplon = rays.vplon;
-    A.vplat = rays.vplat;
-    A.atm_idx = rays.atm_idx;
-    A.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw, atm.geo};
-    A.rayds = ctl->rayds;
-    A.raydz = ctl->raydz;
-    A.transs = ctl->transs;
-    A.refrac = ctl->refrac;
-    A.sca_n = sca_on ? ctl->sca_n : 0;
-    A.alti = e.alti.as<double>();
-    A.nalti = e.nalti;
-    A.altimax = e.altimax;
-    A.altimin = e.altimin;
-    A.nl = e.nl;
-    for (int il = 0; il < NLMAX; il++) {
-      A.top[il] = e.top[il];
-      A.bottom[il] = e.bottom[il];
-      A.trans[il] = e.trans[il];
-    }
-    A.nw = ctl->nw;
-    A.ig_h2o = ig_h2o;
-    A.R = R;
-    A.cap = cap;
-    A.geo = (double4*)tb.geo->need(npt * sizeof(double4));
-    A.geo2 = A.sca_n > 0 ? (double4*)tb.geo2->need(npt * sizeof(double4)) : nullptr;
-    A.rec = (float*)tb.rec->need(npt * R * sizeof(float));
-    A.np_out = (int*)tb.npts->need(rays.nray * sizeof(int));
-    A.tsurf_out = (float*)tb.tsurf->need(rays.nray * sizeof(float));
-    A.tsurf64 = tb.tsurf64 ? (double*)tb.tsurf64->need(rays.nray * sizeof(double)) : nullptr;
-    A.geo_final = tb.geo_final ? (int*)tb.geo_final->need(rays.nray * sizeof(int)) : nullptr;
-    A.tpz = tpz;
-    A.tplon = tplon;
-    A.tplat = tplat;
-    A.rec64 = nullptr;
-    if (engine_option_int("fp64", 0)) {
-      A.rec64 = (double*)e.rec64.need(npt * R * sizeof(double));
-      if (!A.tsurf64) A.tsurf64 = (double*)e.tsurf64.need(rays.nray * sizeof(double));
-    }
-    e.last_rec64 = A.rec64;
-    e.last_tsurf64 = A.tsurf64;
-    e.last_nray = rays.nray;
-    e.last_cap = cap;
-    A.status = (int*)e.status.need(sizeof(int) * 4);
-    launch_zero_int(A.status, 1, s);
-    launch_raytrace(A, s);
+  TraceArgs A = trace_args(ctl, atm, rays, sca_on, cap);
+  const int R = A.R;
+  const size_t npt = (size_t)rays.nray * cap;
+  A.geo = (double4*)tb.geo->need(npt * sizeof(double4));
+  A.geo2 = A.sca_n > 0 ? (double4*)tb.geo2->need(npt * sizeof(double4)) : nullptr;
+  A.rec = (float*)tb.rec->need(npt * R * sizeof(float));
+  A.np_out = (int*)tb.npts->need(rays.nray * sizeof(int));
+  A.tsurf_out = (float*)tb.tsurf->need(rays.nray * sizeof(float));
+  A.tsurf64 = tb.tsurf64 ? (double*)tb.tsurf64->need(rays.nray * sizeof(double)) : nullptr;
+  A.geo_final = tb.geo_final ? (int*)tb.geo_final->need(rays.nray * sizeof(int)) : nullptr;
+  A.tpz = tpz;
+  A.tplon = tplon;
+  A.tplat = tplat;
+  A.rec64 = nullptr;
+  if (engine_option_int("fp64", 0)) {
+    A.rec64 = (double*)e.rec64.need(npt * R * sizeof(double));
+    if (!A.tsurf64) A.tsurf64 = (double*)e.tsurf64.need(rays.nray * sizeof(double));
   }
+  e.last_rec64 = A.rec64;
+  e.last_tsurf64 = A.tsurf64;
+  e.last_nray = rays.nray;
+  e.last_cap = cap;
+  A.status = (int*)e.status.need(sizeof(int) * 4);
+  launch_zero_int(A.status, 1, s);
+  launch_raytrace(A, s);
 }
 }  // namespace
 
@@ -584,21 +621,37 @@ void end_call(hipStream_t s) {
 }  // namespace
 
 namespace {
-// Every address and size a captured clear-air call depends on.
+// Every address, size and setting a captured clear-air call depends on:
+// inputs, outputs, work buffers, the tables and layer optics it reads, the
+// ray-tracing keys, the layer geometry copied into the trace arguments and
+// the kernel options.  A change of the aerosol layers also drops the graph
+// (engine_set_aero).
 std::string graph_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out,
                       int scattering, hipStream_t s) {
   Engine& e = E();
-  char k[1024];
-  snprintf(k, sizeof k, "%s|%s|%d %d %d %d|%p %p %p %p %p|%p %p %p %p %p %p %p %p|%p %p %p %p %p %p|"
-           "%p %p %p %p %p %p %p %p|%p|%d %g %g",
-           e.tbl_key.c_str(), e.chan_key.c_str(), rays.nray, atm.natm, atm.nlev, scattering,
-           (const void*)atm.z, (const void*)atm.p, (const void*)atm.t, (const void*)atm.q, (const void*)atm.k,
-           (const void*)rays.time, (const void*)rays.obsz, (const void*)rays.obslon, (const void*)rays.obslat,
-           (const void*)rays.vpz, (const void*)rays.vplon, (const void*)rays.vplat, (const void*)rays.atm_idx,
-           (void*)out.rad, (void*)out.tau, (void*)out.tpz, (void*)out.tplon, (void*)out.tplat, (void*)out.npts,
-           e.geo.p, e.rec.p, e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, (void*)s,
-           ctl->sca_n, ctl->rayds, ctl->raydz);
-  return k;
+  char k[2048];
+  int n = snprintf(k, sizeof k, "%s|%s|%d %d %d %d|%p %p %p %p %p|%p %p %p %p %p %p %p %p|%p %p %p %p %p %p|"
+                   "%p %p %p %p %p %p %p %p %p|%p|%d %.17g %.17g %.17g %d %d|%s|%d %d %d|",
+                   e.tbl_key.c_str(), e.chan_key.c_str(), rays.nray, atm.natm, atm.nlev, scattering,
+                   (const void*)atm.z, (const void*)atm.p, (const void*)atm.t, (const void*)atm.q,
+                   (const void*)atm.k, (const void*)rays.time, (const void*)rays.obsz, (const void*)rays.obslon,
+                   (const void*)rays.obslat, (const void*)rays.vpz, (const void*)rays.vplon,
+                   (const void*)rays.vplat, (const void*)rays.atm_idx, (void*)out.rad, (void*)out.tau,
+                   (void*)out.tpz, (void*)out.tplon, (void*)out.tplat, (void*)out.npts, e.geo.p, e.geo2.p, e.rec.p,
+                   e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, (void*)s, ctl->sca_n,
+                   ctl->rayds, ctl->raydz, ctl->transs, ctl->refrac, ctl->nw + 1000 * out.ld, ctl->sca_ext,
+                   engine_option_int("fast", 1), engine_option_int("variant", -1), engine_option_int("wpb", -1));
+  std::string key(k, n < (int)sizeof k ? n : (int)sizeof k - 1);
+  // layer geometry and optics (addresses; their contents are in the
+  // snapshot, whose change drops the graph)
+  snprintf(k, sizeof k, "%p %d %.17g %.17g %d|%p %p %p|%p %p %p %p", e.alti.p, e.nalti, e.altimax, e.altimin, e.nl,
+           e.beta_e.p, e.beta_s.p, e.beta_a.p, e.chan.p, e.planck.p, e.t_ue.p, e.t_meta.p);
+  key += k;
+  for (int il = 0; il < e.nl && il < NLMAX; il++) {
+    snprintf(k, sizeof k, "|%.17g %.17g %.17g", e.top[il], e.bottom[il], e.trans[il]);
+    key += k;
+  }
+  return key;
 }
 
 void drop_forward_graph() {
@@ -623,7 +676,7 @@ void capture_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
   launch_zero_u64((unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
   enqueue_trace(ctl, atm, rays, ctl->sca_n > 0, tb, cap, out.tpz, out.tplon, out.tplat, s);
   engine_integrate(ctl, rays.nray, cap, e.rec.as<float>(), e.npts.as<int>(), e.tsurf.as<float>(), 0, nullptr,
-                   nullptr, out.rad, out.tau, rays.nray, 0, s);
+                   nullptr, out.rad, out.tau, out.ld > 0 ? out.ld : rays.nray, 0, s);
   if (out.npts) HIP_OK(hipMemcpyAsync(out.npts, e.npts.p, rays.nray * sizeof(int), hipMemcpyDeviceToDevice, s));
   HIP_OK(hipStreamEndCapture(s, &e.g_graph));
   e.capturing = false;
@@ -663,6 +716,14 @@ bool replay_forward(const RaySetDev& rays, hipStream_t s) {
   e.stats["path_launches"] = 1;
   e.stats["integrated_points"] = (double)pts;
   e.stats["pencils"] = rays.nray;
+  e.stats["leaf_pencils"] = rays.nray;  // recorded calls are clear air / extinction (depth 0)
+  e.stats["scattering_points"] = 0;
+  // running totals: the replay's time is the whole call (no separate
+  // path-kernel timing inside a graph)
+  e.stats["calls_total"] += 1;
+  e.stats["path_ms_total"] += ms;
+  e.stats["integrated_points_total"] += (double)pts;
+  e.stats["pencils_total"] += rays.nray;
   return true;
 }
 }  // namespace
@@ -693,7 +754,7 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
   int cap = engine_trace(ctl, atm, rays, sca_on, tb, out.tpz, out.tplon, out.tplat, s);
   HIP_OK(hipEventRecord(ev[1], s));
   engine_integrate(ctl, rays.nray, cap, e.rec.as<float>(), e.npts.as<int>(), e.tsurf.as<float>(), 0, nullptr,
-                   nullptr, out.rad, out.tau, rays.nray, 0, s, out.rad64, out.tau64);
+                   nullptr, out.rad, out.tau, out.ld > 0 ? out.ld : rays.nray, 0, s, out.rad64, out.tau64);
   HIP_OK(hipEventRecord(ev[2], s));
   if (out.npts) HIP_OK(hipMemcpyAsync(out.npts, e.npts.p, rays.nray * sizeof(int), hipMemcpyDeviceToDevice, s));
   HIP_OK(hipEventSynchronize(ev[2]));
@@ -709,6 +770,66 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
   for (auto& x : ev) HIP_OK(hipEventDestroy(x));
   if (graph) capture_forward(ctl, atm, rays, out, scattering, cap, s);
 }
+
+namespace {
+// Autotuner key of a call shape: emitters, channels, size class (cls),
+// table layout.
+std::string tune_key(const ctl_t* ctl, int nray, int shared, int* cls) {
+  *cls = nray >= (1 << 16) ? 3 : nray >= (1 << 14) ? 2 : nray >= 4096 ? 1 : 0;
+  char key[96];
+  snprintf(key, sizeof key, "tune:%d:%d:%d:%d", ctl->ng, ctl->nd, *cls, shared);
+  return key;
+}
+
+// Arguments of the path integration over traced rays (kernel variant left
+// to the caller).
+PathArgs path_args(const ctl_t* ctl, int nray, int cap, const float* rec, const int* npts, const float* tsurf,
+                   const int* order, int scattering, const float* J, const int* jbase, float* rad, float* tau,
+                   int out_ld, int out_off, double* rad64, double* tau64) {
+  Engine& e = E();
+  PathArgs P;
+  memset(&P, 0, sizeof P);
+  P.nray = nray;
+  P.nd = ctl->nd;
+  P.ng = ctl->ng;
+  P.nw = ctl->nw;
+  P.R = rec_len(ctl->ng, ctl->nw);
+  P.cap = cap;
+  P.rec = rec;
+  P.np = npts;
+  P.tsurf = tsurf;
+  P.order = order;
+  P.T = engine_tables();
+  if (engine_option_int("fast", 1) == 0) P.T.shared = 0;
+  P.cc = e.chan.as<ChanConst>();
+  P.planck = e.planck.as<float>();
+  P.cc64 = e.chan64.as<ChanConst64>();
+  P.planck64 = e.planck64.as<double>();
+  P.beta_e = e.beta_e.as<float>();
+  P.beta_s = e.beta_s.as<float>();
+  P.beta_a = e.beta_a.as<float>();
+  P.beta_e64 = e.beta_e64.as<double>();
+  P.beta_s64 = e.beta_s64.as<double>();
+  P.beta_a64 = e.beta_a64.as<double>();
+  P.sca_n = ctl->sca_n;
+  P.ext_beta_e = strcmp(ctl->sca_ext, "beta_a") != 0;
+  P.scattering = scattering;
+  P.J = J;
+  P.jbase = jbase;
+  P.rad = rad;
+  P.tau = tau;
+  P.out_ld = out_ld;
+  P.out_off = out_off;
+  P.fp64 = engine_option_int("fp64", 0) != 0;
+  P.fp64_split = engine_option_int("fp64_split", -1);
+  P.rad64 = P.fp64 ? rad64 : nullptr;
+  P.tau64 = P.fp64 ? tau64 : nullptr;
+  // the FP64 records belong to the trace just made of these rays
+  P.rec64 = (P.fp64 && e.last_nray == nray && e.last_cap == cap) ? e.last_rec64 : nullptr;
+  P.tsurf64 = P.rec64 ? e.last_tsurf64 : nullptr;
+  return P;
+}
+}  // namespace
 
 // Sort rays by point count and run the path integration.
 void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, const int* npts,
@@ -742,43 +863,8 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     // by the points each ray integrates (all, or those after its prefix)
     launch_sort_rays(prefix ? prem : npts, nullptr, keys, ids, order, tmp, tb, nray, s);
   }
-  PathArgs P;
-  memset(&P, 0, sizeof P);
-  P.nray = nray;
-  P.nd = ctl->nd;
-  P.ng = ctl->ng;
-  P.nw = ctl->nw;
-  P.R = rec_len(ctl->ng, ctl->nw);
-  P.cap = cap;
-  P.rec = rec;
-  P.np = npts;
-  P.tsurf = tsurf;
-  P.order = order;
-  P.T = engine_tables();
-  if (engine_option_int("fast", 1) == 0) P.T.shared = 0;
-  P.cc = e.chan.as<ChanConst>();
-  P.planck = e.planck.as<float>();
-  P.cc64 = e.chan64.as<ChanConst64>();
-  P.planck64 = e.planck64.as<double>();
-  P.beta_e = e.beta_e.as<float>();
-  P.beta_s = e.beta_s.as<float>();
-  P.beta_a = e.beta_a.as<float>();
-  P.sca_n = ctl->sca_n;
-  P.ext_beta_e = strcmp(ctl->sca_ext, "beta_a") != 0;
-  P.scattering = scattering;
-  P.J = J;
-  P.jbase = jbase;
-  P.rad = rad;
-  P.tau = tau;
-  P.out_ld = out_ld;
-  P.out_off = out_off;
-  P.fp64 = engine_option_int("fp64", 0) != 0;
-  P.fp64_split = engine_option_int("fp64_split", -1);
-  P.rad64 = P.fp64 ? rad64 : nullptr;
-  P.tau64 = P.fp64 ? tau64 : nullptr;
-  // the FP64 records belong to the trace just made of these rays
-  P.rec64 = (P.fp64 && e.last_nray == nray && e.last_cap == cap) ? e.last_rec64 : nullptr;
-  P.tsurf64 = P.rec64 ? e.last_tsurf64 : nullptr;
+  PathArgs P = path_args(ctl, nray, cap, rec, npts, tsurf, order, scattering, J, jbase, rad, tau, out_ld, out_off,
+                         rad64, tau64);
   int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
   if (P.fp64) {
     // the FP64 kernels have one configuration (variant and waves per block
@@ -791,9 +877,8 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     // shape times each kernel variant (channels per wave x occupancy) and
     // waves per block once, with HIP events on this stream, and keeps the
     // fastest.  Small batches use the default.
-    char key[96];
-    int cls = nray >= (1 << 16) ? 3 : nray >= (1 << 14) ? 2 : nray >= 4096 ? 1 : 0;
-    snprintf(key, sizeof key, "tune:%d:%d:%d:%d", ctl->ng, ctl->nd, cls, P.T.shared);
+    int cls = 0;
+    const std::string key = tune_key(ctl, nray, P.T.shared, &cls);
     auto it = e.opt.find(key);
     int best = 2 * 8 + 4;  // variant 2, 4 waves per block
     if (it != e.opt.end()) {

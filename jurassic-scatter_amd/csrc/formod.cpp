@@ -2,6 +2,7 @@
 // formod_pencil / raytrace / hydrostatic on the GPU, and the batched
 // extension.  Host code here only stages data and orders the device work.
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <strings.h>
 
@@ -380,11 +381,15 @@ extern "C" double planck(double t, double nu) {
 
 // ---- batched extension ----------------------------------------------------
 
+// The layout the Python binding (pyjurassic/lib.py) mirrors.
+static_assert(offsetof(jur_batch_t, out_ld) == 168 && sizeof(jur_batch_t) == 176, "jur_batch_t layout");
+
 static void check_batch(const ctl_t* ctl, const jur_batch_t* b) {
   if (!b || b->nray < 0 || b->nlev < 2 || b->natm < 1) JUR_FAIL("Invalid batch description!");
   if (!b->z || !b->p || !b->t || (!b->q && ctl->ng > 0) || !b->obsz || !b->vpz || !b->rad || !b->tau)
     JUR_FAIL("Missing batch array!");
   if (b->nlev > NPMAX) JUR_FAIL("Too many data points!");
+  if (b->out_ld != 0 && b->out_ld < b->nray) JUR_FAIL("Invalid batch description!");
   if (ctl->ip != 1) JUR_FAIL("The batched entry point takes vertical profiles (IP=1)!");
 }
 
@@ -406,7 +411,7 @@ extern "C" int jur_formod_batch_device(ctl_t* ctl, const jur_batch_t* b, const a
   if (!A.k && ctl->nw > 0) {
     size_t n = (size_t)b->natm * ctl->nw * b->nlev;
     double* kz = (double*)kzero.need(n * sizeof(double) + 8);
-    HIP_OK(hipMemsetAsync(kz, 0, n * sizeof(double), s));
+    launch_fill_f64(kz, n, 0.0, s);  // a kernel, not a memset node, should the caller capture this call
     A.k = kz;
   }
   RaySetDev R;
@@ -427,6 +432,7 @@ extern "C" int jur_formod_batch_device(ctl_t* ctl, const jur_batch_t* b, const a
   OutDev O;
   O.rad = b->rad;
   O.tau = b->tau;
+  O.ld = b->out_ld;
   static DBuf tz, tlo, tla;
   O.tpz = b->tpz ? b->tpz : (double*)tz.need((size_t)b->nray * sizeof(double) + 8);
   O.tplon = b->tplon ? b->tplon : (double*)tlo.need((size_t)b->nray * sizeof(double) + 8);
@@ -462,14 +468,23 @@ extern "C" int jur_formod_batch(ctl_t* ctl, const jur_batch_t* b, const aero_t* 
   OutDev O = stage_out(ctl, (int)nr);
   d.rad = O.rad;
   d.tau = O.tau;
+  d.out_ld = 0;  // dense on the device; the copies back honour b->out_ld
   d.tpz = O.tpz;
   d.tplon = O.tplon;
   d.tplat = O.tplat;
   d.npts = O.npts;
   HIP_OK(hipStreamSynchronize(s));
   jur_formod_batch_device(ctl, &d, aero, scattering, s);
-  HIP_OK(hipMemcpyAsync(b->rad, d.rad, nd * nr * sizeof(float), hipMemcpyDeviceToHost, s));
-  HIP_OK(hipMemcpyAsync(b->tau, d.tau, nd * nr * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (b->out_ld > 0 && (size_t)b->out_ld != nr && nr > 0) {
+    const size_t ld = b->out_ld;
+    HIP_OK(hipMemcpy2DAsync(b->rad, ld * sizeof(float), d.rad, nr * sizeof(float), nr * sizeof(float), nd,
+                            hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpy2DAsync(b->tau, ld * sizeof(float), d.tau, nr * sizeof(float), nr * sizeof(float), nd,
+                            hipMemcpyDeviceToHost, s));
+  } else {
+    HIP_OK(hipMemcpyAsync(b->rad, d.rad, nd * nr * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(b->tau, d.tau, nd * nr * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
   if (b->tpz) HIP_OK(hipMemcpyAsync(b->tpz, d.tpz, nr * sizeof(double), hipMemcpyDeviceToHost, s));
   if (b->tplon) HIP_OK(hipMemcpyAsync(b->tplon, d.tplon, nr * sizeof(double), hipMemcpyDeviceToHost, s));
   if (b->tplat) HIP_OK(hipMemcpyAsync(b->tplat, d.tplat, nr * sizeof(double), hipMemcpyDeviceToHost, s));

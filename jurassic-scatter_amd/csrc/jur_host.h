@@ -83,11 +83,12 @@ struct AtmSetDev {
 // profiles that cannot be identified.
 DevGeo stage_geo(const ctl_t* ctl, const atm_t* atm, hipStream_t s);
 struct OutDev {
-  float* rad = nullptr;   // [nd][nray]
-  float* tau = nullptr;   // [nd][nray]
+  float* rad = nullptr;   // [nd][ld]: ray r at column r
+  float* tau = nullptr;   // [nd][ld]
+  int ld = 0;             // leading dimension of rad / tau / rad64 / tau64 (0: the ray count)
   double *tpz = nullptr, *tplon = nullptr, *tplat = nullptr;  // [nray]
   int* npts = nullptr;    // [nray] (optional)
-  // FP64 radiance / transmittance [nd][nray] (optional): with option "fp64"
+  // FP64 radiance / transmittance [nd][ld] (optional): with option "fp64"
   // the path integration runs in FP64 and also writes these
   double *rad64 = nullptr, *tau64 = nullptr;
 };

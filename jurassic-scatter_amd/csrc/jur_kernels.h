@@ -50,6 +50,7 @@ struct PathArgs {
   const ChanConst* cc;     // [nd]
   const float* planck;     // [nd][JUR_NPLANCK]
   const float *beta_e, *beta_s, *beta_a;  // [NLMAX][nd] layer optics
+  const double *beta_e64, *beta_s64, *beta_a64;  // the same in FP64 (FP64 kernels)
   int sca_n;
   int ext_beta_e;          // leaf extinction: 1 = beta_e, 0 = beta_a
   int scattering;          // > 0: points with aerofac > 0 use the source J
@@ -95,6 +96,7 @@ void launch_iota(int* ids, int n, hipStream_t s);
 void launch_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,
                          int* rem, hipStream_t s);
 void launch_zero_int(int* p, int n, hipStream_t s);
+void launch_fill_f64(double* p, size_t n, double v, hipStream_t s);
 // Rays by descending point count into order (hist: nbins ints of scratch).
 void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, hipStream_t s);
 void launch_sum_points(const int* np, int n, unsigned long long* acc, hipStream_t s);

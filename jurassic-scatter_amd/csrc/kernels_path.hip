@@ -806,7 +806,7 @@ __device__ __forceinline__ void point64(const PathArgs& A, const ChanConst64& cc
     // scattering source (forwardmodel.c:317-360); a zero-length segment
     // adds nothing (the reference forms -log(1)/0 there)
     if (tau_gas > 0.0 && ds > 0.0) {
-      const double be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
+      const double be = A.beta_e64[ai * A.nd + id], bs = A.beta_s64[ai * A.nd + id];
       const double J = A.J[(size_t)kj * A.nd + id];
       const double bt = -log(tau_gas) / ds + beta + af * be;
       const double eps = 1.0 - exp(-bt * ds);
@@ -818,7 +818,7 @@ __device__ __forceinline__ void point64(const PathArgs& A, const ChanConst64& cc
   } else if (tau_gas > 0.0) {
     // gas, continua and aerosol extinction (forwardmodel.c:363-390)
     double bx = beta;
-    if (A.sca_n > 0 && af != 0.0) bx += af * (A.ext_beta_e ? A.beta_e[ai * A.nd + id] : A.beta_a[ai * A.nd + id]);
+    if (A.sca_n > 0 && af != 0.0) bx += af * (A.ext_beta_e ? A.beta_e64[ai * A.nd + id] : A.beta_a64[ai * A.nd + id]);
     const double eps = 1.0 - tau_gas * exp(-bx * ds);
     rad += B * eps * tau;
     tau *= 1.0 - eps;
@@ -1153,6 +1153,14 @@ void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, h
 
 void launch_zero_int(int* p, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+__global__ void k_fill_f64(double* p, size_t n, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void launch_fill_f64(double* p, size_t n, double v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
 
 void launch_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,

@@ -153,17 +153,18 @@ __global__ void k_records(TraceArgs A) {
   double af64 = 0.0;
   if (ins) {
     double xa[3], xb[3], xc[3];
+    // the neighbours' positions only: each thread rewrites its own point's
+    // segment length (.w) below
+    auto pos = [F](int i, double* x) { geo2cart(F[i].x, F[i].y, F[i].z, x); };
     if (nf >= 2) {
       if (ip == 0 || ip == nf - 1) {
-        const double4 o = F[ip == 0 ? 1 : nf - 2];
         geo2cart(g.x, g.y, g.z, xa);
-        geo2cart(o.x, o.y, o.z, xb);
+        pos(ip == 0 ? 1 : nf - 2, xb);
         ds = 0.5 * dist3(xa, xb);
       } else {
-        const double4 p0 = F[ip - 1], p2 = F[ip + 1];
-        geo2cart(p0.x, p0.y, p0.z, xa);
+        pos(ip - 1, xa);
         geo2cart(g.x, g.y, g.z, xb);
-        geo2cart(p2.x, p2.y, p2.z, xc);
+        pos(ip + 1, xc);
         ds = 0.5 * (dist3(xa, xb) + dist3(xb, xc));
       }
       F[ip].w = ds;

@@ -114,13 +114,14 @@ TraceBufs trace_bufs(State& st, int cap) {
   return TraceBufs{&st.geo, &st.geo2, &st.rec, &st.npts, &st.tsurf, nullptr, &st.gf, cap};
 }
 
-// Trace + integrate all rays of a set in chunks; outputs [nd][n].
+// Trace + integrate all rays of a set in chunks; outputs [nd][ld] (ld 0: n).
 void integrate_set(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& R, int scattering,
                    const float* J, const int* jbase, float* rad, float* tau, double* tpz, double* tplon,
-                   double* tplat, hipStream_t s, double* rad64 = nullptr, double* tau64 = nullptr) {
+                   double* tplat, hipStream_t s, double* rad64 = nullptr, double* tau64 = nullptr, int ld = 0) {
   State& st = ST();
   const int n = R.nray;
   if (n <= 0) return;
+  if (ld <= 0) ld = n;
   double* tpd = (double*)st.tpd.need((size_t)3 * n * sizeof(double));
   for (int r0 = 0; r0 < n;) {
     int m = chunk_rays(ctl, n - r0);
@@ -129,7 +130,7 @@ void integrate_set(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& R, i
     int cap = engine_trace(ctl, atm, C, 1, tb, tpz ? tpz + r0 : tpd, tplon ? tplon + r0 : tpd + n,
                            tplat ? tplat + r0 : tpd + 2 * n, s);
     engine_integrate(ctl, m, cap, st.rec.as<float>(), st.npts.as<int>(), st.tsurf.as<float>(), scattering, J,
-                     jbase ? jbase + r0 : nullptr, rad, tau, n, r0, s, rad64, tau64);
+                     jbase ? jbase + r0 : nullptr, rad, tau, ld, r0, s, rad64, tau64);
     r0 += m;
   }
 }
@@ -447,7 +448,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
       launch_angular(G, s);
     if (L == 0) {
       integrate_set(ctl, atm, V.rays, S, J, V.jbase.as<int>(), out.rad, out.tau, out.tpz, out.tplon, out.tplat, s,
-                    out.rad64, out.tau64);
+                    out.rad64, out.tau64, out.ld);
       if (out.npts) {
         // LOS points of the primaries (of the last chunk's trace when chunked)
         if (V.rays.nray <= chunk_rays(ctl, V.rays.nray))

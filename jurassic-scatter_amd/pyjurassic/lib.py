@@ -69,7 +69,7 @@ class jur_batch_t(C.Structure):
                 ("vplat", C.c_void_p),
                 ("rad", C.c_void_p), ("tau", C.c_void_p),
                 ("tpz", C.c_void_p), ("tplon", C.c_void_p), ("tplat", C.c_void_p),
-                ("npts", C.c_void_p)]
+                ("npts", C.c_void_p), ("out_ld", C.c_int)]
 
 
 class jur_gsl_matrix(C.Structure):

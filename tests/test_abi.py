@@ -42,6 +42,8 @@ def test_every_declared_symbol_is_exported_and_bound():
 def test_struct_layouts():
     J.check_layout()
     import ctypes as C
-    assert C.sizeof(L.jur_batch_t) == 3 * 4 + 4 + 19 * 8  # 3 ints, pad, 19 pointers
+    # 3 ints, pad, 19 pointers, out_ld (+ pad)
+    assert C.sizeof(L.jur_batch_t) == 3 * 4 + 4 + 19 * 8 + 8
+    assert L.jur_batch_t.out_ld.offset == 3 * 4 + 4 + 19 * 8
     # gsl_matrix: three size_t, two pointers, int (+ padding)
     assert C.sizeof(L.jur_gsl_matrix) == 48 and L.jur_gsl_matrix.data.offset == 24

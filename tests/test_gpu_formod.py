@@ -203,3 +203,30 @@ def test_ascii_tables_vs_oracle(glib, olib, in_examples, tmp_path):
         print("ascii tables (injected rows %s): max rel rad vs oracle %.2e" % (inject, rel), flush=True)
         assert rel < 1e-4
     assert np.max(np.abs(out[(True, "oracle_")] / out[(False, "oracle_")] - 1)) > 1e-6  # the overwrite matters
+
+
+@pytest.mark.parametrize("case,depth", [("clear", 0), ("ext1", 0), ("aero0", 0), ("aero0", 1), ("aero1", 1)])
+def test_formod_pencil_vs_oracle(glib, olib, in_examples, case, depth):
+    """The exported formod_pencil (reference src/forwardmodel.h:32-37,
+    forwardmodel.c:224-405), called ray by ray at scattering depth 0
+    (extinction only with aerosol) and 1, against the oracle's pencil:
+    only ray ir's radiance, transmittance and tangent point change."""
+    ctl, obs, atm, aero = read_example(olib, "oracle_", case)
+    ctl2, obs2, atm2, aero2 = read_example(glib, "", case)
+    rays = range(0, obs.nr, max(1, obs.nr // 4))
+    for ir in rays:
+        olib.oracle_formod_pencil(C.byref(ctl), C.byref(atm), C.byref(obs), C.byref(aero), depth, ir)
+        before = arrays(ctl2, obs2)
+        glib.formod_pencil(C.byref(ctl2), C.byref(atm2), C.byref(obs2), C.byref(aero2), depth, ir)
+        after = arrays(ctl2, obs2)
+        others = [r for r in range(obs.nr) if r != ir]
+        for k in (0, 1):  # rad, tau [nd][nr]
+            np.testing.assert_array_equal(after[k][:, others], before[k][:, others])
+        np.testing.assert_array_equal(after[2][others], before[2][others])  # tangent points [nr][3]
+    sel = list(rays)
+    ref, got = arrays(ctl, obs), arrays(ctl2, obs2)
+    rel = np.abs(got[0][:, sel] / ref[0][:, sel] - 1)
+    print("%s depth %d: %d pencils, max rel rad %.2e" % (case, depth, len(sel), rel.max()), flush=True)
+    np.testing.assert_allclose(got[0][:, sel], ref[0][:, sel], rtol=RAD_RTOL)
+    np.testing.assert_allclose(got[1][:, sel], ref[1][:, sel], atol=TAU_ATOL)
+    np.testing.assert_allclose(got[2][sel], ref[2][sel], atol=1e-6)

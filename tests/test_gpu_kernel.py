@@ -104,9 +104,9 @@ def test_kernel_vs_oracle(glib, olib, in_examples, name):
 @pytest.mark.parametrize("case,tol", [("clear", 1e-11), ("ext1", 1e-10), ("aero0", 2e-7)])
 def test_fp64_formod_vs_oracle(glib, olib, in_examples, case, tol):
     """The FP64 path integration (option fp64) against the oracle: the same
-    operation order on the same data; what remains are the device's
-    transcendental functions, the FP32 layer optics and (with scattering)
-    the FP32 angular integral of the source."""
+    operation order on the same data (FP64 layer optics included); what
+    remains are the device's transcendental functions and, with scattering,
+    the source J, which the angular integral forms in FP32."""
     ctl, obs, atm, aero = read_example(glib, "", case)
     glib.jur_set_option(b"fp64", 1)
     try:
