@@ -159,6 +159,27 @@ int jur_formod_batch(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero, int s
 int jur_formod_batch_device(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero, int scattering,
                             void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+   device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc): jur_formod_batch_device for clear air and
+   extinction (scattering depth 0 when ctl->sca_n > 0), FP32 path, whose
+   work buffers come from a fixed device array of JUR_NOALLOC_SCRATCH_BYTES
+   per GPU (held until the process ends) instead of being allocated; it calls
+   no hipMalloc / hipFree.  The caller passes inputs and outputs as for
+   jur_formod_batch_device; optional ones it leaves NULL (tangent points,
+   atm_of_ray, k) take room in the array too.  The array holds nray x
+   (LOS capacity) x (32 B of geometry, 64 B with aerosol layers, + 4 B per
+   record element: 12 at five emitters and one window) bytes, e.g. 6,000
+   clear-air rays at 512 LOS points; larger batches, scattering depths > 0
+   and option fp64 stop with an error.  The tables and layer optics of
+   ctl / aero must already be resident (one allocating call with the same
+   ctl and aero first), else it stops with an error.  All calls on one GPU
+   share its array: calls that overlap on one GPU (other streams or host
+   threads) race.  Returns 0 once the outputs are complete. */
+#define JUR_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int jur_formod_batch_device_noalloc(ctl_t *ctl, const jur_batch_t *b, const aero_t *aero, int scattering,
+                                    void *stream);
+
 /* Wait for the library stream and check device-side error flags. */
 int jur_sync(void);
 

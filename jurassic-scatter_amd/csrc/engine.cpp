@@ -238,6 +238,15 @@ static std::string channel_key(const ctl_t* ctl, const std::string& key) {
   return ck;
 }
 
+static std::vector<char> aero_snapshot(const ctl_t* ctl, const aero_t* aero);
+
+bool engine_resident(const ctl_t* ctl, const aero_t* aero) {
+  Engine& e = E();
+  const std::string key = table_key(ctl);
+  return key == e.tbl_key && channel_key(ctl, key) == e.chan_key && e.aero_ready &&
+         aero_snapshot(ctl, ctl->sca_n > 0 ? aero : nullptr) == e.aero_snap;
+}
+
 void engine_prepare(const ctl_t* ctl) {
   Engine& e = E();
   hipStream_t s = e.stream;
@@ -956,6 +965,90 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   HIP_OK(hipEventRecord(ev.second, s));
 }
 
+namespace {
+// Bump allocation of 256-byte aligned pieces from a fixed region.
+struct Carve {
+  char* p;
+  size_t left;
+  void* take(size_t n) {
+    const size_t a = (n + 255) & ~(size_t)255;
+    if (a > left) return nullptr;
+    void* r = p;
+    p += a;
+    left -= a;
+    return r;
+  }
+};
+}  // namespace
+
+int engine_forward_noalloc(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out,
+                           void* base, size_t bytes, hipStream_t s) {
+  const int n = rays.nray;
+  if (n <= 0) return 0;
+  const int sca_on = ctl->sca_n > 0;
+  const int R = rec_len(ctl->ng, ctl->nw);
+  Carve c{(char*)base, bytes};
+  int* npts = (int*)c.take((size_t)n * sizeof(int));
+  float* tsurf = (float*)c.take((size_t)n * sizeof(float));
+  int* order = (int*)c.take((size_t)n * sizeof(int));
+  int* status = (int*)c.take(4 * sizeof(int));
+  if (!npts || !tsurf || !order || !status) return -1;
+  // per LOS point: march geometry, geometry after layer-edge insertion,
+  // FP32 record; per ray and capacity step a histogram bin of the sort
+  const size_t per_pt = sizeof(double4) * (sca_on ? 2 : 1) + (size_t)R * sizeof(float);
+  const size_t room = c.left > 4 * 256 ? c.left - 4 * 256 : 0;  // alignment slack of the four pieces
+  const long fit_l = (long)((room / ((size_t)n * per_pt + sizeof(int))) - 1);
+  const int fit = (int)(fit_l > NLOS + 2 ? NLOS + 2 : fit_l);
+  if (fit < 16) return -1;
+  int cap = engine_cap();
+  if (cap > fit) cap = fit;
+  TraceArgs A = trace_args(ctl, atm, rays, sca_on, cap);
+  while (true) {
+    Carve cc = c;
+    const size_t npt = (size_t)n * cap;
+    A = trace_args(ctl, atm, rays, sca_on, cap);
+    A.geo = (double4*)cc.take(npt * sizeof(double4));
+    A.geo2 = sca_on ? (double4*)cc.take(npt * sizeof(double4)) : nullptr;
+    A.rec = (float*)cc.take(npt * R * sizeof(float));
+    int* hist = (int*)cc.take((size_t)(cap + 1) * sizeof(int));
+    if (!A.geo || (sca_on && !A.geo2) || !A.rec || !hist) return -1;
+    A.np_out = npts;
+    A.tsurf_out = tsurf;
+    A.tpz = out.tpz;
+    A.tplon = out.tplon;
+    A.tplat = out.tplat;
+    A.status = status;
+    launch_zero_int(status, 1, s);
+    launch_raytrace(A, s);
+    int st = 0;
+    HIP_OK(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (st & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
+    if (st & JUR_ST_NLOS) JUR_FAIL("Too many LOS points!");
+    if (st & JUR_ST_CAP) {
+      if (cap > NLOS) JUR_FAIL("Too many LOS points!");
+      if (cap >= fit) return -1;  // the rays need more LOS points than the region holds
+      cap = cap * 2 > fit ? fit : cap * 2;
+      continue;
+    }
+    launch_count_sort(npts, n, cap + 1, hist, order, s);
+    break;
+  }
+  PathArgs P = path_args(ctl, n, cap, A.rec, npts, tsurf, order, 0, nullptr, nullptr, out.rad, out.tau,
+                         out.ld > 0 ? out.ld : n, 0, nullptr, nullptr);
+  // the variant the autotuner chose for this shape, if any, else the default
+  // (no timing runs here)
+  int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
+  int cls = 0, best = 2 * 8 + 4;
+  auto it = E().opt.find(tune_key(ctl, n, P.T.shared, &cls));
+  if (it != E().opt.end()) best = (int)it->second;
+  P.cpw = variant >= 0 ? variant : best / 8;
+  launch_path(P, wpb >= 0 ? wpb : best % 8, s);
+  if (out.npts) HIP_OK(hipMemcpyAsync(out.npts, npts, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return cap;
+}
+
 void engine_set_prefix(int nr) { E().prefix_nr = nr; }
 
 extern "C" int jur_set_option(const char* key, double value) {
@@ -976,6 +1069,7 @@ extern "C" int jur_init(int device) {
     e.chan_key.clear();
     e.aero_ready = false;
   }
+  noalloc_scratch(false);  // this device's no-allocation work region (looked up here once)
   return 0;
 }
 extern "C" int jur_sync(void) {

@@ -125,6 +125,19 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
                       const float* tsurf, int scattering, const float* J, const int* jbase,
                       float* rad, float* tau, int out_ld, int out_off, hipStream_t s,
                       double* rad64 = nullptr, double* tau64 = nullptr);
+// True when the tables, channel constants and layer optics of ctl / aero
+// are already on the device (nothing to load or upload).
+bool engine_resident(const ctl_t* ctl, const aero_t* aero);
+// Clear-air / extinction forward call (scattering depth 0, FP32 path) whose
+// work buffers are carved from the device region [base, base + bytes):
+// allocates nothing.  Returns the LOS capacity used, or -1 when the rays do
+// not fit the region.
+int engine_forward_noalloc(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out,
+                           void* base, size_t bytes, hipStream_t s);
+// Device address of the no-allocation work region on the current device
+// (looked up once per device); required: stop with an error if it cannot be
+// found, else return NULL then.
+void* noalloc_scratch(bool required);
 // Layer-edge altitudes of the aerosol layers (for the solar-ray search).
 struct AeroGeom {
   const double* alti;

@@ -41,6 +41,7 @@ SIGNATURES = {
     "write_matrix": (None, [C.c_char_p, C.c_char_p, P, P, P, P, P, C.c_char_p, C.c_char_p, C.c_char_p]),
     "jur_formod_batch": (C.c_int, [P, P, P, C.c_int]),
     "jur_formod_batch_device": (C.c_int, [P, P, P, C.c_int, P]),
+    "jur_formod_batch_device_noalloc": (C.c_int, [P, P, P, C.c_int, P]),
     "jur_sync": (C.c_int, []),
     "jur_init": (C.c_int, [C.c_int]),
     "jur_set_option": (C.c_int, [C.c_char_p, C.c_double]),
