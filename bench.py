@@ -5,7 +5,12 @@ Workload (default): 65,536 rays (60 % limb, 40 % nadir), each with its own
 perturbed atmosphere on 121 levels, 5 emitters, 64 channels 700-1400 cm^-1
 ("clear_batch_65536", see jurassic-scatter_amd/pyjurassic/synth.py).  One
 step = one forward call over all rays with inputs resident in GPU memory:
-FP64 ray tracing, sort by path length, FP32 EGA path integration.
+FP64 ray tracing, sort by path length, FP32 EGA path integration.  `value`
+comes from hipGraph replays of that call (--graph 1, the default); the same
+number of eager calls is timed before them (`eager_ms_per_step`), and the
+path kernel's time (`roofline.kernel_ms`) comes from the HIP events the
+library records around its launches in those eager calls (events recorded
+inside a captured graph cannot be timed on this runtime).
 
 Prints ONE JSON line.  `roofline` is for the dominant kernel (path
 integration), timed with HIP events recorded by the library on its own
@@ -42,8 +47,13 @@ HBM_PEAK_GBS = 8000.0
 # construction (angles would leave 0-180 deg), so it keeps the reference's 28.
 FAN = {"aerosol_limb_8192": 16}
 
-# Rays of the CPU oracle sample per workload (about 10-30 s on 16 cores).
-CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 32768, "aerosol_limb_8192": 320, "cirrus_4096": 4}
+# Rays of the CPU oracle sample per workload (about 10-30 s on 16 cores; the
+# clear-air batch: all of its rays, ~31 s, so that `accuracy` covers every ray) ...
+CPU_SAMPLE = {"clear_limb_17": 17, "clear_batch_65536": 65536, "aerosol_limb_8192": 320, "cirrus_4096": 1}
+# ... and its channels where all of them would take longer (depth 2: a
+# nadir ray through the cloud, 8 of 32 channels, ~20 s)
+CPU_SAMPLE_CHANNELS = {"cirrus_4096": 8}
+CPU_SAMPLE_NADIR = ("cirrus_4096",)
 
 # HBM bytes per launch of the path kernel measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a plain copy
@@ -73,13 +83,17 @@ def parse():
     ap.add_argument("--variant", type=int, default=None,
                     help="path-kernel variant (default: autotuned on the first call)")
     ap.add_argument("--wpb", type=int, default=None, help="waves per block (default: autotuned)")
-    ap.add_argument("--graph", type=int, default=0,
-                    help="1: replay the clear-air call from a captured hipGraph, 0: eager calls (default)")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1 (default): `value` from hipGraph replays of the clear-air call (eager calls are "
+                         "timed beside them); 0: eager calls only")
     ap.add_argument("--weights", default=None,
                     help="trained surrogate for --workload cirrus_16384 (default: the packaged one)")
     ap.add_argument("--fan", type=int, default=None,
                     help="secondary pencils per scattering point (default: the workload's, else 28)")
-    ap.add_argument("--plain-rays", type=int, default=256,
+    ap.add_argument("--chunk", type=int, default=None,
+                    help="rays per call: one step = consecutive calls over blocks of this many rays, filling "
+                         "one output array (progress printed per call)")
+    ap.add_argument("--plain-rays", type=int, default=1024,
                     help="rays of the plain-method comparison for --workload cirrus_16384")
     return ap.parse_args()
 
@@ -160,9 +174,10 @@ def ncores():
         return os.cpu_count() or 1
 
 
-def cpu_oracle(w, argc, argv, sel, scattering, fan=28):
-    """The FP64 oracle on rays `sel` of w with all host cores: (rad, tau
-    [ns][nd], seconds of the timed call, cores)."""
+def cpu_oracle(w, argc, argv, sel, scattering, fan=28, nd=None):
+    """The FP64 oracle on rays `sel` of w (the first nd channels, default
+    all) with all host cores: (rad, tau [ns][nd], seconds of the timed call,
+    cores)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle_ctypes
     from pyjurassic import jtypes as J
@@ -170,6 +185,8 @@ def cpu_oracle(w, argc, argv, sel, scattering, fan=28):
     olib.oracle_set_fan(fan)
     octl = J.ctl_t()
     quiet(lambda: olib.oracle_read_ctl(argc, argv, C.byref(octl)))
+    if nd is not None:
+        octl.nd = nd
     oaero = None
     if w.aerofile:
         oaero = J.aero_t()
@@ -351,7 +368,7 @@ def bench_jacobian(args, lib):
     rays = (n + 1) * nr
     # the last batched call holds every perturbed copy (one group)
     alg_bytes = ipts[-1] * nd * (128 * ng + 8) + lib.jur_stat(b"pencils") * nd * 8
-    path_avg = float(np.mean(path_ms))
+    path_avg = float(np.mean(path_timed))
     achieved = alg_bytes / (path_avg * 1e-3) / 1e9
     line = {
         "metric": "Jacobian: ray x channel radiances/s over the n+1 forward models of kernel()",
@@ -690,42 +707,82 @@ def main():
     lib.jur_set_option(b"fan", fan)
     nd, ng, nray = ctl.nd, ctl.ng, w.nray
     b, (rad, tau, tpz, npts), keep = device_batch(w, nd)
+    # --chunk: one step is several calls over consecutive blocks of rays,
+    # each writing its columns of the same outputs (jur_batch_t.out_ld)
+    chunks = [b]
+    if args.chunk and args.chunk < nray:
+        chunks = []
+        for r0 in range(0, nray, args.chunk):
+            m = min(args.chunk, nray - r0)
+            c = L.jur_batch_t()
+            C.memmove(C.byref(c), C.byref(b), C.sizeof(b))
+            c.nray, c.out_ld = m, nray
+            if w.natm == nray:
+                c.natm = m
+                c.z, c.p, c.t = [x + 8 * r0 * w.nlev for x in (b.z, b.p, b.t)]
+                c.q = b.q + 8 * r0 * ng * w.nlev
+            for f in ("time", "obsz", "obslon", "obslat", "vpz", "vplon", "vplat", "tpz"):
+                setattr(c, f, getattr(b, f) + 8 * r0)
+            c.rad, c.tau, c.npts = b.rad + 4 * r0, b.tau + 4 * r0, b.npts + 4 * r0
+            chunks.append(c)
+
+    # per-step statistics, summed over the step's calls
+    SUMMED = ("path_ms", "trace_ms", "integrated_points", "pencils", "path_launches", "graph_ms")
+    last = {}
 
     def step():
-        lib.jur_formod_batch_device(C.byref(ctl), C.byref(b), aero_p, scattering, None)
-        lib.jur_sync()
+        t_call = time.perf_counter()
+        acc = dict.fromkeys(SUMMED, 0.0)
+        for i, c in enumerate(chunks):
+            lib.jur_formod_batch_device(C.byref(ctl), C.byref(c), aero_p, scattering, None)
+            lib.jur_sync()
+            for k in SUMMED:
+                acc[k] += lib.jur_stat(k.encode())
+            if len(chunks) > 1:
+                print("chunk %d/%d: %d rays, %.1f s" % (i + 1, len(chunks), c.nray, time.perf_counter() - t_call),
+                      file=sys.stderr, flush=True)
+        last.update(acc)
 
-    # kernel times from eager calls (a replayed graph has no per-kernel events)
-    for _ in range(max(args.warmup, 1)):
+    # at least one untimed call of a clear-air shape (the autotuner's run)
+    for _ in range(args.warmup if scattering else max(args.warmup, 1)):
         step()
     torch.cuda.synchronize()
+    # eager calls: path-kernel and ray-trace times from the library's events
     path_ms, trace_ms, ipts = [], [], []
-    for _ in range(2 if args.graph else 0):
-        step()
-        path_ms.append(lib.jur_stat(b"path_ms"))
-        trace_ms.append(lib.jur_stat(b"trace_ms"))
-        ipts.append(lib.jur_stat(b"integrated_points"))
-    graph = bool(args.graph) and not scattering
-    if graph:
-        lib.jur_set_option(b"graph", 1)
-        step()  # eager call + capture
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-        if not graph:
-            path_ms.append(lib.jur_stat(b"path_ms"))
-            trace_ms.append(lib.jur_stat(b"trace_ms"))
-        ipts.append(lib.jur_stat(b"integrated_points"))
+        path_ms.append(last["path_ms"])
+        trace_ms.append(last["trace_ms"])
+        ipts.append(last["integrated_points"])
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    lib.jur_set_option(b"graph", 0)
+    dt_eager = time.perf_counter() - t0
+    # hipGraph replays of the same call (clear air / extinction)
+    graph = bool(args.graph) and not scattering
+    gpath, gcall = [], []
+    if graph:
+        lib.jur_set_option(b"graph", 1)
+        step()  # eager call + capture
+        for _ in range(max(args.warmup, 1)):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            gpath.append(last["path_ms"])
+            gcall.append(last["graph_ms"])
+            ipts.append(last["integrated_points"])
+        torch.cuda.synchronize()
+        dt_graph = time.perf_counter() - t0
+        lib.jur_set_option(b"graph", 0)
+    dt = dt_graph if graph else dt_eager
     ms_per_step = 1e3 * dt / args.steps
     value = nray * nd * args.steps / dt
+    path_timed = gpath if graph and np.all(np.isfinite(gpath)) else path_ms
 
     npts_h = npts.cpu().numpy().astype(np.int64)
     pts = int(npts_h.sum())
-    pencils = int(lib.jur_stat(b"pencils"))
+    pencils = int(last["pencils"])
     # Algorithmic bytes of the path-integration kernel per call: per
     # (LOS point x channel x emitter) 128 B of table nodes (4 (p,T) corners x
     # 2 lookups x 2 bracketing nodes x (u, eps) x 4 B), per (point x channel)
@@ -736,8 +793,9 @@ def main():
     alg_bytes = ipts_call * nd * (128 * ng + 8) + pencils * nd * 8
     path_avg = float(np.mean(path_ms))
     achieved = alg_bytes / (path_avg * 1e-3) / 1e9
-    variant = args.variant if args.variant is not None else int(lib.jur_stat(b"tuned_variant"))
-    wpb = args.wpb if args.wpb is not None else int(lib.jur_stat(b"tuned_wpb"))
+    as_int = lambda v: int(v) if np.isfinite(v) else None
+    variant = args.variant if args.variant is not None else as_int(lib.jur_stat(b"tuned_variant"))
+    wpb = args.wpb if args.wpb is not None else as_int(lib.jur_stat(b"tuned_wpb"))
     traffic, traffic_src = MEASURED_TRAFFIC.get((args.workload, nray, variant, wpb), (None, None))
 
     line = {
@@ -758,25 +816,41 @@ def main():
                    "fan": fan if scattering else None,
                    "pencils": pencils, "integrated_points": ipts_call, "hip_graph": graph,
                    "parallelism": "single GPU"},
+        "eager_ms_per_step": 1e3 * dt_eager / args.steps,
+        "graph_ms_per_step": ms_per_step if graph else None,
+        "graph_call_ms": float(np.mean(gcall)) if graph else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_path_fast", "kernel_ms": path_avg,
+                     "kernel_ms_source": "events inside the replayed graph" if path_timed is gpath
+                     else "events around the eager launches",
+                     "kernel_ms_eager": float(np.mean(path_ms)),
                      "trace_ms": float(np.mean(trace_ms)) if not scattering else None,
-                     "launches_per_call": int(lib.jur_stat(b"path_launches")),
+                     "launches_per_call": int(last["path_launches"]),
                      "alg_bytes_per_call": alg_bytes, "variant": variant, "wpb": wpb},
     }
 
     if not args.no_cpu:
         ns = min(args.cpu_sample or CPU_SAMPLE[args.workload], nray)
-        # every (nray/ns)-th ray: the sample spans limb and nadir geometries
-        sel = np.linspace(0, nray - 1, ns).astype(np.int64)
-        orad, otau, cdt, nc = cpu_oracle(w, argc, argv, sel, scattering, fan)
-        line["cpu_baseline"] = {"value": ns * nd / cdt, "unit": "ray*channel/s", "cores": nc,
+        ncs = min(CPU_SAMPLE_CHANNELS.get(args.workload, nd), nd)
+        if args.workload in CPU_SAMPLE_NADIR:
+            # nadir rays through the cloud (a depth-2 limb ray costs the
+            # oracle several minutes)
+            sel = np.where((w.geom[4] == 0) & (w.geom[6] != 0))[0][:ns]
+            how = "the first %d nadir rays" % len(sel)
+        else:
+            # every (nray/ns)-th ray: the sample spans limb and nadir geometries
+            sel = np.linspace(0, nray - 1, ns).astype(np.int64)
+            how = "%d of the %d rays (evenly spaced)" % (ns, nray)
+        orad, otau, cdt, nc = cpu_oracle(w, argc, argv, sel, scattering, fan, ncs)
+        line["cpu_baseline"] = {"value": len(sel) * ncs / cdt, "unit": "ray*channel/s", "cores": nc,
                                 "kind": "port",
-                                "sample": "%d of the %d rays (evenly spaced), all %d channels, FP64 oracle, "
-                                          "%.1f s wall" % (ns, nray, nd, cdt)}
-        line["accuracy"] = accuracy(rad.cpu().numpy().T[sel], tau.cpu().numpy().T[sel], orad, otau, ctl)
-        line["accuracy"]["sample_rays"] = ns
+                                "sample": "%s, %s %d channels, FP64 oracle, %.1f s wall" %
+                                          (how, "all" if ncs == nd else "the first", ncs, cdt)}
+        line["accuracy"] = accuracy(rad.cpu().numpy().T[sel][:, :ncs], tau.cpu().numpy().T[sel][:, :ncs], orad,
+                                    otau, ctl)
+        line["accuracy"]["sample_rays"] = int(len(sel))
+        line["accuracy"]["sample_channels"] = ncs
     emit(line)
 
 

@@ -236,8 +236,10 @@ int jur_set_option(const char *key, double value);
    kernel call), "kernel_columns", "oe_formod_ms", "oe_kernel_ms",
    "oe_total_ms" (wall time of the last optimal_estimation call: its
    forward models, its Jacobians, in total); running totals over all calls since
-   the library loaded: "calls_total", "path_ms_total" (a replayed graph
-   adds its whole time), "integrated_points_total", "pencils_total".
+   the library loaded: "calls_total", "path_ms_total",
+   "integrated_points_total", "pencils_total".  A replayed graph reports
+   "graph_ms" (the whole call; it also adds that to "path_ms_total") and no
+   per-kernel times ("path_ms", "trace_ms" NaN).
    Unknown keys give NaN. */
 double jur_stat(const char *key);
 

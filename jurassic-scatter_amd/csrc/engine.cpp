@@ -720,6 +720,7 @@ bool replay_forward(const RaySetDev& rays, hipStream_t s) {
     return false;
   }
   e.stats["graph_ms"] = ms;
+  // events recorded inside a graph cannot be timed here: no per-kernel time
   e.stats["path_ms"] = NAN;
   e.stats["trace_ms"] = NAN;
   e.stats["path_launches"] = 1;
@@ -727,8 +728,7 @@ bool replay_forward(const RaySetDev& rays, hipStream_t s) {
   e.stats["pencils"] = rays.nray;
   e.stats["leaf_pencils"] = rays.nray;  // recorded calls are clear air / extinction (depth 0)
   e.stats["scattering_points"] = 0;
-  // running totals: the replay's time is the whole call (no separate
-  // path-kernel timing inside a graph)
+  // running totals (a replay adds its whole time)
   e.stats["calls_total"] += 1;
   e.stats["path_ms_total"] += ms;
   e.stats["integrated_points_total"] += (double)pts;
