@@ -6,8 +6,10 @@ perturbed atmosphere on 121 levels, 5 emitters, 64 channels 700-1400 cm^-1
 ("clear_batch_65536", see jurassic-scatter_amd/pyjurassic/synth.py).  One
 step = one forward call over all rays with inputs resident in GPU memory:
 FP64 ray tracing, sort by path length, FP32 EGA path integration.  `value`
-comes from hipGraph replays of that call (--graph 1, the default); the same
-number of eager calls is timed before them (`eager_ms_per_step`), and the
+comes from hipGraph replays of that call (--graph 1, the default; the
+scattering workloads' calls are recorded from their device-resident form,
+which runs from the sizes of the first call); the same number of calls
+without a graph is timed before them (`eager_ms_per_step`), and the
 path kernel's time (`roofline.kernel_ms`) comes from the HIP events the
 library records around its launches in those eager calls (events recorded
 inside a captured graph cannot be timed on this runtime).
@@ -42,10 +44,14 @@ sys.path.insert(0, str(ROOT / "jurassic-scatter_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# Secondary pencils per scattering point: the aerosol configuration asks for
-# 16; the cirrus configuration's 32 has no form in the reference's fan grid
-# construction (angles would leave 0-180 deg), so it keeps the reference's 28.
-FAN = {"aerosol_limb_8192": 16}
+# The configurations' own scattering parameters (--ref-params: the
+# reference's 28 pencils per fan, 170 Gauss-Hermite radii, 181 angles): the
+# aerosol configuration's 16 pencils per fan and Mie grid (64 log-spaced
+# radii 0.01-20 um, 180 phase angles), the cirrus configuration's 32 pencils
+# (a grid of this project's: the reference's construction leaves 0-180 deg
+# at 32).
+CONFIG_PARAMS = {"aerosol_limb_8192": {"fan": 16, "mie_grid": 1}, "cirrus_4096": {"fan": 32, "mie_grid": 0}}
+REF_PARAMS = {"fan": 28, "mie_grid": 0}
 
 # Rays of the CPU oracle sample per workload (about 10-30 s on 16 cores; the
 # clear-air batch: all of its rays, ~31 s, so that `accuracy` covers every ray) ...
@@ -89,7 +95,10 @@ def parse():
     ap.add_argument("--weights", default=None,
                     help="trained surrogate for --workload cirrus_16384 (default: the packaged one)")
     ap.add_argument("--fan", type=int, default=None,
-                    help="secondary pencils per scattering point (default: the workload's, else 28)")
+                    help="secondary pencils per scattering point (default: the configuration's, else 28)")
+    ap.add_argument("--ref-params", action="store_true",
+                    help="scattering workloads at the reference's fan (28) and Mie grid instead of the "
+                         "configuration's")
     ap.add_argument("--chunk", type=int, default=None,
                     help="rays per call: one step = consecutive calls over blocks of this many rays, filling "
                          "one output array (progress printed per call)")
@@ -174,7 +183,7 @@ def ncores():
         return os.cpu_count() or 1
 
 
-def cpu_oracle(w, argc, argv, sel, scattering, fan=28, nd=None):
+def cpu_oracle(w, argc, argv, sel, scattering, fan=28, nd=None, mie_grid=0):
     """The FP64 oracle on rays `sel` of w (the first nd channels, default
     all) with all host cores: (rad, tau [ns][nd], seconds of the timed call,
     cores)."""
@@ -183,6 +192,7 @@ def cpu_oracle(w, argc, argv, sel, scattering, fan=28, nd=None):
     from pyjurassic import jtypes as J
     olib = oracle_ctypes.load()
     olib.oracle_set_fan(fan)
+    olib.oracle_set_mie_grid(mie_grid)
     octl = J.ctl_t()
     quiet(lambda: olib.oracle_read_ctl(argc, argv, C.byref(octl)))
     if nd is not None:
@@ -697,14 +707,18 @@ def main():
         return bench_retrieval(args, lib)
     if args.workload == "aerosol3d_steep_16":
         return bench_scatter3d(args, lib)
+    params = dict(REF_PARAMS if args.ref_params else CONFIG_PARAMS.get(args.workload, REF_PARAMS))
+    if args.fan:
+        params["fan"] = args.fan
+    fan, mie_grid = params["fan"], params["mie_grid"]
+    lib.jur_set_option(b"fan", fan)
+    lib.jur_set_option(b"mie_grid", mie_grid)  # before the layer optics (setup)
     w, ctl, aero, scattering, (argc, argv) = setup(lib, args.workload, args.nray)
     aero_p = C.byref(aero) if aero is not None else None
     if args.variant is not None:
         lib.jur_set_option(b"variant", args.variant)
     if args.wpb is not None:
         lib.jur_set_option(b"wpb", args.wpb)
-    fan = args.fan or FAN.get(args.workload, 28)
-    lib.jur_set_option(b"fan", fan)
     nd, ng, nray = ctl.nd, ctl.ng, w.nray
     b, (rad, tau, tpz, npts), keep = device_batch(w, nd)
     # --chunk: one step is several calls over consecutive blocks of rays,
@@ -748,17 +762,19 @@ def main():
         step()
     torch.cuda.synchronize()
     # eager calls: path-kernel and ray-trace times from the library's events
-    path_ms, trace_ms, ipts = [], [], []
+    path_ms, trace_ms, ipts, launches = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
         path_ms.append(last["path_ms"])
         trace_ms.append(last["trace_ms"])
         ipts.append(last["integrated_points"])
+        launches.append(last["path_launches"])
     torch.cuda.synchronize()
     dt_eager = time.perf_counter() - t0
-    # hipGraph replays of the same call (clear air / extinction)
-    graph = bool(args.graph) and not scattering
+    # hipGraph replays of the same call (one call per step: the recorded
+    # graph holds the call's buffer addresses)
+    graph = bool(args.graph) and len(chunks) == 1
     gpath, gcall = [], []
     if graph:
         lib.jur_set_option(b"graph", 1)
@@ -813,7 +829,7 @@ def main():
         "data": "synthetic (seeded generator, reference file formats)",
         "config": {"workload": args.workload, "rays": nray, "channels": nd, "emitters": ng,
                    "levels": w.nlev, "los_points": pts, "scattering_depth": scattering,
-                   "fan": fan if scattering else None,
+                   "fan": fan if scattering else None, "mie_grid": mie_grid if scattering else None,
                    "pencils": pencils, "integrated_points": ipts_call, "hip_graph": graph,
                    "parallelism": "single GPU"},
         "eager_ms_per_step": 1e3 * dt_eager / args.steps,
@@ -826,7 +842,7 @@ def main():
                      else "events around the eager launches",
                      "kernel_ms_eager": float(np.mean(path_ms)),
                      "trace_ms": float(np.mean(trace_ms)) if not scattering else None,
-                     "launches_per_call": int(last["path_launches"]),
+                     "launches_per_call": int(launches[-1]),
                      "alg_bytes_per_call": alg_bytes, "variant": variant, "wpb": wpb},
     }
 
@@ -842,7 +858,7 @@ def main():
             # every (nray/ns)-th ray: the sample spans limb and nadir geometries
             sel = np.linspace(0, nray - 1, ns).astype(np.int64)
             how = "%d of the %d rays (evenly spaced)" % (ns, nray)
-        orad, otau, cdt, nc = cpu_oracle(w, argc, argv, sel, scattering, fan, ncs)
+        orad, otau, cdt, nc = cpu_oracle(w, argc, argv, sel, scattering, fan, ncs, mie_grid)
         line["cpu_baseline"] = {"value": len(sel) * ncs / cdt, "unit": "ray*channel/s", "cores": nc,
                                 "kind": "port",
                                 "sample": "%s, %s %d channels, FP64 oracle, %.1f s wall" %

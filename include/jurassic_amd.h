@@ -202,8 +202,14 @@ int jur_init(int device);
    work buffers keep their addresses; a replay whose rays overflow the
    recorded LOS capacity is detected and the call re-run), "surrogate",
    "hull_margin", "collect" (learned scattering source, below),
-   "fan" (secondary pencils per scattering point, even, 10-28; default the
-   reference's 28), "kernel_rays" (rays per batched call of kernel/jur_kernel, default
+   "fan" (secondary pencils per scattering point, even, 10-32; default the
+   reference's 28; 10-28 use the reference's zenith-angle construction, 30
+   and 32 keep its horizon band and first steps and continue in equal steps
+   to 0 and 180 deg), "mie_grid" (1: get_opt_prop integrates each mode over
+   64 radii log-spaced 0.01-20 um with the trapezoid rule in ln r and
+   evaluates the phase function at 180 angles (k + 0.5) deg, resampled to the
+   181 nodes of aero_t; 0, the default: the reference's 170 Gauss-Hermite
+   nodes and 181 angles), "kernel_rays" (rays per batched call of kernel/jur_kernel, default
    131072: the perturbed atmospheres are grouped into calls of about this
    many rays), "fp64" (1: path integration in FP64 arithmetic, the
    reference's operation order, on FP64 tables grids, Planck table,

@@ -54,6 +54,9 @@ struct Engine {
   // captured clear-air forward call (option "graph"): the key holds every
   // address and size the graph was recorded with
   bool capturing = false;
+  // device-resident (planned) calls: kernels only between the host's
+  // launches -- the counting sort, no autotuning runs
+  bool planned = false;
   DBuf hist;
   // inputs of the last engine_set_aero
   bool aero_ready = false;
@@ -544,10 +547,15 @@ TraceArgs trace_args(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
 
 // Enqueue one ray trace of a ray set with per-ray capacity cap (status
 // flags cleared first); no host synchronisation.
+// status: the word the kernels raise flags in (null: the engine's own,
+// cleared first); nvalid / nvalid_off: device-side count of the valid rays.
 void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on, TraceBufs& tb,
-                   int cap, double* tpz, double* tplon, double* tplat, hipStream_t s) {
+                   int cap, double* tpz, double* tplon, double* tplat, hipStream_t s, int* status = nullptr,
+                   const int* nvalid = nullptr, int nvalid_off = 0) {
   Engine& e = E();
   TraceArgs A = trace_args(ctl, atm, rays, sca_on, cap);
+  A.nvalid = nvalid;
+  A.nvalid_off = nvalid_off;
   const int R = A.R;
   const size_t npt = (size_t)rays.nray * cap;
   A.geo = (double4*)tb.geo->need(npt * sizeof(double4));
@@ -569,11 +577,22 @@ void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays
   e.last_tsurf64 = A.tsurf64;
   e.last_nray = rays.nray;
   e.last_cap = cap;
-  A.status = (int*)e.status.need(sizeof(int) * 4);
-  launch_zero_int(A.status, 1, s);
+  if (status) {
+    A.status = status;
+  } else {
+    A.status = (int*)e.status.need(sizeof(int) * 4);
+    launch_zero_int(A.status, 1, s);
+  }
   launch_raytrace(A, s);
 }
 }  // namespace
+
+void engine_trace_async(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on, TraceBufs& tb,
+                        int cap, double* tpz, double* tplon, double* tplat, int* status, const int* nvalid,
+                        int nvalid_off, hipStream_t s) {
+  enqueue_trace(ctl, atm, rays, sca_on, tb, cap, tpz, tplon, tplat, s, status, nvalid, nvalid_off);
+  tb.cap = cap;
+}
 
 // Trace a set of rays into the given buffers; grows the per-ray capacity
 // until every ray fits.  Returns the capacity used.
@@ -618,12 +637,13 @@ void end_call(hipStream_t s) {
   unsigned long long pts = 0;
   HIP_OK(hipMemcpyAsync(&pts, e.pts_acc.p, sizeof pts, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
-  e.stats["path_ms"] = ms;
+  // a replayed graph has no per-launch events
+  e.stats["path_ms"] = e.path_ev_used ? ms : NAN;
   e.stats["path_launches"] = (double)e.path_ev_used;
   e.stats["integrated_points"] = (double)pts;
   // running totals over calls (a retrieval makes many)
   e.stats["calls_total"] += 1;
-  e.stats["path_ms_total"] += ms;
+  e.stats["path_ms_total"] += e.path_ev_used ? ms : engine_stat("scatter_ms");
   e.stats["integrated_points_total"] += (double)pts;
   e.stats["pencils_total"] += e.stats["pencils"];
 }
@@ -663,6 +683,22 @@ std::string graph_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& r
   return key;
 }
 
+}  // namespace
+
+// graph_key plus the option / autotuner entries: what a recorded call of
+// the scattering tree depends on besides its own buffers.
+std::string engine_call_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out,
+                            int scattering, hipStream_t s) {
+  std::string k = graph_key(ctl, atm, rays, out, scattering, s);
+  char b[160];
+  for (const auto& kv : E().opt) {
+    snprintf(b, sizeof b, "|%s=%.17g", kv.first.c_str(), kv.second);
+    k += b;
+  }
+  return k;
+}
+
+namespace {
 void drop_forward_graph() {
   Engine& e = E();
   if (e.g_exec) HIP_OK(hipGraphExecDestroy(e.g_exec));
@@ -843,7 +879,8 @@ PathArgs path_args(const ctl_t* ctl, int nray, int cap, const float* rec, const 
 // Sort rays by point count and run the path integration.
 void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, const int* npts,
                       const float* tsurf, int scattering, const float* J, const int* jbase, float* rad,
-                      float* tau, int out_ld, int out_off, hipStream_t s, double* rad64, double* tau64) {
+                      float* tau, int out_ld, int out_off, hipStream_t s, double* rad64, double* tau64,
+                      long nJ) {
   Engine& e = E();
   int* order = (int*)e.order.need(nray * sizeof(int));
   // FP64 prefix reuse (engine_set_prefix): clear air, FP64 records of this
@@ -860,7 +897,7 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     prem = (int*)e.prem.need(nray * sizeof(int));
     launch_prefix_start(nray, pnr, R_, cap, e.last_rec64, npts, pstart, prem, s);
   }
-  if (e.capturing) {
+  if (e.capturing || e.planned) {
     // counting sort: kernels only (no library temporaries or memsets)
     launch_count_sort(npts, nray, cap + 1, (int*)e.hist.need((size_t)(cap + 1) * sizeof(int)), order, s);
   } else {
@@ -874,6 +911,7 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   }
   PathArgs P = path_args(ctl, nray, cap, rec, npts, tsurf, order, scattering, J, jbase, rad, tau, out_ld, out_off,
                          rad64, tau64);
+  P.nJ = J ? nJ : 0;
   int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
   if (P.fp64) {
     // the FP64 kernels have one configuration (variant and waves per block
@@ -892,7 +930,7 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     int best = 2 * 8 + 4;  // variant 2, 4 waves per block
     if (it != e.opt.end()) {
       best = (int)it->second;
-    } else if (cls >= 2 && engine_option_int("autotune", 1) && !e.capturing) {
+    } else if (cls >= 2 && engine_option_int("autotune", 1) && !e.capturing && !e.planned) {
       hipEvent_t a, b;
       HIP_OK(hipEventCreate(&a));
       HIP_OK(hipEventCreate(&b));
@@ -1050,6 +1088,13 @@ int engine_forward_noalloc(const ctl_t* ctl, const AtmSetDev& atm, const RaySetD
 }
 
 void engine_set_prefix(int nr) { E().prefix_nr = nr; }
+void engine_set_mode(bool planned, bool capturing) {
+  E().planned = planned;
+  E().capturing = capturing;
+}
+bool engine_capturing() { return E().capturing; }
+void engine_begin_call(hipStream_t s) { begin_call(s); }
+void engine_end_call(hipStream_t s) { end_call(s); }
 
 extern "C" int jur_set_option(const char* key, double value) {
   Engine& e = E();

@@ -28,13 +28,18 @@
     }                                                                \
   } while (0)
 
-// Growable device buffer (memory is kept across calls; growth reallocates).
+// Growable device buffer (memory is kept across calls; growth reallocates,
+// after the device has finished the work that may still read the old
+// memory).
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
   void* need(size_t n) {
     if (n > bytes) {
-      if (p) HIP_OK(hipFree(p));
+      if (p) {
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipFree(p));
+      }
       size_t nb = n + n / 4 + 256;
       HIP_OK(hipMalloc(&p, nb));
       bytes = nb;
@@ -120,11 +125,30 @@ struct TraceBufs {
 // fits and returns it.  sca_on inserts aerosol-layer points.
 int engine_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on,
                  TraceBufs& tb, double* tpz, double* tplon, double* tplat, hipStream_t s);
+// Enqueue one trace at a fixed per-ray capacity without waiting for it:
+// flags are raised in *status (JUR_ST_CAP when a ray needs more points), rays
+// r with r + nvalid_off >= *nvalid (nvalid optional) are empty.
+void engine_trace_async(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int sca_on, TraceBufs& tb,
+                        int cap, double* tpz, double* tplon, double* tplat, int* status, const int* nvalid,
+                        int nvalid_off, hipStream_t s);
+// Every address, size and setting a recorded forward call depends on
+// (inputs, outputs, tables, layer geometry and optics, work buffers,
+// options and autotuner choices).
+std::string engine_call_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out,
+                            int scattering, hipStream_t s);
+// planned: device-resident calls (counting sort, no autotuning runs);
+// capturing: the calls are being recorded into a graph (no event timing).
+void engine_set_mode(bool planned, bool capturing);
+bool engine_capturing();
+// Per-call accounting: path-kernel events and the LOS-point counter
+// (end_call waits for the stream).
+void engine_begin_call(hipStream_t s);
+void engine_end_call(hipStream_t s);
 // Sort by point count and integrate along the traced rays.
 void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, const int* npts,
                       const float* tsurf, int scattering, const float* J, const int* jbase,
                       float* rad, float* tau, int out_ld, int out_off, hipStream_t s,
-                      double* rad64 = nullptr, double* tau64 = nullptr);
+                      double* rad64 = nullptr, double* tau64 = nullptr, long nJ = 0);
 // True when the tables, channel constants and layer optics of ctl / aero
 // are already on the device (nothing to load or upload).
 bool engine_resident(const ctl_t* ctl, const aero_t* aero);

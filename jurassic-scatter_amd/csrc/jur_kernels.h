@@ -11,6 +11,7 @@ enum {
   JUR_ST_NLOS = 2,           // "Too many LOS points!"
   JUR_ST_BELOW_SURFACE = 4,  // "Observer below surface!"
   JUR_ST_QUEUE = 8,          // device queue overflow
+  JUR_ST_PLAN = 16,          // a batch outgrew the capacities it was launched with (host re-runs it)
 };
 
 struct TraceArgs {
@@ -37,6 +38,10 @@ struct TraceArgs {
   double *tpz, *tplon, *tplat;
   int* geo_final;      // optional: 1 if the final points are in geo2
   int* status;
+  // optional device-side count of the valid rays of a batch sized by its
+  // capacity: ray r is empty when r + nvalid_off >= *nvalid
+  const int* nvalid;
+  int nvalid_off;
 };
 void launch_raytrace(const TraceArgs& A, hipStream_t s);
 
@@ -54,7 +59,8 @@ struct PathArgs {
   int sca_n;
   int ext_beta_e;          // leaf extinction: 1 = beta_e, 0 = beta_a
   int scattering;          // > 0: points with aerofac > 0 use the source J
-  const float* J;          // [npoints][nd]
+  const float* J;          // [nJ][nd]
+  long nJ;                 // rows of J (a point at or beyond them reads no source)
   const int* jbase;        // [nray] index of the ray's first scattering point
   float* rad;              // [nd][out_ld], ray r at column out_off + r
   float* tau;

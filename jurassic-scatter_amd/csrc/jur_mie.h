@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// Radius nodes of the aerosol configuration's Mie grid (option "mie_grid").
+#define MIE_GRID_NR 64
+
 // One sphere of the Gauss-Hermite radius quadrature at one channel.
 struct MieItem {
   double x, nr, ni;  // size parameter, refractive index
@@ -12,8 +15,10 @@ struct MieNode {
   double rad, K1, w;
 };
 
+// grid 0: the reference's 181 angles; 1: 180 angles (k + 0.5) deg resampled
+// to the 181-node layout (option "mie_grid").
 void launch_bhmie(const MieItem* items, int item0, int nitems, double2* scratch, size_t stride,
-                  double* qext, double* qsca, double* phase, hipStream_t s);
+                  double* qext, double* qsca, double* phase, int grid, hipStream_t s);
 void launch_mie_mode(const MieNode* nodes, int nnode, const int* item_of, int nd, const double* qext,
                      const double* qsca, const double* phase, double* mbe, double* mbs, double* mp,
                      hipStream_t s);

@@ -6,9 +6,11 @@
 
 // Secondary pencils per scattering point: the reference's nalpha = 28
 // (scatter.c:680); its Fibonacci grid construction (scatter.c:698-715)
-// gives increasing angles in [0, 180] deg for every even count from 10 to 28
-// (option "fan").
-#define NALPHA 28
+// gives increasing angles in [0, 180] deg for every even count from 10 to 28;
+// 30 and 32 take this project's grid (alpha_grid in scatter.cpp) (option
+// "fan").
+#define NALPHA_REF 28
+#define NALPHA 32
 #define NALPHA_MIN 10
 
 // One scattering point: Cartesian position, LOS tangent, layer, atmosphere
@@ -54,6 +56,7 @@ struct AngArgs {
   const double* theta_grid;  // [NTHETA] pi i / 180
   const double* bsun;        // [nd] Planck radiance of the Sun (TSUN)
   float* J;                  // [npt][nd]
+  const int* d_npt;          // optional device-side count of the points (npt: capacity)
   // surrogate: sub_idx[k] >= 0 is the point's fan block in rad (plain
   // angular sum); -1 takes the thermal source from the network output
   // ymlp[k][nd] (normalised ln(J / B(T)), J = exp(y ysd + ymu + lnb))
@@ -66,12 +69,22 @@ struct AngArgs {
 };
 
 void launch_count_points(int nray, const int* np, const float* rec, int R, int cap, int* count, hipStream_t s);
+// Points at pts[jbase[r] ...] in LOS order; an index at or beyond pts_cap
+// raises JUR_ST_PLAN in *status instead of writing.
 void launch_emit_points(int nray, const int* np, const float* rec, int R, int cap, const double4* geo,
                         const double4* geo2, const int* geo_final, const int* jbase, const double* time,
-                        const int* atm_idx, ScaPt* pts, hipStream_t s);
-void launch_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s);
+                        const int* atm_idx, ScaPt* pts, long pts_cap, int* status, hipStream_t s);
+// npt: the points' capacity (grid); d_npt (optional): their device-side
+// count, and d_nchild (optional) receives the fan rays' count.
+void launch_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s,
+                     const int* d_npt = nullptr, int* d_nchild = nullptr);
 void launch_sun(int npt, const ScaPt* pts, const SunArgs& S, int* valid, double* theta, RayArrays out,
-                hipStream_t s);
+                hipStream_t s, const int* d_npt = nullptr);
+// Exclusive prefix sum with a device base (graph-capture safe): out[i] =
+// *base + sum_{j<i} in[j], *total = *base + sum (base may be null); tmp holds
+// scan_base_temp_ints(n) ints.
+size_t scan_base_temp_ints(int n);
+void launch_scan_base(const int* in, int n, int* out, int* tmp, const int* base, int* total, hipStream_t s);
 void launch_angular(const AngArgs& G, hipStream_t s);
 // srcfunc_sca_3d (scatter.c:811-895), the method for IP 2 / 3: a direct
 // fan of DIR3D_NT x DIR3D_NPHI pencils per point (children k DIR3D_N +

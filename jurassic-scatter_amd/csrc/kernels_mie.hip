@@ -1,7 +1,10 @@
 // kernels_mie.hip -- Lorenz-Mie optics of log-normal particle modes in FP64.
 //
 // k_bhmie: one workgroup per (mode, radius node, channel) sphere, the
-// Bohren & Huffman series of the reference (src/scatter.c:56-234).  Lane 0
+// Bohren & Huffman series of the reference (src/scatter.c:56-234); with
+// grid 1 the phase function of the aerosol configuration's 180 angles
+// (k + 0.5) deg resampled to the 181 nodes, Q_ext from the forward
+// amplitude.  Lane 0
 // runs the two inherently sequential recurrences (logarithmic derivative
 // D_n downward from nmx, Riccati-Bessel psi/chi upward with the
 // coefficients a_n, b_n and Q_sca), keeping D_n, a_n, b_n in a global
@@ -32,18 +35,20 @@ __device__ inline cxd cdiv(cxd a, cxd b) {
 }
 
 constexpr int NT = (NTHETA + 1) / 2;  // 91 angles of the half range
+constexpr int MIE_CELLS = 180;        // the configuration grid's angles (k + 0.5) deg
 
 }  // namespace
 
 __global__ void __launch_bounds__(128) k_bhmie(const MieItem* items, int item0, double2* scratch,
                                                size_t stride, double* qext_o, double* qsca_o,
-                                               double* phase_o) {
+                                               double* phase_o, int grid) {
   const int it = item0 + blockIdx.x;
   const MieItem I = items[it];
   double2* D = scratch + (size_t)blockIdx.x * 3 * stride;
   double2* An = D + stride;
   double2* Bn = An + stride;
-  __shared__ double qsca_s;
+  __shared__ double qsca_s, qext_s;
+  __shared__ double cell[MIE_CELLS];  // grid 1: the phase function at the 180 cell centres
   const double x = I.x;
   const cxd ref = mk(I.nr, I.ni);
   const int nstop = I.nstop, nmx = I.nmx;
@@ -63,6 +68,7 @@ __global__ void __launch_bounds__(128) k_bhmie(const MieItem* items, int item0, 
     double psi0 = cos(x), psi1 = sin(x), chi0 = -sin(x), chi1 = cos(x), apsi1 = psi1;
     cxd xi1 = mk(apsi1, -chi1);
     double qs = 0;
+    cxd sf = mk(0, 0);  // forward amplitude sum (2n + 1)/2 (a_n + b_n)
     for (int n = 1; n <= nstop; n++) {
       double rn = n;
       double psi = (2.0 * rn - 1.0) * psi1 / x - psi0;
@@ -81,6 +87,7 @@ __global__ void __launch_bounds__(128) k_bhmie(const MieItem* items, int item0, 
       bn = cdiv(bn, bd);
       double aa = cabs_(an), bb = cabs_(bn);
       qs = qs + (2. * rn + 1.) * (aa * aa + bb * bb);
+      sf = cadd(sf, cmul(mk((2. * rn + 1.) / 2., 0.0), cadd(an, bn)));
       An[n] = make_double2(an.re, an.im);
       Bn[n] = make_double2(bn.re, bn.im);
       psi0 = psi1;
@@ -91,9 +98,44 @@ __global__ void __launch_bounds__(128) k_bhmie(const MieItem* items, int item0, 
       xi1 = mk(apsi1, -chi1);
     }
     qsca_s = (2.0 / (x * x)) * qs;
+    qext_s = (4.0 / (x * x)) * sf.re;
   }
   __syncthreads();
   const double qsca = qsca_s;
+  if (grid == 1) {
+    // The configuration's angles (k + 0.5) deg: 90 of the half range and
+    // their mirrors, no angle on 90 deg; then the 181-node layout, node i
+    // the mean of cells i - 1 and i (the end nodes the end cells).
+    for (int j = threadIdx.x; j < MIE_CELLS / 2; j += blockDim.x) {
+      const double amu = cos((j + 0.5) * M_PI / 180.);
+      double pi0 = 0.0, pi1 = 1.0;
+      cxd s1 = mk(0, 0), s2 = mk(0, 0), s1m = mk(0, 0), s2m = mk(0, 0);
+      for (int n = 1; n <= nstop; n++) {
+        const double rn = n, fn = (2.0 * rn + 1.0) / (rn * (rn + 1.0));
+        const double pi = pi1, tau = rn * amu * pi - (rn + 1.0) * pi0;
+        const cxd an = mk(An[n].x, An[n].y), bn = mk(Bn[n].x, Bn[n].y);
+        s1 = cadd(s1, cmul(mk(fn, 0.0), cadd(cmul(an, mk(pi, 0.0)), cmul(bn, mk(tau, 0.0)))));
+        s2 = cadd(s2, cmul(mk(fn, 0.0), cadd(cmul(an, mk(tau, 0.0)), cmul(bn, mk(pi, 0.0)))));
+        const double p = (n & 1) ? 1.0 : -1.0, t = -p;  // (-1)^(n-1), (-1)^n
+        s1m = cadd(s1m, cmul(mk(fn, 0.0), cadd(cmul(an, mk(pi * p, 0.0)), cmul(bn, mk(tau * t, 0.0)))));
+        s2m = cadd(s2m, cmul(mk(fn, 0.0), cadd(cmul(an, mk(tau * t, 0.0)), cmul(bn, mk(pi * p, 0.0)))));
+        pi1 = ((2. * rn + 1.) * amu * pi - (rn + 1.) * pi0) / rn;
+        pi0 = pi;
+      }
+      const double f = 2 / (x * x * qsca);
+      cell[j] = f * (s1.re * s1.re + s1.im * s1.im + s2.re * s2.re + s2.im * s2.im);
+      cell[MIE_CELLS - 1 - j] = f * (s1m.re * s1m.re + s1m.im * s1m.im + s2m.re * s2m.re + s2m.im * s2m.im);
+    }
+    __syncthreads();
+    double* ph = phase_o + (size_t)it * NTHETA;
+    for (int i = threadIdx.x; i < NTHETA; i += blockDim.x)
+      ph[i] = i == 0 ? cell[0] : (i == NTHETA - 1 ? cell[MIE_CELLS - 1] : 0.5 * (cell[i - 1] + cell[i]));
+    if (threadIdx.x == 0) {
+      qext_o[it] = qext_s;
+      qsca_o[it] = qsca;
+    }
+    return;
+  }
   // Scattering amplitudes, one half-range angle per lane (scatter.c:163-214).
   for (int j = 1 + threadIdx.x; j <= NT; j += blockDim.x) {
     const int jj = 2 * NT - j;
@@ -195,10 +237,10 @@ __global__ void k_mie_layers(int nl, const int* mode0, const int* nmod, int nd, 
 }
 
 void launch_bhmie(const MieItem* items, int item0, int nitems, double2* scratch, size_t stride,
-                  double* qext, double* qsca, double* phase, hipStream_t s) {
+                  double* qext, double* qsca, double* phase, int grid, hipStream_t s) {
   if (nitems > 0)
     hipLaunchKernelGGL(k_bhmie, dim3(nitems), dim3(128), 0, s, items, item0, scratch, stride, qext, qsca,
-                       phase);
+                       phase, grid);
 }
 void launch_mie_mode(const MieNode* nodes, int nnode, const int* item_of, int nd, const double* qext,
                      const double* qsca, const double* phase, double* mbe, double* mbs, double* mp,

@@ -203,7 +203,7 @@ __device__ inline void point_radiance(const PathArgs& A, const ChanConst& cc, in
     if (!(ds > 0.f)) return;
     // scattering source (forwardmodel.c:332-358)
     const float be = A.beta_e[ai * A.nd + id], bs = A.beta_s[ai * A.nd + id];
-    const float J = A.J[(size_t)kj * A.nd + id];
+    const float J = kj < A.nJ ? A.J[(size_t)kj * A.nd + id] : 0.f;  // a batch over its planned capacity is re-run
     const float lg = et < 0.5f ? -log1pf(-et) : -logf(tg);
     const float bt = lg / ds + beta + af * be;
     const float x = bt * ds;
@@ -807,7 +807,7 @@ __device__ __forceinline__ void point64(const PathArgs& A, const ChanConst64& cc
     // adds nothing (the reference forms -log(1)/0 there)
     if (tau_gas > 0.0 && ds > 0.0) {
       const double be = A.beta_e64[ai * A.nd + id], bs = A.beta_s64[ai * A.nd + id];
-      const double J = A.J[(size_t)kj * A.nd + id];
+      const double J = kj < A.nJ ? (double)A.J[(size_t)kj * A.nd + id] : 0.0;
       const double bt = -log(tau_gas) / ds + beta + af * be;
       const double eps = 1.0 - exp(-bt * ds);
       const double src = ((bt - af * bs) * B + af * bs * J) / bt;

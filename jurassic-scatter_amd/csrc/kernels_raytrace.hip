@@ -42,6 +42,16 @@ struct InsertSink {
 __global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= A.nray) return;
+  if (A.nvalid && r + A.nvalid_off >= *A.nvalid) {
+    // beyond the device-side count of a batch sized by its capacity: an
+    // empty ray (no LOS points, no surface)
+    A.np_out[r] = 0;
+    A.tsurf_out[r] = -999.f;
+    if (A.tsurf64) A.tsurf64[r] = -999.0;
+    A.tpz[r] = A.tplon[r] = A.tplat[r] = NAN;
+    if (A.geo_final) A.geo_final[r] = 0;
+    return;
+  }
   const int a = A.atm_idx ? A.atm_idx[r] : 0;
   const DevAtm& M = A.atm;
   double4* G = A.geo + (size_t)r * A.cap;

@@ -96,7 +96,7 @@ __global__ void k_count_points(int nray, const int* np, const float* rec, int R,
 // after the last is the last point itself).
 __global__ void k_emit_points(int nray, const int* np, const float* rec, int R, int cap, const double4* geo,
                               const double4* geo2, const int* geo_final, const int* jbase, const double* time,
-                              const int* atm_idx, ScaPt* pts) {
+                              const int* atm_idx, ScaPt* pts, long pts_cap, int* status) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nray) return;
   const float* pr = rec + (size_t)r * cap * R;
@@ -120,6 +120,10 @@ __global__ void k_emit_points(int nray, const int* np, const float* rec, int R, 
     P.time = time[r];
     P.af = pr[(size_t)ip * R + REC_AF];
     P.pad = 0;
+    if (k >= pts_cap) {
+      atomicOr(status, (int)JUR_ST_PLAN);
+      return;
+    }
     pts[k++] = P;
   }
 }
@@ -127,9 +131,13 @@ __global__ void k_emit_points(int nray, const int* np, const float* rec, int R, 
 // The fan of nalpha secondary pencils of each point (scatter.c:717-740):
 // view points 10 km away at zenith angles alpha in the local vertical-north
 // plane.
-__global__ void k_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out) {
+__global__ void k_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out,
+                           const int* d_npt, int* d_nchild) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long)npt * nalpha) return;
+  // npt: the points' capacity; d_npt (optional): their device-side count
+  const int nv = d_npt ? min(*d_npt, npt) : npt;
+  if (d_nchild && i == 0) *d_nchild = nv * nalpha;
+  if (i >= (long)nv * nalpha) return;
   int k = (int)(i / nalpha), ir = (int)(i % nalpha);
   const ScaPt P = pts[k];
   double lx[3], ly[3], lz[3], xv[3];
@@ -152,9 +160,14 @@ __global__ void k_children(int npt, int nalpha, const ScaPt* pts, const double* 
 // traces towards the Sun, correcting the geometric zenith angle by the
 // golden ratio of the beam's end-point mismatch.  The sun contributes when
 // the final trace does not hit the ground.
-__global__ void k_sun(int npt, const ScaPt* pts, SunArgs S, int* valid, double* theta, RayArrays out) {
+__global__ void k_sun(int npt, const ScaPt* pts, SunArgs S, int* valid, double* theta, RayArrays out,
+                      const int* d_npt) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npt) return;
+  if (d_npt && k >= *d_npt) {
+    valid[k] = 0;
+    return;
+  }
   const ScaPt P = pts[k];
   double lx[3], ly[3], lz[3], ek[3] = {0, 0, 0}, x1[3], x0[3], dout[3];
   local_frame(P.x, lx, ly, lz);
@@ -225,6 +238,7 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
   __shared__ double Aw[180][NA];
   __shared__ double alpha[NA];
   const int k = blockIdx.x;
+  if (G.d_npt && k >= *G.d_npt) return;  // beyond the device-side count of the points
   const ScaPt P = G.pts[k];
   const int sk = G.sub_idx ? G.sub_idx[k] : k;
   if (sk < 0) {
@@ -406,18 +420,89 @@ void launch_count_points(int nray, const int* np, const float* rec, int R, int c
 }
 void launch_emit_points(int nray, const int* np, const float* rec, int R, int cap, const double4* geo,
                         const double4* geo2, const int* geo_final, const int* jbase, const double* time,
-                        const int* atm_idx, ScaPt* pts, hipStream_t s) {
+                        const int* atm_idx, ScaPt* pts, long pts_cap, int* status, hipStream_t s) {
   if (nray > 0)
     hipLaunchKernelGGL(k_emit_points, dim3(nblk(nray, 128)), dim3(128), 0, s, nray, np, rec, R, cap, geo, geo2,
-                       geo_final, jbase, time, atm_idx, pts);
+                       geo_final, jbase, time, atm_idx, pts, pts_cap, status);
 }
-void launch_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s) {
+void launch_children(int npt, int nalpha, const ScaPt* pts, const double* alpha, RayArrays out, hipStream_t s,
+                     const int* d_npt, int* d_nchild) {
   long n = (long)npt * nalpha;
-  if (n > 0) hipLaunchKernelGGL(k_children, dim3(nblk(n, 128)), dim3(128), 0, s, npt, nalpha, pts, alpha, out);
+  if (n > 0)
+    hipLaunchKernelGGL(k_children, dim3(nblk(n, 128)), dim3(128), 0, s, npt, nalpha, pts, alpha, out, d_npt,
+                       d_nchild);
+  else if (d_nchild)
+    launch_zero_int(d_nchild, 1, s);
 }
 void launch_sun(int npt, const ScaPt* pts, const SunArgs& S, int* valid, double* theta, RayArrays out,
-                hipStream_t s) {
-  if (npt > 0) hipLaunchKernelGGL(k_sun, dim3(nblk(npt, 64)), dim3(64), 0, s, npt, pts, S, valid, theta, out);
+                hipStream_t s, const int* d_npt) {
+  if (npt > 0)
+    hipLaunchKernelGGL(k_sun, dim3(nblk(npt, 64)), dim3(64), 0, s, npt, pts, S, valid, theta, out, d_npt);
+}
+
+// ---- exclusive prefix sums with a carried base (graph-capture safe) --------
+// out[i] = *base + sum_{j<i} in[j], *total = *base + sum in (base null: 0);
+// three kernels over blocks of 1024 elements, no library temporaries and no
+// memsets.
+namespace {
+constexpr int SCAN_B = 1024;
+__global__ void __launch_bounds__(SCAN_B) k_scan_blocks(const int* in, int n, int* out, int* bsum) {
+  __shared__ int v[SCAN_B];
+  const int t = threadIdx.x;
+  const long i = (long)blockIdx.x * SCAN_B + t;
+  const int x = i < n ? in[i] : 0;
+  v[t] = x;
+  __syncthreads();
+  for (int o = 1; o < SCAN_B; o <<= 1) {
+    const int y = t >= o ? v[t - o] : 0;
+    __syncthreads();
+    v[t] += y;
+    __syncthreads();
+  }
+  if (i < n) out[i] = v[t] - x;  // exclusive within the block
+  if (t == SCAN_B - 1) bsum[blockIdx.x] = v[t];
+}
+// One block: block sums -> exclusive block offsets (plus the base), total.
+__global__ void __launch_bounds__(SCAN_B) k_scan_sums(int* bsum, int nb, const int* base, int* total) {
+  __shared__ int part[SCAN_B];
+  const int t = threadIdx.x, per = (nb + SCAN_B - 1) / SCAN_B;
+  int sum = 0;
+  for (int j = 0; j < per; j++) {
+    const int b = t * per + j;
+    if (b < nb) sum += bsum[b];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < SCAN_B; o <<= 1) {
+    const int y = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += y;
+    __syncthreads();
+  }
+  const int b0 = base ? *base : 0;
+  int run = b0 + part[t] - sum;
+  for (int j = 0; j < per; j++) {
+    const int b = t * per + j;
+    if (b < nb) {
+      const int c = bsum[b];
+      bsum[b] = run;
+      run += c;
+    }
+  }
+  if (t == SCAN_B - 1) *total = b0 + part[t];
+}
+__global__ void k_scan_add(int* out, int n, const int* boff) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += boff[i / SCAN_B];
+}
+}  // namespace
+
+size_t scan_base_temp_ints(int n) { return (size_t)(n + SCAN_B - 1) / SCAN_B + 1; }
+void launch_scan_base(const int* in, int n, int* out, int* tmp, const int* base, int* total, hipStream_t s) {
+  const int nb = (n + SCAN_B - 1) / SCAN_B;
+  if (n > 0) hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_B), 0, s, in, n, out, tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, s, tmp, nb, base, total);
+  if (n > 0) hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, s, out, n, tmp);
 }
 template <int NA>
 void launch_angular_t(const AngArgs& G, hipStream_t s) {
@@ -435,6 +520,8 @@ void launch_angular(const AngArgs& G, hipStream_t s) {
     case 22: launch_angular_t<22>(G, s); break;
     case 24: launch_angular_t<24>(G, s); break;
     case 26: launch_angular_t<26>(G, s); break;
+    case 30: launch_angular_t<30>(G, s); break;
+    case 32: launch_angular_t<32>(G, s); break;
     default: launch_angular_t<28>(G, s); break;
   }
 }

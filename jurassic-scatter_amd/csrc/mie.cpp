@@ -73,7 +73,7 @@ MieBufs& MB() {
 
 // Run k_bhmie over all items, in chunks that keep the D_n / a_n / b_n scratch
 // under ~1 GB.
-void run_bhmie(const std::vector<MieItem>& items, hipStream_t s) {
+void run_bhmie(const std::vector<MieItem>& items, hipStream_t s, int grid = 0) {
   MieBufs& b = MB();
   const int n = (int)items.size();
   if (n == 0) return;
@@ -92,7 +92,8 @@ void run_bhmie(const std::vector<MieItem>& items, hipStream_t s) {
   double2* scr = (double2*)b.scratch.need(per * chunk);
   for (int i0 = 0; i0 < n; i0 += chunk) {
     int m = n - i0 < chunk ? n - i0 : chunk;
-    launch_bhmie(di, i0, m, scr, stride, b.qext.as<double>(), b.qsca.as<double>(), b.phase.as<double>(), s);
+    launch_bhmie(di, i0, m, scr, stride, b.qext.as<double>(), b.qsca.as<double>(), b.phase.as<double>(), grid,
+                 s);
   }
 }
 
@@ -213,9 +214,14 @@ void engine_get_opt_prop(ctl_t* ctl, aero_t* aero) {
     gh = true;
   }
   const int nmode = aero->nm;
+  // radius quadrature: the reference's NRAD Gauss-Hermite nodes, or with
+  // option "mie_grid" 1 the aerosol configuration's 64 radii log-spaced
+  // over 0.01-20 um (trapezoid rule in ln r over the log-normal density)
+  const int grid = engine_option_int("mie_grid", 0) ? 1 : 0;
+  const int NR = grid ? MIE_GRID_NR : NRAD;
   std::vector<MieItem> items;
-  std::vector<int> item_of((size_t)nmode * NRAD * nd, -1);
-  std::vector<MieNode> nodes((size_t)nmode * NRAD);
+  std::vector<int> item_of((size_t)nmode * NR * nd, -1);
+  std::vector<MieNode> nodes((size_t)nmode * NR);
   std::vector<double> nr(nd), ni(nd);
   // tabulated modes ("Ext"): their properties, host side, into the mode rows below
   std::vector<char> is_ext(nmode, 0);
@@ -258,20 +264,31 @@ void engine_get_opt_prop(ctl_t* ctl, aero_t* aero) {
     if (aero->nn[m] <= 0 || aero->rr[m] <= 0 || aero->ss[m] <= 1)
       JUR_FAIL("The log-normal parameters are nonsense. ((?_?)) ");
     const double K1 = aero->nn[m] * 1e-3 * sqrt(M_PI);
-    for (int nn = 0; nn < NRAD; nn++) {
-      double rad = exp(sqrt(2) * log(aero->ss[m]) * zs[nn] + log(aero->rr[m]));
-      nodes[(size_t)m * NRAD + nn] = {rad, K1, wts[nn]};
+    for (int nn = 0; nn < NR; nn++) {
+      double rad, wn;
+      if (grid) {
+        // the Gauss-Hermite form's weight becomes exp(-z^2) dz, z = (ln r -
+        // ln r_m) / (sqrt 2 ln s), with the trapezoid step in ln r
+        const double l0 = log(0.01), dl = (log(20.) - l0) / (MIE_GRID_NR - 1), sg = sqrt(2) * log(aero->ss[m]);
+        rad = exp(l0 + nn * dl);
+        const double z = (log(rad) - log(aero->rr[m])) / sg;
+        wn = exp(-z * z) * dl / sg * ((nn == 0 || nn == MIE_GRID_NR - 1) ? 0.5 : 1.0);
+      } else {
+        rad = exp(sqrt(2) * log(aero->ss[m]) * zs[nn] + log(aero->rr[m]));
+        wn = wts[nn];
+      }
+      nodes[(size_t)m * NR + nn] = {rad, K1, wn};
       if (rad >= 0.001 && rad <= 1000. && K1 > 0.) {
         for (int id = 0; id < nd; id++) {
           double lambda = 1. / (ctl->nu[id]) * pow(10, 4.);
           double x = 2 * M_PI * rad / lambda;
-          item_of[((size_t)m * NRAD + nn) * nd + id] = (int)items.size();
+          item_of[((size_t)m * NR + nn) * nd + id] = (int)items.size();
           items.push_back(mie_item(x, nr[id], ni[id]));
         }
       }
     }
   }
-  run_bhmie(items, s);
+  run_bhmie(items, s, grid);
   // Mode sums over radius nodes, then layer sums over modes.
   MieNode* dn = (MieNode*)b.nodes.need(nodes.size() * sizeof(MieNode));
   int* dio = (int*)b.item_of.need(item_of.size() * sizeof(int));
@@ -290,7 +307,7 @@ void engine_get_opt_prop(ctl_t* ctl, aero_t* aero) {
                             (size_t)nd * NTHETA * sizeof(double), hipMemcpyHostToDevice, s));
       continue;
     }
-    launch_mie_mode(dn + (size_t)m * NRAD, NRAD, dio + (size_t)m * NRAD * nd, nd, b.qext.as<double>(),
+    launch_mie_mode(dn + (size_t)m * NR, NR, dio + (size_t)m * NR * nd, nd, b.qext.as<double>(),
                     b.qsca.as<double>(), b.phase.as<double>(), mbe + (size_t)m * nd, mbs + (size_t)m * nd,
                     mp + (size_t)m * nd * NTHETA, s);
   }

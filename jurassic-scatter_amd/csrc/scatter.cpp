@@ -10,9 +10,22 @@
 // is then integrated bottom-up: leaves (depth 0, extinction only) and solar
 // rays first, then per level the angular integral gives J at every point
 // and the level's rays are integrated with that source.  Large levels are
-// traced and integrated in chunks that fit a memory budget.
+// traced and integrated in chunks that fit a memory budget; the scattering
+// points of a level are emitted chunk by chunk behind a running device-side
+// offset, so every ray is traced once.
+//
+// Two ways to run a call.  Eager: the host reads each level's counts back
+// and sizes the next level exactly (and grows the LOS capacity on
+// overflow).  Planned: a later call of the same shape reuses the eager
+// call's sizes with a margin as capacities; every count stays on the device
+// (kernels launched over a capacity read the count and leave the rest
+// empty), the host waits once at the end, and a call that outgrows a
+// capacity (a flag in the status word) is run again eagerly.  A planned
+// call is a fixed sequence of launches, so with option "graph" it is
+// recorded once as a hipGraph and replayed.
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "jur_scatter.h"
 #include "jur_scatter_dev.h"
@@ -55,14 +68,17 @@ struct RayBuf {
 };
 
 struct Level {
-  RaySetDev rays;        // the level's rays (level 0: the caller's)
-  RayBuf own;            // storage of levels >= 1
-  int npt = 0;           // scattering points
-  DBuf count, jbase, pts, J;
-  int nsun = 0;
-  DBuf sun_valid, sun_pos, sun_theta, sun_idx;
-  RayBuf sun_tmp, sun;   // solar rays (all points, then compacted)
-  DBuf rad, tau, srad, stau;
+  RaySetDev rays;            // the level's rays (level 0: the caller's); planned: capacity
+  const int* d_nray = nullptr;  // planned levels >= 1: device-side count of the valid rays
+  RayBuf own;                // storage of levels >= 1
+  long npt = 0;              // scattering points (eager: exact; planned: capacity)
+  DBuf count, jbase, ctot, J;
+  DBuf pts;
+  long nsun = 0;             // solar rays (eager: exact; planned: capacity)
+  DBuf sun_valid, sun_pos, sun_theta, sun_idx, d_nsun;
+  RayBuf sun_tmp, sun;
+  DBuf rad, tau, srad, stau, d_nchild;
+  const int* d_npt = nullptr;   // device-side count of the points (in ctot)
 };
 
 struct State {
@@ -72,7 +88,7 @@ struct State {
   // layer phase-function moments for the surrogate features ([nl][nd])
   DBuf asym, pfwd, pback;
   // surrogate: features, network output, hull flags, fallback compaction
-  DBuf fx, fy, fflag, fpos, fsub, fpts, flnb;
+  DBuf fx, fy, fflag, fpos, fsub, fpts, flnb, fnfan;
   // the two vertical pencils of every surrogate point (radiance summary
   // features): zenith angles, rays, radiances
   DBuf ud_alpha, ud_rad, ud_tau;
@@ -80,6 +96,18 @@ struct State {
   // collected training pairs of the last call (raw features, thermal J)
   DBuf craw, cjth;
   long ncollect = 0;
+  DBuf counts_h;  // device staging of the planned call's counts
+  std::vector<Level> lv;
+  // plan of the device-resident call: capacities from the last eager call
+  struct Plan {
+    std::string key;
+    int cap = 0;
+    std::vector<long> nray, npt;  // [S + 1] ray capacities, [S] point capacities
+  } plan;
+  // the planned call recorded as a graph
+  std::string g_key;
+  hipGraph_t g_graph = nullptr;
+  hipGraphExec_t g_exec = nullptr;
 };
 State& ST() {
   static State s;
@@ -100,10 +128,11 @@ RaySetDev sub(const RaySetDev& R, int r0, int n) {
   return S;
 }
 
-// Rays per chunk under the memory budget for the trace buffers.
-int chunk_rays(const ctl_t* ctl, int n) {
+// Rays per chunk under the memory budget for the trace buffers at LOS
+// capacity cap.
+int chunk_rays(const ctl_t* ctl, long n, int cap) {
   const int R = rec_len(ctl->ng, ctl->nw);
-  const size_t per = (size_t)engine_cap() * (2 * sizeof(double4) + R * sizeof(float)) + 64;
+  const size_t per = (size_t)cap * (2 * sizeof(double4) + R * sizeof(float)) + 64;
   const size_t budget = (size_t)engine_option_int("chunk_mb", 6144) << 20;
   size_t c = budget / per;
   if (c < 64) c = 64;
@@ -114,30 +143,66 @@ TraceBufs trace_bufs(State& st, int cap) {
   return TraceBufs{&st.geo, &st.geo2, &st.rec, &st.npts, &st.tsurf, nullptr, &st.gf, cap};
 }
 
+// How the current call runs: eager (host-sized, cap grown on demand) or
+// planned (fixed capacities, device-side counts, flags in status).
+struct Mode {
+  bool planned = false;
+  int cap = 0;          // planned: LOS capacity of every trace
+  int* status = nullptr;
+};
+
+// Trace one chunk of rays: eager with capacity growth, planned at the fixed
+// capacity against the device-side count nvalid (rays r0.. of the set).
+int trace_chunk(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& C, TraceBufs& tb, double* tpz,
+                double* tplon, double* tplat, const Mode& M, const int* nvalid, int r0, hipStream_t s) {
+  if (!M.planned) return engine_trace(ctl, atm, C, 1, tb, tpz, tplon, tplat, s);
+  engine_trace_async(ctl, atm, C, 1, tb, M.cap, tpz, tplon, tplat, M.status, nvalid, r0, s);
+  return M.cap;
+}
+
 // Trace + integrate all rays of a set in chunks; outputs [nd][ld] (ld 0: n).
+// nvalid (planned): device-side count of the set's valid rays.
 void integrate_set(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& R, int scattering,
                    const float* J, const int* jbase, float* rad, float* tau, double* tpz, double* tplon,
-                   double* tplat, hipStream_t s, double* rad64 = nullptr, double* tau64 = nullptr, int ld = 0) {
+                   double* tplat, hipStream_t s, const Mode& M, const int* nvalid = nullptr,
+                   double* rad64 = nullptr, double* tau64 = nullptr, int ld = 0, long nJ = 0) {
   State& st = ST();
   const int n = R.nray;
   if (n <= 0) return;
   if (ld <= 0) ld = n;
   double* tpd = (double*)st.tpd.need((size_t)3 * n * sizeof(double));
   for (int r0 = 0; r0 < n;) {
-    int m = chunk_rays(ctl, n - r0);
+    const int m = chunk_rays(ctl, n - r0, M.planned ? M.cap : engine_cap());
     TraceBufs tb = trace_bufs(st, 0);
     RaySetDev C = sub(R, r0, m);
-    int cap = engine_trace(ctl, atm, C, 1, tb, tpz ? tpz + r0 : tpd, tplon ? tplon + r0 : tpd + n,
-                           tplat ? tplat + r0 : tpd + 2 * n, s);
+    const int cap = trace_chunk(ctl, atm, C, tb, tpz ? tpz + r0 : tpd, tplon ? tplon + r0 : tpd + n,
+                                tplat ? tplat + r0 : tpd + 2 * n, M, nvalid, r0, s);
     engine_integrate(ctl, m, cap, st.rec.as<float>(), st.npts.as<int>(), st.tsurf.as<float>(), scattering, J,
-                     jbase ? jbase + r0 : nullptr, rad, tau, ld, r0, s, rad64, tau64);
+                     jbase ? jbase + r0 : nullptr, rad, tau, ld, r0, s, rad64, tau64, nJ);
     r0 += m;
   }
 }
 
-// The 28 fan zenith angles of srcfunc_sca_1d (scatter.c:698-715): 83..90
-// deg in 1 deg steps, then steps growing along 3, 3, 5, 8, 13, 13, ... deg
-// towards 0 and 180 deg.
+// Grow a device buffer to n bytes keeping its first `used` bytes.
+void* need_keep(DBuf& b, size_t used, size_t n, hipStream_t s) {
+  if (n <= b.bytes) return b.p;
+  DBuf nb;
+  nb.need(n);
+  if (used) HIP_OK(hipMemcpyAsync(nb.p, b.p, used, hipMemcpyDeviceToDevice, s));
+  HIP_OK(hipStreamSynchronize(s));
+  b.release();
+  b.p = nb.p;
+  b.bytes = nb.bytes;
+  return b.p;
+}
+
+// The fan zenith angles of srcfunc_sca_1d (scatter.c:698-715): 83..90 deg
+// in 1 deg steps, then steps growing along 3, 3, 5, 8, 13, 13, ... deg
+// towards 0 and 180 deg (the reference's 28 and every even count down to
+// 10).  30 and 32 pencils (the cirrus configuration's 32 would leave
+// 0-180 deg in that construction): the horizon band and the first four
+// steps on either side (to 111 and 62 deg), then nalpha/2 - 10 equal steps
+// to 180 and 0 deg.
 void alpha_grid(double* alpha, int nalpha) {
   int n1 = 1, n2 = 2;
   double midang = 83, up = 92, down = 81, step = 1 + 2;
@@ -146,7 +211,8 @@ void alpha_grid(double* alpha, int nalpha) {
     alpha[ir] = midang * M_PI / 180.;
     midang++;
   }
-  for (int ir = 0; ir < nalpha / 2 - 5; ++ir) {
+  const int nref = nalpha <= 28 ? nalpha / 2 - 5 : 5;
+  for (int ir = 0; ir < nref; ++ir) {
     alpha[nalpha / 2 + ir + 4] = up * M_PI / 180.;
     alpha[nalpha / 2 - ir - 5] = down * M_PI / 180.;
     up += step;
@@ -155,6 +221,13 @@ void alpha_grid(double* alpha, int nalpha) {
       step = (double)n1 + (double)n2;
       n1 = n2;
       n2 = (int)step;
+    }
+  }
+  if (nalpha > 28) {
+    const int m = nalpha / 2 - 10;
+    for (int k = 1; k <= m; k++) {
+      alpha[nalpha / 2 + 8 + k] = (111. + k * (180. - 111.) / (m + 1)) * M_PI / 180.;
+      alpha[nalpha / 2 - 9 - k] = (62. - k * 62. / (m + 1)) * M_PI / 180.;
     }
   }
   alpha[nalpha - 1] = M_PI;
@@ -212,99 +285,79 @@ void scatter_set_phase(const ctl_t* ctl, const aero_t* aero, hipStream_t s) {
   st.nd_phase = nd;
 }
 
-void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int S, const OutDev& out,
-                     hipStream_t s) {
+namespace {
+
+// Enqueue the scattering tree of one call (eager: also size each level from
+// the counts read back, and record the exact sizes in act_npt / act_nsun).
+void run_tree(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int S, const OutDev& out,
+              hipStream_t s, const Mode& M, int nalpha, bool use_sur, bool collect, bool dir3d,
+              std::vector<long>* act_npt, std::vector<long>* act_nsun, long* pencils_eager) {
   State& st = ST();
   const int nd = ctl->nd, Rl = rec_len(ctl->ng, ctl->nw);
-  // IP 2 / 3: srcfunc_sca_3d's direct fans (scatter.c:654-657)
-  const bool dir3d = ctl->ip != 1;
-  // surrogate for the thermal source at the primaries' points (option
-  // "surrogate", once a network is trained or loaded); "collect" keeps the
-  // plain method's (features, thermal J) pairs of those points
-  // the surrogate learns srcfunc_sca_1d's source: vertical profiles only
-  const bool use_sur = engine_option_int("surrogate", 0) && mlp_trained() && !dir3d;
-  const bool collect = engine_option_int("collect", 0) != 0 && !dir3d;
-  st.ncollect = 0;
-  engine_set_stat("surrogate_points", 0);
-  engine_set_stat("fallback_points", 0);
-  // pencils per fan (option "fan", default the reference's 28)
-  const int nalpha = engine_option_int("fan", NALPHA);
-  if (nalpha < NALPHA_MIN || nalpha > NALPHA || nalpha % 2) JUR_FAIL("Unsupported number of secondary pencils!");
-  if (st.nalpha != nalpha) {
-    std::vector<double> alpha(nalpha);
-    alpha_grid(alpha.data(), nalpha);
-    upload_vec(st.alpha, alpha, s);
-    st.nalpha = nalpha;
-  }
-  static std::vector<Level> lv;
+  auto& lv = st.lv;
   if ((int)lv.size() < S + 1) lv.resize(S + 1);
   lv[0].rays = rays;
-  int* status = (int*)st.status.need(sizeof(int));
-  launch_zero_int(status, 1, s);
+  lv[0].d_nray = nullptr;
+  const State::Plan& P = st.plan;
+  int* status = M.status;
   AeroGeom ag = engine_aero_geom();
-  hipEvent_t e0, e1;
-  HIP_OK(hipEventCreate(&e0));
-  HIP_OK(hipEventCreate(&e1));
-  HIP_OK(hipEventRecord(e0, s));
   long pencils = rays.nray;
 
   // ---- build the tree top-down ------------------------------------------
   for (int L = 0; L < S; L++) {
     Level& V = lv[L];
-    const int n = V.rays.nray;
+    const long n = V.rays.nray;
     int* count = (int*)V.count.need((size_t)(n + 1) * sizeof(int));
     int* jbase = (int*)V.jbase.need((size_t)(n + 1) * sizeof(int));
-    // pass 1: trace and count the scattering points of every ray
-    int nchunks = 0, last_cap = 0;
-    for (int r0 = 0; r0 < n; nchunks++) {
-      int m = chunk_rays(ctl, n - r0);
+    // running point offsets: ctot[c] = points of the chunks before chunk c
+    int* ctot = (int*)V.ctot.need(((size_t)n / 64 + 3) * sizeof(int));
+    launch_zero_int(ctot, 1, s);
+    int* scantmp = (int*)st.scan.need(scan_base_temp_ints((int)(n > 0 ? n : 1)) * sizeof(int) + 256);
+    long pts_cap = M.planned ? P.npt[L] : 0, used = 0;
+    ScaPt* pts = (ScaPt*)V.pts.need((size_t)(pts_cap > 0 ? pts_cap : 1) * sizeof(ScaPt));
+    // trace each chunk once: count its points, place them behind the
+    // running offset, emit them
+    int nch = 0, last_cap = 0;
+    for (long r0 = 0; r0 < n; nch++) {
+      const int m = chunk_rays(ctl, n - r0, M.planned ? M.cap : engine_cap());
       TraceBufs tb = trace_bufs(st, 0);
       double* tpd = (double*)st.tpd.need((size_t)3 * m * sizeof(double));
-      last_cap = engine_trace(ctl, atm, sub(V.rays, r0, m), 1, tb, tpd, tpd + m, tpd + 2 * m, s);
+      RaySetDev C = sub(V.rays, (int)r0, m);
+      last_cap = trace_chunk(ctl, atm, C, tb, tpd, tpd + m, tpd + 2 * m, M, V.d_nray, (int)r0, s);
       launch_count_points(m, st.npts.as<int>(), st.rec.as<float>(), Rl, last_cap, count + r0, s);
-      r0 += m;
-    }
-    if (n == 0) {
-      V.npt = 0;
-    } else {
-      size_t tb = scan_temp_bytes(n);
-      launch_exclusive_scan(count, jbase, st.scan.need(tb), tb, n, s);
-      V.npt = read_int(jbase + n - 1, s) + read_int(count + n - 1, s);
-    }
-    ScaPt* pts = (ScaPt*)V.pts.need((size_t)(V.npt > 0 ? V.npt : 1) * sizeof(ScaPt));
-    // pass 2: emit the points (re-trace unless one chunk is still in the buffers)
-    for (int r0 = 0; r0 < n;) {
-      int m = chunk_rays(ctl, n - r0);
-      int cap = last_cap;
-      if (nchunks > 1) {
-        TraceBufs tb = trace_bufs(st, 0);
-        double* tpd = (double*)st.tpd.need((size_t)3 * m * sizeof(double));
-        cap = engine_trace(ctl, atm, sub(V.rays, r0, m), 1, tb, tpd, tpd + m, tpd + 2 * m, s);
+      launch_scan_base(count + r0, m, jbase + r0, scantmp, ctot + nch, ctot + nch + 1, s);
+      if (!M.planned) {
+        const long tot = read_int(ctot + nch + 1, s);
+        pts = (ScaPt*)need_keep(V.pts, (size_t)used * sizeof(ScaPt), (size_t)(tot + 1) * sizeof(ScaPt), s);
+        pts_cap = used = tot;
       }
-      RaySetDev C = sub(V.rays, r0, m);
-      launch_emit_points(m, st.npts.as<int>(), st.rec.as<float>(), Rl, cap, st.geo.as<double4>(),
-                         st.geo2.as<double4>(), st.gf.as<int>(), jbase + r0, C.time, C.atm_idx, pts, s);
+      launch_emit_points(m, st.npts.as<int>(), st.rec.as<float>(), Rl, last_cap, st.geo.as<double4>(),
+                         st.geo2.as<double4>(), st.gf.as<int>(), jbase + r0, C.time, C.atm_idx, pts, pts_cap,
+                         status, s);
       r0 += m;
     }
-    // surrogate / training-pair collection at the primaries' points
+    V.d_npt = ctot + nch;
+    V.npt = M.planned ? P.npt[L] : used;
+    if (act_npt) (*act_npt)[L] = used;
+    // surrogate / training-pair collection at the primaries' points (eager)
     const ScaPt* fan_pts = pts;
-    int nfan = V.npt;
+    long nfan = V.npt;
     if (L == 0 && (use_sur || collect) && V.npt > 0) {
       const long nrow = (long)V.npt * nd;
       // radiance summary of every point: the extinction-only radiances of
       // its zenith and nadir pencils (the fan's first and last directions)
       const std::vector<double> ud_ang = {0.0, M_PI};
       upload_vec(st.ud_alpha, ud_ang, s);
-      st.ud.alloc(2 * V.npt);
-      launch_children(V.npt, 2, pts, st.ud_alpha.as<double>(), st.ud.arrays(), s);
+      st.ud.alloc(2 * (int)V.npt);
+      launch_children((int)V.npt, 2, pts, st.ud_alpha.as<double>(), st.ud.arrays(), s);
       float* ud_rad = (float*)st.ud_rad.need((size_t)nd * 2 * V.npt * sizeof(float));
       float* ud_tau = (float*)st.ud_tau.need((size_t)nd * 2 * V.npt * sizeof(float));
-      integrate_set(ctl, atm, st.ud.view(), 0, nullptr, nullptr, ud_rad, ud_tau, nullptr, nullptr, nullptr, s);
+      integrate_set(ctl, atm, st.ud.view(), 0, nullptr, nullptr, ud_rad, ud_tau, nullptr, nullptr, nullptr, s, M);
       pencils += 2L * V.npt;
       FeatArgs F;
       memset(&F, 0, sizeof F);
       F.ud = ud_rad;
-      F.npt = V.npt;
+      F.npt = (int)V.npt;
       F.nd = nd;
       F.pts = pts;
       F.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw, atm.geo};
@@ -324,7 +377,7 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
         F.x = (uint16_t*)st.fx.need((size_t)nrow * MLP_INP * 2);
         F.lnb = (float*)st.flnb.need((size_t)nrow * sizeof(float));
         F.out_hull = flag;
-        launch_zero_int(flag, V.npt, s);
+        launch_zero_int(flag, (int)V.npt, s);
       }
       launch_features(F, s);
       st.ncollect = collect ? nrow : 0;
@@ -343,34 +396,41 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
         engine_set_stat("mlp_ms", mms);
         engine_set_stat("mlp_samples", (double)nrow);
         int* pos = (int*)st.fpos.need((size_t)(V.npt + 1) * sizeof(int));
-        size_t tb = scan_temp_bytes(V.npt);
-        launch_exclusive_scan(flag, pos, st.scan.need(tb), tb, V.npt, s);
-        nfan = read_int(pos + V.npt - 1, s) + read_int(flag + V.npt - 1, s);
+        int* tot = (int*)st.fnfan.need(2 * sizeof(int));
+        int* tmp = (int*)st.scan.need(scan_base_temp_ints((int)V.npt) * sizeof(int) + 256);
+        launch_scan_base(flag, (int)V.npt, pos, tmp, nullptr, tot, s);
+        nfan = read_int(tot, s);
         ScaPt* sub_pts = (ScaPt*)st.fpts.need((size_t)(nfan + 1) * sizeof(ScaPt));
-        launch_fallback_compact(V.npt, flag, pos, pts, (int*)st.fsub.need((size_t)(V.npt + 1) * sizeof(int)),
+        launch_fallback_compact((int)V.npt, flag, pos, pts, (int*)st.fsub.need((size_t)(V.npt + 1) * sizeof(int)),
                                 sub_pts, s);
         fan_pts = sub_pts;
         engine_set_stat("surrogate_points", (double)(V.npt - nfan));
         engine_set_stat("fallback_points", (double)nfan);
       }
     }
-    // secondary fans -> level L+1
+    // secondary fans -> level L+1 (planned: over the points' capacity, the
+    // device-side count decides which rays exist)
     Level& W = lv[L + 1];
-    long nchild = (long)nfan * (dir3d ? DIR3D_N : nalpha);
+    const long nchild = nfan * (dir3d ? DIR3D_N : nalpha);
     if (nchild > 0x7fffffffL) JUR_FAIL("Too many queue items!");
     W.own.alloc((int)nchild);
+    int* d_nchild = (int*)W.d_nchild.need(2 * sizeof(int));
     if (dir3d)
-      launch_children3d(nfan, fan_pts, W.own.arrays(), s);
+      launch_children3d((int)nfan, fan_pts, W.own.arrays(), s);
     else
-      launch_children(nfan, nalpha, fan_pts, st.alpha.as<double>(), W.own.arrays(), s);
+      launch_children((int)nfan, nalpha, fan_pts, st.alpha.as<double>(), W.own.arrays(), s,
+                      M.planned ? V.d_npt : nullptr, d_nchild);
     W.rays = W.own.view();
+    W.d_nray = M.planned ? d_nchild : nullptr;
     pencils += nchild;
-    // solar rays of the points
-    int* valid = (int*)V.sun_valid.need((size_t)(V.npt + 1) * sizeof(int));
-    int* pos = (int*)V.sun_pos.need((size_t)(V.npt + 1) * sizeof(int));
-    double* theta = (double*)V.sun_theta.need((size_t)(V.npt + 1) * sizeof(double));
-    int* sidx = (int*)V.sun_idx.need((size_t)(V.npt + 1) * sizeof(int));
-    V.sun_tmp.alloc(V.npt);
+    // solar rays of the points, compacted
+    const long np_ = V.npt;
+    int* valid = (int*)V.sun_valid.need((size_t)(np_ + 1) * sizeof(int));
+    int* pos = (int*)V.sun_pos.need((size_t)(np_ + 1) * sizeof(int));
+    double* theta = (double*)V.sun_theta.need((size_t)(np_ + 1) * sizeof(double));
+    int* sidx = (int*)V.sun_idx.need((size_t)(np_ + 1) * sizeof(int));
+    int* d_nsun = (int*)V.d_nsun.need(2 * sizeof(int));
+    V.sun_tmp.alloc((int)np_);
     SunArgs SA;
     SA.atm = {atm.z, atm.p, atm.t, atm.q, atm.k, atm.nlev, ctl->ng, ctl->nw, atm.geo};
     SA.rayds = ctl->rayds;
@@ -381,39 +441,41 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     SA.altimax = ag.altimax;
     SA.altimin = ag.altimin;
     SA.status = status;
-    launch_sun(V.npt, pts, SA, valid, theta, V.sun_tmp.arrays(), s);
-    V.nsun = 0;
-    if (V.npt > 0) {
-      size_t tb = scan_temp_bytes(V.npt);
-      launch_exclusive_scan(valid, pos, st.scan.need(tb), tb, V.npt, s);
-      V.nsun = read_int(pos + V.npt - 1, s) + read_int(valid + V.npt - 1, s);
-    }
-    launch_sun_compact(V.npt, valid, pos, sidx, s);
-    V.sun.alloc(V.nsun);
-    launch_gather_rays(V.npt, valid, pos, V.sun_tmp.arrays(), V.sun.arrays(), s);
+    launch_sun((int)np_, pts, SA, valid, theta, V.sun_tmp.arrays(), s, M.planned ? V.d_npt : nullptr);
+    int* tmp = (int*)st.scan.need(scan_base_temp_ints((int)(np_ > 0 ? np_ : 1)) * sizeof(int) + 256);
+    launch_scan_base(valid, (int)np_, pos, tmp, nullptr, d_nsun, s);
+    V.nsun = M.planned ? np_ : read_int(d_nsun, s);
+    if (act_nsun) (*act_nsun)[L] = V.nsun;
+    launch_sun_compact((int)np_, valid, pos, sidx, s);
+    V.sun.alloc((int)V.nsun);
+    launch_gather_rays((int)np_, valid, pos, V.sun_tmp.arrays(), V.sun.arrays(), s);
     pencils += V.nsun;
   }
-  int st_h = read_int(status, s);
-  if (st_h & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
-  if (st_h & JUR_ST_NLOS) JUR_FAIL("Too many LOS points!");
+  if (!M.planned && S > 0) {
+    const int st_h = read_int(status, s);
+    if (st_h & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
+    if (st_h & JUR_ST_NLOS) JUR_FAIL("Too many LOS points!");
+  }
+  if (pencils_eager) *pencils_eager = pencils;
 
   // ---- integrate bottom-up ---------------------------------------------------
-  long leaves = lv[S].rays.nray;
   {
     Level& V = lv[S];
     float* rad = (float*)V.rad.need((size_t)nd * (V.rays.nray + 1) * sizeof(float));
     float* tau = (float*)V.tau.need((size_t)nd * (V.rays.nray + 1) * sizeof(float));
-    integrate_set(ctl, atm, V.rays, 0, nullptr, nullptr, rad, tau, nullptr, nullptr, nullptr, s);
+    integrate_set(ctl, atm, V.rays, 0, nullptr, nullptr, rad, tau, nullptr, nullptr, nullptr, s, M, V.d_nray);
   }
   for (int L = S - 1; L >= 0; L--) {
     Level& V = lv[L];
     float* srad = (float*)V.srad.need((size_t)nd * (V.nsun + 1) * sizeof(float));
     float* stau = (float*)V.stau.need((size_t)nd * (V.nsun + 1) * sizeof(float));
-    integrate_set(ctl, atm, V.sun.view(), 0, nullptr, nullptr, srad, stau, nullptr, nullptr, nullptr, s);
-    leaves += V.nsun;
+    integrate_set(ctl, atm, V.sun.view(), 0, nullptr, nullptr, srad, stau, nullptr, nullptr, nullptr, s, M,
+                  M.planned ? V.d_nsun.as<int>() : nullptr);
     float* J = (float*)V.J.need((size_t)(V.npt + 1) * nd * sizeof(float));
     AngArgs G;
-    G.npt = V.npt;
+    memset(&G, 0, sizeof G);
+    G.npt = (int)V.npt;
+    G.d_npt = M.planned ? V.d_npt : nullptr;
     G.nd = nd;
     G.pts = V.pts.as<ScaPt>();
     G.alpha = st.alpha.as<double>();
@@ -424,16 +486,11 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
     G.sun_idx = V.sun_idx.as<int>();
     G.sun_theta = V.sun_theta.as<double>();
     G.tau_sun = stau;
-    G.nsun = V.nsun;
+    G.nsun = (int)V.nsun;
     G.theta_grid = st.theta_grid.as<double>();
     G.bsun = st.bsun.as<double>();
     G.J = J;
-    G.sub_idx = nullptr;
-    G.ymlp = nullptr;
-    G.lnb = nullptr;
-    G.ymu = 0.f;
     G.ysd = 1.f;
-    G.jth = nullptr;
     if (L == 0 && use_sur && V.npt > 0) {
       G.sub_idx = st.fsub.as<int>();
       G.ymlp = st.fy.as<float>();
@@ -448,31 +505,209 @@ void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ra
       launch_angular(G, s);
     if (L == 0) {
       integrate_set(ctl, atm, V.rays, S, J, V.jbase.as<int>(), out.rad, out.tau, out.tpz, out.tplon, out.tplat, s,
-                    out.rad64, out.tau64, out.ld);
+                    M, nullptr, out.rad64, out.tau64, out.ld, V.npt);
       if (out.npts) {
         // LOS points of the primaries (of the last chunk's trace when chunked)
-        if (V.rays.nray <= chunk_rays(ctl, V.rays.nray))
+        if (V.rays.nray <= chunk_rays(ctl, V.rays.nray, M.planned ? M.cap : engine_cap()))
           HIP_OK(hipMemcpyAsync(out.npts, st.npts.p, V.rays.nray * sizeof(int), hipMemcpyDeviceToDevice, s));
       }
     } else {
       float* rad = (float*)V.rad.need((size_t)nd * (V.rays.nray + 1) * sizeof(float));
       float* tau = (float*)V.tau.need((size_t)nd * (V.rays.nray + 1) * sizeof(float));
-      integrate_set(ctl, atm, V.rays, S - L, J, V.jbase.as<int>(), rad, tau, nullptr, nullptr, nullptr, s);
+      integrate_set(ctl, atm, V.rays, S - L, J, V.jbase.as<int>(), rad, tau, nullptr, nullptr, nullptr, s, M,
+                    V.d_nray, nullptr, nullptr, 0, V.npt);
     }
   }
-  HIP_OK(hipEventRecord(e1, s));
-  HIP_OK(hipEventSynchronize(e1));
-  float ms = 0;
-  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-  engine_set_stat("scatter_ms", ms);
+}
+
+// The plan's identity: rays, depth, fan, channels, emitters.
+std::string plan_key(const ctl_t* ctl, const RaySetDev& rays, int S, int nalpha) {
+  char k[256];
+  snprintf(k, sizeof k, "%d|%d|%d|%d|%d|%d|%d", rays.nray, S, nalpha, ctl->nd, ctl->ng, ctl->nw, ctl->sca_n);
+  return k;
+}
+
+// Everything a recorded planned call depends on: the caller's arrays, the
+// engine's tables, layer geometry and buffers, and every buffer of the tree.
+std::string tree_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, const OutDev& out, int S,
+                     hipStream_t s) {
+  State& st = ST();
+  std::string k = engine_call_key(ctl, atm, rays, out, S, s) + "|" + st.plan.key;
+  char b[64];
+  auto add = [&](const void* p) {
+    snprintf(b, sizeof b, "%p,", p);
+    k += b;
+  };
+  for (const DBuf* d : {&st.geo, &st.geo2, &st.rec, &st.npts, &st.tsurf, &st.gf, &st.scan, &st.alpha,
+                        &st.theta_grid, &st.bsun, &st.phase, &st.status, &st.tpd})
+    add(d->p);
+  for (Level& V : st.lv) {
+    for (const DBuf* d : {&V.count, &V.jbase, &V.ctot, &V.J, &V.pts, &V.sun_valid, &V.sun_pos, &V.sun_theta,
+                          &V.sun_idx, &V.d_nsun, &V.rad, &V.tau, &V.srad, &V.stau, &V.d_nchild})
+      add(d->p);
+    for (RayBuf* r : {&V.own, &V.sun_tmp, &V.sun}) {
+      add(r->time.p);
+      add(r->atm.p);
+    }
+  }
+  return k;
+}
+
+void drop_tree_graph() {
+  State& st = ST();
+  if (st.g_exec) HIP_OK(hipGraphExecDestroy(st.g_exec));
+  if (st.g_graph) HIP_OK(hipGraphDestroy(st.g_graph));
+  st.g_exec = nullptr;
+  st.g_graph = nullptr;
+  st.g_key.clear();
+}
+
+// After a planned call (or replay) has completed: false when it outgrew a
+// capacity; the reference's error exits; the call's counts as statistics.
+bool planned_done(int S, long nray, hipStream_t s) {
+  State& st = ST();
+  std::vector<int> h(1 + 3 * S, 0);
+  HIP_OK(hipMemcpyAsync(&h[0], st.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  for (int L = 0; L < S; L++) {
+    Level& V = st.lv[L];
+    HIP_OK(hipMemcpyAsync(&h[1 + 3 * L], V.d_npt, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&h[2 + 3 * L], V.d_nsun.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&h[3 + 3 * L], st.lv[L + 1].d_nchild.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  if (h[0] & JUR_ST_BELOW_SURFACE) JUR_FAIL("Observer below surface!");
+  if (h[0] & JUR_ST_NLOS) JUR_FAIL("Too many LOS points!");
+  if (h[0] & (JUR_ST_PLAN | JUR_ST_CAP)) return false;
+  long pencils = nray, leaves = 0, npts = 0;
+  for (int L = 0; L < S; L++) {
+    pencils += (long)h[3 + 3 * L] + h[2 + 3 * L];
+    leaves += h[2 + 3 * L];
+    npts += h[1 + 3 * L];
+  }
+  leaves += S > 0 ? h[3 + 3 * (S - 1)] : nray;
   engine_set_stat("pencils", (double)pencils);
   engine_set_stat("leaf_pencils", (double)leaves);
-  long npts0 = 0;
-  for (int L = 0; L < S; L++) npts0 += lv[L].npt;
+  engine_set_stat("scattering_points", (double)npts);
+  engine_set_stat("scattering_points0", S > 0 ? (double)h[1] : 0.0);
+  return true;
+}
+
+}  // namespace
+
+void scatter_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays, int S, const OutDev& out,
+                     hipStream_t s) {
+  State& st = ST();
+  // IP 2 / 3: srcfunc_sca_3d's direct fans (scatter.c:654-657)
+  const bool dir3d = ctl->ip != 1;
+  // surrogate for the thermal source at the primaries' points (option
+  // "surrogate", once a network is trained or loaded); "collect" keeps the
+  // plain method's (features, thermal J) pairs of those points; the
+  // surrogate learns srcfunc_sca_1d's source: vertical profiles only
+  const bool use_sur = engine_option_int("surrogate", 0) && mlp_trained() && !dir3d;
+  const bool collect = engine_option_int("collect", 0) != 0 && !dir3d;
+  st.ncollect = 0;
+  engine_set_stat("surrogate_points", 0);
+  engine_set_stat("fallback_points", 0);
+  // pencils per fan (option "fan", default the reference's 28)
+  const int nalpha = engine_option_int("fan", NALPHA_REF);
+  if (nalpha < NALPHA_MIN || nalpha > NALPHA || nalpha % 2) JUR_FAIL("Unsupported number of secondary pencils!");
+  if (st.nalpha != nalpha) {
+    std::vector<double> alpha(nalpha);
+    alpha_grid(alpha.data(), nalpha);
+    upload_vec(st.alpha, alpha, s);
+    st.nalpha = nalpha;
+    drop_tree_graph();
+  }
+  int* status = (int*)st.status.need(2 * sizeof(int));
+  // device-resident calls: option "sca_plan" (default 1), the plain method
+  // on vertical profiles in FP32
+  const bool plannable = !dir3d && !use_sur && !collect && engine_option_int("sca_plan", 1) &&
+                         !engine_option_int("fp64", 0);
+  const bool graph = plannable && engine_option_int("graph", 0);
+  const std::string pkey = plan_key(ctl, rays, S, nalpha);
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  auto finish = [&](const char* how) {
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    engine_set_stat("scatter_ms", ms);
+    engine_set_stat("sca_mode", how[0] == 'e' ? 0 : (how[0] == 'p' ? 1 : 2));
+    HIP_OK(hipEventDestroy(e0));
+    HIP_OK(hipEventDestroy(e1));
+  };
+  if (plannable && st.plan.key == pkey) {
+    const Mode M{true, st.plan.cap, status};
+    if (graph && st.g_exec && st.g_key == tree_key(ctl, atm, rays, out, S, s)) {
+      HIP_OK(hipEventRecord(e0, s));
+      HIP_OK(hipGraphLaunch(st.g_exec, s));
+      HIP_OK(hipEventRecord(e1, s));
+      if (planned_done(S, rays.nray, s)) {
+        finish("graph");
+        engine_set_stat("graph_ms", engine_stat("scatter_ms"));
+        return;
+      }
+      drop_tree_graph();
+    } else {
+      HIP_OK(hipEventRecord(e0, s));
+      launch_zero_int(status, 1, s);
+      engine_set_mode(true, false);
+      run_tree(ctl, atm, rays, S, out, s, M, nalpha, false, false, false, nullptr, nullptr, nullptr);
+      engine_set_mode(false, false);
+      HIP_OK(hipEventRecord(e1, s));
+      if (planned_done(S, rays.nray, s)) {
+        finish("planned");
+        if (graph) {
+          // record the same launch sequence for the calls that follow
+          drop_tree_graph();
+          engine_set_mode(true, true);
+          HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          launch_zero_int(status, 1, s);
+          run_tree(ctl, atm, rays, S, out, s, M, nalpha, false, false, false, nullptr, nullptr, nullptr);
+          HIP_OK(hipStreamEndCapture(s, &st.g_graph));
+          engine_set_mode(false, false);
+          HIP_OK(hipGraphInstantiate(&st.g_exec, st.g_graph, nullptr, nullptr, 0));
+          st.g_key = tree_key(ctl, atm, rays, out, S, s);
+        }
+        return;
+      }
+    }
+    // a count or a LOS outgrew the plan: run the call eagerly and plan again
+    const double n_over = engine_stat("plan_overflows");
+    engine_set_stat("plan_overflows", (std::isfinite(n_over) ? n_over : 0.0) + 1);
+  }
+  HIP_OK(hipEventRecord(e0, s));
+  launch_zero_int(status, 1, s);
+  std::vector<long> act_npt(S, 0), act_nsun(S, 0);
+  long pencils = 0;
+  run_tree(ctl, atm, rays, S, out, s, Mode{false, 0, status}, nalpha, use_sur, collect, dir3d, &act_npt, &act_nsun,
+           &pencils);
+  HIP_OK(hipEventRecord(e1, s));
+  HIP_OK(hipEventSynchronize(e1));
+  finish("eager");
+  long leaves = st.lv[S].rays.nray, npts0 = 0;
+  for (int L = 0; L < S; L++) {
+    leaves += act_nsun[L];
+    npts0 += act_npt[L];
+  }
+  engine_set_stat("pencils", (double)pencils);
+  engine_set_stat("leaf_pencils", (double)leaves);
   engine_set_stat("scattering_points", (double)npts0);
-  engine_set_stat("scattering_points0", (double)lv[0].npt);
-  HIP_OK(hipEventDestroy(e0));
-  HIP_OK(hipEventDestroy(e1));
+  engine_set_stat("scattering_points0", S > 0 ? (double)act_npt[0] : 0.0);
+  if (plannable) {
+    // capacities of the planned calls: these sizes with a margin
+    State::Plan& P = st.plan;
+    P.key = pkey;
+    P.cap = engine_cap();
+    P.nray.assign(S + 1, 0);
+    P.npt.assign(S, 0);
+    P.nray[0] = rays.nray;
+    for (int L = 0; L < S; L++) {
+      P.npt[L] = act_npt[L] + act_npt[L] / 4 + 64;
+      P.nray[L + 1] = P.npt[L] * nalpha;
+    }
+    drop_tree_graph();
+  }
 }
 
 long scatter_collected(float* feat, float* jth, long cap) {

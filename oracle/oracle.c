@@ -1405,6 +1405,92 @@ void oracle_gauher(double *x, double *w) {
   }
 }
 
+/* The aerosol configuration's Mie grid (oracle_set_mie_grid(1); not a
+   reference function, defined by this project for that configuration):
+   64 radii log-spaced over 0.01-20 um with the trapezoid rule in ln r over
+   the log-normal density, and the phase function evaluated at 180 angles
+   (k + 0.5) deg, k = 0..179, by the same Bohren-Huffman series as
+   oracle_bhmie (its mirror symmetry about 90 deg over the 90 angles of the
+   half range), then resampled onto the 181 nodes i deg of the aero_t layout:
+   node i the mean of the cells (i - 0.5) and (i + 0.5) deg, the end nodes
+   the end cells.  Q_ext from the forward amplitude
+   S(0) = sum (2n + 1)/2 (a_n + b_n); Q_sca as in bhmie. */
+static int o_mie_grid = 0;
+void oracle_set_mie_grid(int grid) { o_mie_grid = grid ? 1 : 0; }
+#define OMIE_NR 64
+#define OMIE_NA 180
+
+void oracle_bhmie180(double x, double n_real, double n_imag, double *phase180, double *phase, double *qext,
+                     double *qsca) {
+  enum { NH = OMIE_NA / 2 };
+  static __thread ocx cxd[10001];
+  ocx cxref = cx(n_real, n_imag), cxy = cx_mul(cx(x, 0.0), cxref);
+  const double xstop = x + 4.E0 * pow(x, 0.3333) + 2.0;
+  const int nstop = (int)xstop;
+  const double ymod = cx_abs(cxy);
+  const int nmx = (int)((xstop > ymod ? xstop : ymod) + 15);
+  if (nmx > 10000) OFAIL("Too many Mie terms!");
+  double amu[NH], pi0[NH], pi1[NH];
+  ocx s1[NH], s2[NH], s1m[NH], s2m[NH], sf = cx(0, 0);
+  for (int j = 0; j < NH; j++) {
+    amu[j] = cos((j + 0.5) * M_PI / 180.);
+    pi0[j] = 0;
+    pi1[j] = 1;
+    s1[j] = s2[j] = s1m[j] = s2m[j] = cx(0, 0);
+  }
+  cxd[nmx] = cx(0, 0);
+  for (int n = 1; n <= nmx - 1; n++) {
+    double rn = nmx - n + 1;
+    ocx t = cx_add(cxd[nmx - n + 1], cx_div(cx(rn, 0.0), cxy));
+    t = cx_div(cx(1.0, 0.0), t);
+    cxd[nmx - n] = cx_sub(cx_div(cx(rn, 0.0), cxy), t);
+  }
+  double psi0 = cos(x), psi1 = sin(x), chi0 = -sin(x), chi1 = cos(x), qs = 0;
+  ocx xi1 = cx(psi1, -chi1);
+  for (int n = 1; n <= nstop; n++) {
+    const double rn = n, fn = (2. * rn + 1.) / (rn * (rn + 1.));
+    const double psi = (2. * rn - 1.) * psi1 / x - psi0, chi = (2. * rn - 1.) * chi1 / x - chi0;
+    const ocx xi = cx(psi, -chi);
+    ocx a = cx_add(cx_div(cxd[n], cxref), cx(rn / x, 0.0));
+    ocx an = cx_div(cx_sub(cx_mul(a, cx(psi, 0.0)), cx(psi1, 0.0)), cx_sub(cx_mul(a, xi), xi1));
+    ocx b = cx_add(cx_mul(cxref, cxd[n]), cx(rn / x, 0.0));
+    ocx bn = cx_div(cx_sub(cx_mul(b, cx(psi, 0.0)), cx(psi1, 0.0)), cx_sub(cx_mul(b, xi), xi1));
+    qs += (2. * rn + 1.) * (cx_abs(an) * cx_abs(an) + cx_abs(bn) * cx_abs(bn));
+    sf = cx_add(sf, cx_mul(cx((2. * rn + 1.) / 2., 0.0), cx_add(an, bn)));
+    const double p = (n & 1) ? 1.0 : -1.0, t = -p; /* (-1)^(n-1), (-1)^n */
+    for (int j = 0; j < NH; j++) {
+      const double pi = pi1[j], tau = rn * amu[j] * pi - (rn + 1.) * pi0[j];
+      s1[j] = cx_add(s1[j], cx_mul(cx(fn, 0.0), cx_add(cx_mul(an, cx(pi, 0.0)), cx_mul(bn, cx(tau, 0.0)))));
+      s2[j] = cx_add(s2[j], cx_mul(cx(fn, 0.0), cx_add(cx_mul(an, cx(tau, 0.0)), cx_mul(bn, cx(pi, 0.0)))));
+      s1m[j] = cx_add(s1m[j],
+                      cx_mul(cx(fn, 0.0), cx_add(cx_mul(an, cx(pi * p, 0.0)), cx_mul(bn, cx(tau * t, 0.0)))));
+      s2m[j] = cx_add(s2m[j],
+                      cx_mul(cx(fn, 0.0), cx_add(cx_mul(an, cx(tau * t, 0.0)), cx_mul(bn, cx(pi * p, 0.0)))));
+      pi1[j] = ((2. * rn + 1.) * amu[j] * pi - (rn + 1.) * pi0[j]) / rn;
+      pi0[j] = pi;
+    }
+    psi0 = psi1;
+    psi1 = psi;
+    chi0 = chi1;
+    chi1 = chi;
+    xi1 = cx(psi1, -chi1);
+  }
+  *qsca = (2. / (x * x)) * qs;
+  *qext = (4. / (x * x)) * sf.re;
+  double v[OMIE_NA];
+  const double f = 2 / (x * x * (*qsca));
+  for (int j = 0; j < NH; j++) {
+    v[j] = f * (s1[j].re * s1[j].re + s1[j].im * s1[j].im + s2[j].re * s2[j].re + s2[j].im * s2[j].im);
+    v[OMIE_NA - 1 - j] =
+        f * (s1m[j].re * s1m[j].re + s1m[j].im * s1m[j].im + s2m[j].re * s2m[j].re + s2m[j].im * s2m[j].im);
+  }
+  if (phase180)
+    for (int k = 0; k < OMIE_NA; k++) phase180[k] = v[k];
+  phase[0] = v[0];
+  phase[NTHETA - 1] = v[OMIE_NA - 1];
+  for (int i = 1; i < NTHETA - 1; i++) phase[i] = 0.5 * (v[i - 1] + v[i]);
+}
+
 /* opt_prop_mie_log (scatter.c:422-508): one log-normal mode. */
 static void o_mie_log(ctl_t *ctl, aero_t *aero, int count, double *beta_ext, double *beta_sca,
                       double phase[NDMAX][NTHETA]) {
@@ -1436,17 +1522,33 @@ static void o_mie_log(ctl_t *ctl, aero_t *aero, int count, double *beta_ext, dou
     oracle_gauher(zs, weights);
   }
   double K1 = aero->nn[count] * 1e-3 * sqrt(M_PI), qext, qsca, qphase[NTHETA];
-  for (int nn = 0; nn < NRAD; ++nn) {
-    double rad = exp(sqrt(2) * log(aero->ss[count]) * zs[nn] + log(aero->rr[count]));
+  const int nrad = o_mie_grid ? OMIE_NR : NRAD;
+  for (int nn = 0; nn < nrad; ++nn) {
+    double rad, wn;
+    if (o_mie_grid) {
+      /* r_i log-spaced over 0.01-20 um; the Gauss-Hermite form's weight
+         becomes exp(-z^2) dz with z = (ln r - ln r_m) / (sqrt 2 ln s) and
+         the trapezoid step d ln r */
+      const double l0 = log(0.01), dl = (log(20.) - l0) / (OMIE_NR - 1), sg = sqrt(2) * log(aero->ss[count]);
+      rad = exp(l0 + nn * dl);
+      const double z = (log(rad) - log(aero->rr[count])) / sg;
+      wn = exp(-z * z) * dl / sg * ((nn == 0 || nn == OMIE_NR - 1) ? 0.5 : 1.0);
+    } else {
+      rad = exp(sqrt(2) * log(aero->ss[count]) * zs[nn] + log(aero->rr[count]));
+      wn = weights[nn];
+    }
     if (rad >= 0.001 && rad <= 1000. && K1 > 0.) {
       for (int id = 0; id < ctl->nd; id++) {
         double lambda = 1. / (ctl->nu[id]) * pow(10, 4.);
         double x = 2 * M_PI * rad / lambda;
-        oracle_bhmie(x, nr_c[id], ni_c[id], qphase, &qext, &qsca);
-        beta_ext[id] += K1 * pow(rad, 2) * qext * weights[nn];
-        beta_sca[id] += K1 * pow(rad, 2) * qsca * weights[nn];
+        if (o_mie_grid)
+          oracle_bhmie180(x, nr_c[id], ni_c[id], NULL, qphase, &qext, &qsca);
+        else
+          oracle_bhmie(x, nr_c[id], ni_c[id], qphase, &qext, &qsca);
+        beta_ext[id] += K1 * pow(rad, 2) * qext * wn;
+        beta_sca[id] += K1 * pow(rad, 2) * qsca * wn;
         for (int jj = 0; jj < NTHETA; ++jj)
-          phase[id][jj] += K1 * qsca * pow(rad, 2) * qphase[jj] * weights[nn];
+          phase[id][jj] += K1 * qsca * pow(rad, 2) * qphase[jj] * wn;
       }
     }
   }
@@ -1613,10 +1715,16 @@ void oracle_suncoord(double sec, double lon, double lat, double *azi, double *sz
    construction gives increasing angles in [0, 180] deg. */
 static int o_nalpha = 28;
 void oracle_set_fan(int nalpha) {
-  if (nalpha < 10 || nalpha > 28 || nalpha % 2) OFAIL("Unsupported number of secondary pencils!");
+  if (nalpha < 10 || nalpha > 32 || nalpha % 2) OFAIL("Unsupported number of secondary pencils!");
   o_nalpha = nalpha;
 }
 
+/* The reference's construction for nalpha <= 28.  30 and 32 pencils (the
+   cirrus configuration asks for 32, for which the reference's steps would
+   leave 0-180 deg) use a grid defined by this project: the reference's
+   horizon band (83-90 deg, 1 deg steps) and its first four growing steps
+   on either side (92, 95, 98, 103, 111 deg and 81, 78, 75, 70, 62 deg), then
+   nalpha/2 - 10 equal steps on each side to 180 and 0 deg. */
 static void o_alpha_grid(double *alpha, int nalpha) {
   int n1 = 1, n2 = 2;
   double midang = 83, up = 92, down = 81, step = 1 + 2;
@@ -1625,7 +1733,8 @@ static void o_alpha_grid(double *alpha, int nalpha) {
     alpha[ir] = midang * M_PI / 180.;
     midang++;
   }
-  for (int ir = 0; ir < nalpha / 2 - 5; ++ir) {
+  const int nref = nalpha <= 28 ? nalpha / 2 - 5 : 5;
+  for (int ir = 0; ir < nref; ++ir) {
     alpha[nalpha / 2 + ir + 4] = up * M_PI / 180.;
     alpha[nalpha / 2 - ir - 5] = down * M_PI / 180.;
     up += step;
@@ -1636,7 +1745,19 @@ static void o_alpha_grid(double *alpha, int nalpha) {
       n2 = (int)step;
     }
   }
+  if (nalpha > 28) {
+    const int m = nalpha / 2 - 10;  /* equal steps after 111 / 62 deg */
+    for (int k = 1; k <= m; k++) {
+      alpha[nalpha / 2 + 8 + k] = (111. + k * (180. - 111.) / (m + 1)) * M_PI / 180.;
+      alpha[nalpha / 2 - 9 - k] = (62. - k * 62. / (m + 1)) * M_PI / 180.;
+    }
+  }
   alpha[nalpha - 1] = M_PI;
+}
+
+void oracle_alpha_grid(int nalpha, double *alpha) {
+  if (nalpha < 10 || nalpha > 32 || nalpha % 2) OFAIL("Unsupported number of secondary pencils!");
+  o_alpha_grid(alpha, nalpha);
 }
 
 static void o_pencil(ctl_t *ctl, const atm_t *atm, const aero_t *aero, oray_t *ray,
@@ -1650,7 +1771,7 @@ static int o_inner = 1;
 static void o_srcfunc_sca_1d(ctl_t *ctl, const atm_t *atm, const aero_t *aero, double time,
                              const double *x, const double *dx, int il, double *src_sca,
                              int scattering) {
-  enum { NALPHA = 28 };
+  enum { NALPHA = 32 };
   const int nalpha = o_nalpha;
   double alpha[NALPHA], dnorth[3], ek[3], lx[3], ly[3], lz[3], sx[3], sy[3], sz[3],
          theta[NTHETA], wsum[NDMAX];

@@ -176,8 +176,19 @@ void oracle_srcfunc_sca_sun(ctl_t *ctl, const atm_t *atm, const aero_t *aero, do
    (formod_fov, forwardmodel.c:156-220; FOV shape file ctl->fov). */
 void oracle_formod_fov(ctl_t *ctl, obs_t *obs);
 
-/* Secondary pencils per scattering point (reference: 28); even, 10-28. */
+/* Secondary pencils per scattering point (reference: 28); even, 10-32
+   (30 and 32: this project's grid for the cirrus configuration, see
+   oracle.c), and the fan's zenith angles [rad]. */
 void oracle_set_fan(int nalpha);
+void oracle_alpha_grid(int nalpha, double *alpha);
+/* 1: the aerosol configuration's Mie grid (64 log-spaced radii 0.01-20 um,
+   phase function at 180 angles resampled to the 181-node layout); 0: the
+   reference's (170 Gauss-Hermite nodes, 181 angles). */
+void oracle_set_mie_grid(int grid);
+/* One sphere at the 180 angles (k + 0.5) deg (phase180, optional) and
+   resampled to the 181 nodes (phase), with Q_ext from the forward amplitude. */
+void oracle_bhmie180(double x, double n_real, double n_imag, double *phase180, double *phase, double *qext,
+                     double *qsca);
 
 /* Drop the cached tables, filter functions and Planck tables. */
 void oracle_reset(void);

@@ -52,6 +52,9 @@ def load():
         "oracle_kernel_columns": (None, [P, P, P, P, D, C.c_size_t, C.c_size_t, C.c_size_t,
                                          C.POINTER(C.c_int), C.c_int]),
         "oracle_set_fan": (None, [C.c_int]),
+        "oracle_alpha_grid": (None, [C.c_int, D]),
+        "oracle_set_mie_grid": (None, [C.c_int]),
+        "oracle_bhmie180": (None, [C.c_double, C.c_double, C.c_double, D, D, D, D]),
         "oracle_read_ret": (None, [C.c_int, P, P, P]),
         "oracle_srcfunc_sca": (None, [P, P, P, C.c_double, D, D, C.c_int, D, C.c_int]),
         "oracle_cov_apr": (None, [P, P, P, P, D]),

@@ -36,6 +36,11 @@ CASES = {
     "aerosol_limb_64": ("aerosol_limb_8192", 256, lambda w: _crossing(w, 20.0, 64, 0), None, None, 16, ()),
     # cirrus, scattering depth 2, the reference's 28 pencils per fan
     "cirrus_8": ("cirrus_4096", 256, lambda w: _crossing(w, 12.0, 4, 4), 8, None, 28, ()),
+    # the configurations' own parameters: the aerosol layer's Mie grid (64
+    # log-spaced radii, 180 phase angles) and the cirrus's 32 pencils per fan
+    "aerosol_limb_64_cfg": ("aerosol_limb_8192", 256, lambda w: _crossing(w, 20.0, 64, 0), None, None, 16,
+                            (("mie_grid", 1),)),
+    "cirrus_8_cfg": ("cirrus_4096", 256, lambda w: _crossing(w, 12.0, 4, 4), 8, None, 32, ()),
 }
 
 

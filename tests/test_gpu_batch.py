@@ -133,7 +133,7 @@ def test_unsupported_fan_exits(tmp_path):
         "import sys; sys.path[:0] = %r\n"
         "from pathlib import Path\n"
         "from pyjurassic import lib as L, synth; from helpers import gpu_batch\n"
-        "lib = L.load(); lib.jur_set_option(b'fan', 32)\n"
+        "lib = L.load(); lib.jur_set_option(b'fan', 34)\n"
         "w = synth.make('aerosol_limb_8192', Path(%r), nray=2)\n"
         "gpu_batch(lib, w, Path(%r) / 'g.ctl')\n" % (sys.path[:4], str(tmp_path), str(tmp_path)))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)

@@ -7,7 +7,10 @@ digest is checked first):
   all 32 channels, 5 emitters, single scattering by day, 16 pencils per
   fan (the configuration's count);
 * cirrus_8: 4 limb and 4 nadir rays through the cirrus at 9-12 km,
-  8 channels, 5 emitters, scattering depth 2 by day, 28 pencils per fan.
+  8 channels, 5 emitters, scattering depth 2 by day, 28 pencils per fan;
+* aerosol_limb_64_cfg, cirrus_8_cfg: the same at the configurations' own
+  parameters (the aerosol layer's Mie grid of 64 log-spaced radii and 180
+  phase angles, option mie_grid; 32 pencils per fan for the cirrus).
 
 Bars as for the other batched tests: radiance within 1e-4 relative,
 transmittance within 1e-5 absolute, tangent points within 1e-6 km, LOS

@@ -125,6 +125,25 @@ def test_mie_layer_optics_vs_oracle(glib, olib, in_examples):
         np.testing.assert_allclose(pb, pa, rtol=1e-11)
 
 
+def test_mie_grid_layer_optics_vs_oracle(glib, olib, in_examples):
+    """Option mie_grid (the aerosol configuration's 64 log-spaced radii and
+    180 phase angles) against the oracle's mie_grid on the six modes of
+    aero1.tab: the device's 180-angle series, resampling and radius rule."""
+    olib.oracle_set_mie_grid(1)
+    glib.jur_set_option(b"mie_grid", 1)
+    try:
+        ctl, _, _, aero = read_example(olib, "oracle_", "aero1")
+        ctl2, _, _, aero2 = read_example(glib, "", "aero1")
+    finally:
+        olib.oracle_set_mie_grid(0)
+        glib.jur_set_option(b"mie_grid", 0)
+    assert glib.jur_stat(b"mie_spheres") > 0
+    for il in range(aero.nl):
+        for f in ["beta_e", "beta_s", "beta_a"]:
+            assert getattr(aero2, f)[il][0] == pytest.approx(getattr(aero, f)[il][0], rel=1e-12), (f, il)
+        np.testing.assert_allclose(np.array(aero2.p[il][0]), np.array(aero.p[il][0]), rtol=1e-11)
+
+
 def test_bhmie_vs_oracle(glib, olib):
     for x, nr, ni in [(0.05, 1.33, 0.0), (2.5, 1.5, 0.01), (40.0, 1.31, 0.3), (785.0, 1.45, 0.2)]:
         pa, pb = (C.c_double * 181)(), (C.c_double * 181)()

@@ -56,6 +56,7 @@ def write_opaque_table(path):
 
 @pytest.mark.parametrize("workload,zlayer,depth,fan", [("aerosol_limb_8192", 17.0, 1, 28),
                                                        ("aerosol_limb_8192", 17.0, 1, 16),
+                                                       ("cirrus_4096", 10.5, 1, 32),
                                                        ("cirrus_4096", 10.5, 2, 10)])
 def test_isothermal_opaque_radiance_is_planck(olib, tmp_path, workload, zlayer, depth, fan):
     w = synth.make(workload, tmp_path, nray=2)
