@@ -378,7 +378,7 @@ def bench_jacobian(args, lib):
     rays = (n + 1) * nr
     # the last batched call holds every perturbed copy (one group)
     alg_bytes = ipts[-1] * nd * (128 * ng + 8) + lib.jur_stat(b"pencils") * nd * 8
-    path_avg = float(np.mean(path_timed))
+    path_avg = float(np.mean(path_ms))
     achieved = alg_bytes / (path_avg * 1e-3) / 1e9
     line = {
         "metric": "Jacobian: ray x channel radiances/s over the n+1 forward models of kernel()",
