@@ -71,7 +71,11 @@ MEASURED_TRAFFIC = {
     ("clear_batch_65536", 65536, 6, 4): (1.9911e+11, "profiles/k_path_fast_clear_batch_65536.md"),
 }
 # The same for the FP64 path kernel of kernel() on the Jacobian workload.
-MEASURED_TRAFFIC_JACOBIAN = (1.0259e+11, "profiles/k_path64_jacobian_limb_17.md")
+# HBM bytes per kernel() call of the path kernels (FETCH_SIZE / WRITE_SIZE
+# passes): with base-ray reuse (the default), and integrating every copy
+# from the observer (option kernel_prefix 0)
+MEASURED_TRAFFIC_JACOBIAN = {1: (3.072e+10, "profiles/k_path64_jacobian_limb_17.md"),
+                             0: (1.0259e+11, "profiles/k_path64_jacobian_limb_17.md")}
 
 
 def parse():
@@ -376,10 +380,20 @@ def bench_jacobian(args, lib):
     dt = time.perf_counter() - t0
     nd, nr = ctl.nd, obs.nr
     rays = (n + 1) * nr
-    # the last batched call holds every perturbed copy (one group)
-    alg_bytes = ipts[-1] * nd * (128 * ng + 8) + lib.jur_stat(b"pencils") * nd * 8
+    # the last batched call holds every perturbed copy (one group); with
+    # base-ray reuse, the (point, emitter) evaluations actually made
+    ept = lib.jur_stat(b"emitter_points")
+    reuse = bool(np.isfinite(ept))
+    evals = ept if reuse else ipts[-1] * ng
+    pencils = lib.jur_stat(b"pencils")
+    alg_bytes = evals * nd * 128 + ipts[-1] * nd * 8 + pencils * nd * 8
     path_avg = float(np.mean(path_ms))
     achieved = alg_bytes / (path_avg * 1e-3) / 1e9
+    # the bytes every copy integrated from the observer would move, over the
+    # same time: what the reuse buys
+    full_pts = lib.jur_stat(b"reuse_all_points") if reuse else ipts[-1]
+    full_bytes = full_pts * nd * (128 * ng + 8) + pencils * nd * 8
+    traffic = MEASURED_TRAFFIC_JACOBIAN[1 if reuse else 0]
     line = {
         "metric": "Jacobian: ray x channel radiances/s over the n+1 forward models of kernel()",
         "value": rays * nd * steps / dt,
@@ -394,9 +408,18 @@ def bench_jacobian(args, lib):
                    "forward_rays_per_kernel": rays, "columns_per_s": n * steps / dt,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_JACOBIAN[0],
-                     "traffic_source": MEASURED_TRAFFIC_JACOBIAN[1], "kernel": "k_path64",
-                     "kernel_ms": path_avg, "alg_bytes_per_call": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic[0], "traffic_source": traffic[1],
+                     "kernel": ("k_path64 (reuse pass) + k_path64e (unperturbed rays)" if reuse
+                                else "k_path64"),
+                     "kernel_ms": path_avg, "alg_bytes_per_call": alg_bytes,
+                     "full_work_bytes_per_call": full_bytes,
+                     "full_work_frac": full_bytes / (path_avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "integrated_points": ipts[-1], "emitter_points": evals,
+                     "reuse_wave_iterations": {k: lib.jur_stat(("reuse_wave_" + k).encode())
+                                               for k in ("points", "emitters", "optics")},
+                     "reuse_waves_by_emitters": [[lib.jur_stat(b"reuse_waves_e%d" % e),
+                                                  lib.jur_stat(b"reuse_wave_points_e%d" % e)]
+                                                 for e in range(ng + 1)]},
     }
     if not args.no_cpu:
         import oracle_ctypes
@@ -509,16 +532,19 @@ def bench_retrieval(args, lib):
     torch.cuda.synchronize()
     steps = args.steps
     tot = lambda: np.array([lib.jur_stat(k) for k in (b"calls_total", b"path_ms_total",
-                                                      b"integrated_points_total", b"pencils_total")])
+                                                      b"integrated_points_total", b"pencils_total",
+                                                      b"emitter_points_total")])
     s0 = tot()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     dt = time.perf_counter() - t0
-    calls, path_ms, ipts, pencils = (tot() - s0) / steps
+    calls, path_ms, ipts, pencils, evals = (tot() - s0) / steps
     costs = [ln.split() for ln in (tmp / "costs.tab").read_text().splitlines() if ln and ln[0].isdigit()]
     nd, ng = ctl.nd, ctl.ng
-    alg_bytes = ipts * nd * (128 * ng + 8) + pencils * nd * 8
+    # (point, emitter) evaluations made (the Jacobians reuse the unperturbed
+    # rays' emitter chains)
+    alg_bytes = evals * nd * 128 + ipts * nd * 8 + pencils * nd * 8
     achieved = alg_bytes / (path_ms * 1e-3) / 1e9
     line = {
         "metric": "retrievals/s (optimal_estimation: forward models, Jacobians, linear algebra, outputs)",

@@ -221,17 +221,25 @@ int jur_init(int device);
    resolves; optimal_estimation's forward calls follow the same setting),
    "fp64_split" (FP64 path with a ray's emitters on separate lanes: -1, the
    default, for calls too small to fill the GPU with one lane per ray; 1
-   always; 0 never; results are identical), "kernel_prefix" (1: kernel
-   integrates each perturbed copy's rays from their first LOS point whose
-   records differ from the unperturbed ray's, starting from its path state
-   there -- bit-identical, 30-45 % fewer points on the reference examples,
-   but slower on the GPU as measured (the wave's lanes no longer walk the
-   same points of copies of one ray); default 0).
+   always; 0 never; results are identical), "kernel_prefix" (default 1:
+   kernel integrates each perturbed copy's rays from their first LOS point
+   whose FP64 records differ from the unperturbed ray's, starting from its
+   path state there, and takes each emitter's factor 1 - eps and the
+   point's clear-air optics from the unperturbed ray up to the first point
+   whose inputs to them differ -- bit-identical; 0: every copy integrated
+   from the observer), "reuse_group" (default 1: the copies are ordered so
+   that a wave holds copies of one ray that compute the same emitters).
    Returns 0. */
 int jur_set_option(const char *key, double value);
 /* Statistics of the last call: "path_ms" (path-integration kernel time,
    summed over its launches), "path_launches", "integrated_points" (LOS
-   points integrated, all pencils), "trace_ms" (clear-air ray tracing),
+   points integrated, all pencils), "emitter_points" (point x emitter
+   evaluations made when kernel_prefix reused an unperturbed ray, else NaN),
+   "reuse_all_points" (LOS points of all the call's rays, reused or not),
+   "reuse_wave_points", "reuse_wave_emitters", "reuse_wave_optics" (that
+   pass's iterations as its waves run them, per channel),
+   "reuse_waves_e<k>", "reuse_wave_points_e<k>" (its waves computing k
+   emitters, k = 0..8, and their point iterations), "trace_ms" (clear-air ray tracing),
    "pencils" (primaries + secondary + solar pencils), "leaf_pencils",
    "scattering_points", "scatter_ms" (whole scattering call),
    "tuned_variant", "tuned_wpb", "tuned_ms" (autotuner choice),
@@ -243,7 +251,8 @@ int jur_set_option(const char *key, double value);
    "oe_total_ms" (wall time of the last optimal_estimation call: its
    forward models, its Jacobians, in total); running totals over all calls since
    the library loaded: "calls_total", "path_ms_total",
-   "integrated_points_total", "pencils_total".  A replayed graph reports
+   "integrated_points_total", "emitter_points_total" (point x emitter
+   evaluations), "pencils_total".  A replayed graph reports
    "graph_ms" (the whole call; it also adds that to "path_ms_total") and no
    per-kernel times ("path_ms", "trace_ms" NaN).
    Unknown keys give NaN. */

@@ -17,7 +17,9 @@ namespace {
 
 struct Engine {
   int prefix_nr = 0;          // FP64 prefix reuse: rays are copies of the first prefix_nr
-  DBuf pstart, prem, pids, snap;
+  DBuf pstart, prem, pids, snap, fdiff, fcont, pevals, pkeys, ev_acc;
+  int ev_ng = -1;  // >= 0: the call reused base rays, ev_acc holds its emitter evaluations
+  int call_ng = 0;  // emitters of the call's path integration
   bool ready = false;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -622,6 +624,7 @@ namespace {
 void begin_call(hipStream_t s) {
   Engine& e = E();
   e.path_ev_used = 0;
+  e.ev_ng = -1;
   launch_zero_u64((unsigned long long*)e.pts_acc.need(sizeof(unsigned long long)), s);
 }
 // After the stream has reached the end of the call: path-kernel time and
@@ -634,17 +637,33 @@ void end_call(hipStream_t s) {
     HIP_OK(hipEventElapsedTime(&t, e.path_ev[i].first, e.path_ev[i].second));
     ms += t;
   }
-  unsigned long long pts = 0;
+  unsigned long long pts = 0, ev[24] = {0};
   HIP_OK(hipMemcpyAsync(&pts, e.pts_acc.p, sizeof pts, hipMemcpyDeviceToHost, s));
+  if (e.ev_ng >= 0) HIP_OK(hipMemcpyAsync(ev, e.ev_acc.p, sizeof ev, hipMemcpyDeviceToHost, s));
   HIP_OK(hipStreamSynchronize(s));
   // a replayed graph has no per-launch events
   e.stats["path_ms"] = e.path_ev_used ? ms : NAN;
   e.stats["path_launches"] = (double)e.path_ev_used;
   e.stats["integrated_points"] = (double)pts;
+  // (point, emitter) EGA evaluations made: with base-ray reuse the copies'
+  // own and every point of the bases; else NaN (every integrated point
+  // evaluates every emitter)
+  e.stats["emitter_points"] = e.ev_ng >= 0 ? (double)ev[0] + (double)e.ev_ng * (double)ev[1] : NAN;
+  // the reuse pass as its waves run it (per channel): point, emitter and
+  // clear-air optics iterations
+  e.stats["reuse_wave_points"] = e.ev_ng >= 0 ? (double)ev[2] : NAN;
+  e.stats["reuse_wave_emitters"] = e.ev_ng >= 0 ? (double)ev[3] : NAN;
+  e.stats["reuse_wave_optics"] = e.ev_ng >= 0 ? (double)ev[4] : NAN;
+  e.stats["reuse_all_points"] = e.ev_ng >= 0 ? (double)ev[23] : NAN;  // LOS points of every ray
+  for (int k = 0; k <= 8; k++) {
+    e.stats["reuse_waves_e" + std::to_string(k)] = e.ev_ng >= 0 ? (double)ev[5 + 2 * k] : NAN;
+    e.stats["reuse_wave_points_e" + std::to_string(k)] = e.ev_ng >= 0 ? (double)ev[6 + 2 * k] : NAN;
+  }
   // running totals over calls (a retrieval makes many)
   e.stats["calls_total"] += 1;
   e.stats["path_ms_total"] += e.path_ev_used ? ms : engine_stat("scatter_ms");
   e.stats["integrated_points_total"] += (double)pts;
+  e.stats["emitter_points_total"] += e.ev_ng >= 0 ? e.stats["emitter_points"] : (double)e.call_ng * (double)pts;
   e.stats["pencils_total"] += e.stats["pencils"];
 }
 }  // namespace
@@ -768,6 +787,7 @@ bool replay_forward(const RaySetDev& rays, hipStream_t s) {
   e.stats["calls_total"] += 1;
   e.stats["path_ms_total"] += ms;
   e.stats["integrated_points_total"] += (double)pts;
+  e.stats["emitter_points_total"] += (double)e.call_ng * (double)pts;
   e.stats["pencils_total"] += rays.nray;
   return true;
 }
@@ -882,20 +902,37 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
                       float* tau, int out_ld, int out_off, hipStream_t s, double* rad64, double* tau64,
                       long nJ) {
   Engine& e = E();
+  e.call_ng = ctl->ng;
   int* order = (int*)e.order.need(nray * sizeof(int));
-  // FP64 prefix reuse (engine_set_prefix): clear air, FP64 records of this
-  // trace, rays = copies of the first pnr, snapshot buffer of modest size
+  // FP64 reuse of base rays (engine_set_prefix): clear air, FP64 records of
+  // this trace, rays = copies of the first pnr, snapshot buffer of modest
+  // size
   const int R_ = rec_len(ctl->ng, ctl->nw);
   const bool fp64 = engine_option_int("fp64", 0) != 0;
   const int pnr = e.prefix_nr;
-  const size_t snap_bytes = (size_t)(pnr > 0 ? pnr : 0) * ctl->nd * (size_t)(cap + 1) * (ctl->ng + 2) * sizeof(double);
+  const size_t snap_bytes =
+      (size_t)(pnr > 0 ? pnr : 0) * ctl->nd * (size_t)(cap + 1) * (2 * ctl->ng + 4) * sizeof(double);
   const bool prefix = fp64 && scattering == 0 && pnr > 0 && nray > pnr && nray % pnr == 0 && !e.capturing &&
                       e.last_rec64 && e.last_nray == nray && e.last_cap == cap && snap_bytes < ((size_t)1 << 30);
-  int *pstart = nullptr, *prem = nullptr;
+  int *pstart = nullptr, *prem = nullptr, *fdiff = nullptr, *fcont = nullptr, *pevals = nullptr, *pkeys = nullptr;
   if (prefix) {
     pstart = (int*)e.pstart.need(nray * sizeof(int));
     prem = (int*)e.prem.need(nray * sizeof(int));
-    launch_prefix_start(nray, pnr, R_, cap, e.last_rec64, npts, pstart, prem, s);
+    fdiff = (int*)e.fdiff.need((size_t)nray * (ctl->ng > 0 ? ctl->ng : 1) * sizeof(int));
+    fcont = (int*)e.fcont.need(nray * sizeof(int));
+    pevals = (int*)e.pevals.need(nray * sizeof(int));
+    pkeys = (int*)e.pkeys.need(nray * sizeof(int));
+    // group a wave's lanes by base ray and the emitters they compute (option
+    // "reuse_group", default 1; the key packs (class, base) into 19 bits)
+    const int keyed = engine_option_int("reuse_group", 1) != 0 && (long)(NGMAX + 2) * pnr < (1L << 19);
+    // emitters whose column amounts enter the continua
+    unsigned long long cmask = 0;
+    for (const char* name : {"CO2", "H2O"}) {
+      const int ig = find_emitter(ctl, name);
+      if (ig >= 0) cmask |= 1ull << ig;
+    }
+    launch_reuse_start(nray, pnr, R_, ctl->ng, ctl->nw, cap, cmask, e.last_rec64, npts, pstart, prem, fdiff, fcont,
+                       pevals, pkeys, keyed, s);
   }
   if (e.capturing || e.planned) {
     // counting sort: kernels only (no library temporaries or memsets)
@@ -906,8 +943,8 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     size_t tb = sort_temp_bytes(nray);
     void* tmp = e.sorttmp.need(tb);
     launch_iota(ids, nray, s);
-    // by the points each ray integrates (all, or those after its prefix)
-    launch_sort_rays(prefix ? prem : npts, nullptr, keys, ids, order, tmp, tb, nray, s);
+    // by the points each ray integrates, or for base-ray reuse by the key above
+    launch_sort_rays(prefix ? pkeys : npts, nullptr, keys, ids, order, tmp, tb, nray, s);
   }
   PathArgs P = path_args(ctl, nray, cap, rec, npts, tsurf, order, scattering, J, jbase, rad, tau, out_ld, out_off,
                          rad64, tau64);
@@ -967,6 +1004,15 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   if (prefix) {
     launch_sum_points(prem, nray, acc, s);
     launch_sum_points(npts, pnr, acc, s);
+    // emitter evaluations: the copies' (those not taken from the base) and
+    // the bases' (all emitters at every point, added on the host)
+    unsigned long long* ev = (unsigned long long*)e.ev_acc.need(24 * sizeof(unsigned long long));
+    launch_zero_u64(ev, s, 24);
+    launch_sum_points(pevals, nray, ev, s);
+    launch_sum_points(npts, pnr, ev + 1, s);
+    launch_reuse_waves(order, pstart, npts, fdiff, fcont, nray, ctl->ng, ev + 2, s);
+    launch_sum_points(npts, nray, ev + 23, s);
+    e.ev_ng = ctl->ng;  // read at the end of the call
   } else {
     launch_sum_points(npts, nray, acc, s);
   }
@@ -997,6 +1043,8 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
     P.snap_mode = 2;
     P.pnr = pnr;
     P.pstart = pstart;
+    P.fdiff = fdiff;
+    P.fcont = fcont;
     P.snap = B.snap;
   }
   launch_path(P, wpb, s);

@@ -140,9 +140,10 @@ void run_copies(ctl_t* ctl, const atm_t* atm, const obs_t* obs, const aero_t* ae
   engine_prepare(ctl);
   engine_set_aero(ctl, ctl->sca_n > 0 ? aero : nullptr);
   // the copies' rays repeat copy 0's up to the first point a perturbation
-  // reaches (option "kernel_prefix", default 0: measured slower, see the
-  // option's description in jurassic_amd.h)
-  engine_set_prefix(C > 1 && engine_option_int("kernel_prefix", 0) ? nr : 0);
+  // reaches, and each emitter's chain and the clear-air optics up to the
+  // first point whose inputs to them differ: integrated from copy 0's path
+  // state and factors (option "kernel_prefix", default 1)
+  engine_set_prefix(C > 1 && engine_option_int("kernel_prefix", 1) ? nr : 0);
   engine_forward(ctl, A, R, ctl->sca_mult, O, s);
   engine_set_prefix(0);
   double *rad, *tau;

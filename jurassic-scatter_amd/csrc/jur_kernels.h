@@ -68,13 +68,19 @@ struct PathArgs {
   int cpw;                 // kernel variant (channels per wave x occupancy), see launch_path
   int fp64;                // 1: FP64 arithmetic (k_path64); also writes rad64/tau64 if set
   int fp64_split;          // FP64: emitters on separate lanes (k_path64e): 1 on, 0 off, -1 small calls
-  // FP64 prefix reuse (clear air): rays [0, pnr) are bases, ray i >= pnr
-  // repeats base i % pnr up to its first differing LOS point.  snap_mode 1:
-  // the (base) rays write their path state before every point and after the
-  // last into snap [pnr][nd][cap + 1][ng + 2] (tau_path per emitter, rad,
-  // tau); 2: every ray starts at pstart[ray] from its base's state there.
+  // FP64 reuse of base rays (clear air): rays [0, pnr) are bases, ray i >=
+  // pnr repeats base i % pnr up to its first differing LOS point.
+  // snap_mode 1: the (base) rays write their path state before every point
+  // and after the last, each emitter's factor 1 - eps and the point's
+  // clear-air optics at every point, into snap [pnr][nd][cap + 1][2 ng + 4]
+  // (tau_path per emitter, rad, tau, factor per emitter, exp(-beta ds), B);
+  // 2: every ray starts at pstart[ray] from its base's state there, takes
+  // emitter g's factor from the base before point fdiff[ray][g] and the
+  // optics before point fcont[ray].
   int snap_mode, pnr;
   const int* pstart;
+  const int* fdiff;
+  const int* fcont;
   double* snap;
   double *rad64, *tau64;   // [nd][out_ld] (optional)
   const double* rec64;     // FP64 records / surface temperatures (optional, fp64)
@@ -96,14 +102,25 @@ size_t sort_temp_bytes(int n);
 void launch_sort_rays(const int* np, int* keys_tmp, int* keys_out, int* ids_in, int* ids_out,
                       void* temp, size_t temp_bytes, int n, hipStream_t s);
 void launch_iota(int* ids, int n, hipStream_t s);
-// First LOS point at which ray i >= pnr differs from ray i % pnr (FP64
-// records, point counts); start[i] (np[i] for the bases) and the points left,
-// rem[i] = np[i] - start[i].
-void launch_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,
-                         int* rem, hipStream_t s);
+// Where ray i >= pnr starts to differ from ray i % pnr (FP64 records, point
+// counts): start[i] (any field; np[i] for the bases), the points left
+// rem[i] = np[i] - start[i], per emitter fdiff[i][g] (p, T or u_g), the
+// emitter evaluations left evals[i], the first point fcont[i] whose
+// clear-air optics inputs differ (every field but the column amounts of
+// emitters outside cmask, the continuum emitters), and the sort key key[i]
+// (keyed: grouped by the emitters computed and the base ray; else rem).
+void launch_reuse_start(int nray, int pnr, int R, int ng, int nw, int cap, unsigned long long cmask,
+                        const double* rec64, const int* np, int* start, int* rem, int* fdiff, int* fcont, int* evals,
+                        int* key, int keyed, hipStream_t s);
+// Iterations of the reuse pass per wave of the sorted order (points,
+// emitter evaluations, clear-air optics), summed into acc[0..2]; waves and
+// point iterations by emitters computed into acc[3..20].
+void launch_reuse_waves(const int* order, const int* start, const int* np, const int* fdiff, const int* fcont,
+                        int nray, int ng, unsigned long long* acc, hipStream_t s);
 void launch_zero_int(int* p, int n, hipStream_t s);
 void launch_fill_f64(double* p, size_t n, double v, hipStream_t s);
 // Rays by descending point count into order (hist: nbins ints of scratch).
 void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, hipStream_t s);
 void launch_sum_points(const int* np, int n, unsigned long long* acc, hipStream_t s);
-void launch_zero_u64(unsigned long long* p, hipStream_t s);
+// p[0 .. n) = 0 (n <= 64) with a kernel (capturable, no memset node)
+void launch_zero_u64(unsigned long long* p, hipStream_t s, int n = 1);

@@ -685,9 +685,18 @@ __device__ inline int near_grid64(const double* xx, int n, double x, int h) {
 // intpol_tbl_u then intpol_tbl_eps of one (p, T) corner (forwardmodel.c:460-474).
 // HINT: strictly increasing columns; h holds the inverse lookup's bracket.
 // Per-lane hint slots in LDS: slot k of a lane at base[64 k].
+// (16-bit: brackets are below TBLNUMAX = 320)
 struct LaneHints {
-  int* base;
-  __device__ int& operator[](int k) const { return base[k * 64]; }
+  short* base;
+  struct Ref {
+    short* p;
+    __device__ operator int() const { return *p; }
+    __device__ Ref& operator=(int v) {
+      *p = (short)v;
+      return *this;
+    }
+  };
+  __device__ Ref operator[](int k) const { return Ref{base + k * 64}; }
 };
 // Hint slots in registers (one emitter per lane: 7 slots).
 struct RegHints {
@@ -695,8 +704,8 @@ struct RegHints {
   __device__ int& operator[](int k) const { return h[k]; }
 };
 
-template <bool HINT>
-__device__ inline double corner64(const float2* col, int n, double e_path, double du, int& h) {
+template <bool HINT, class H>
+__device__ inline double corner64(const float2* col, int n, double e_path, double du, H&& h) {
   const int i = HINT ? near_col64<true>(col, n, e_path, h) : locate_col64<true>(col, n, e_path);
   h = i;
   const double uu = lin_f(col[i].y, col[i].x, col[i + 1].y, col[i + 1].x, e_path) + du;
@@ -790,18 +799,38 @@ __device__ inline double planck64(const double* row, double t) {
 }
 
 // Radiance and transmittance update at one LOS point given the gas
-// segment transmittance tau_gas (forwardmodel.c:317-390).
-template <class Rec>
+// segment transmittance tau_gas (forwardmodel.c:317-390).  Clear-air
+// optics of the point, E = exp(-beta_x ds) and the source B: PM 0 computed;
+// PM 1 computed and written to eb[0], eb[1] (also where tau_gas = 0); PM 2
+// read from eb when use_eb (a base ray's, same inputs), else computed.
+template <int PM = 0, class Rec>
 __device__ __forceinline__ void point64(const PathArgs& A, const ChanConst64& cc, const double* prow, int id,
                                         int uo, const Rec& rec, int ip, double p, double t, double ds, double af,
-                                        double qh, double tau_gas, int& kj, double& rad, double& tau) {
-  const double uco2 = cc.co2_on ? rec(ip, uo + cc.co2_ig) : 0.0;
-  const double uh2o = cc.h2o_on ? rec(ip, uo + cc.h2o_ig) : 0.0;
-  const double kw = A.nw > 0 ? rec(ip, REC_K + cc.window) : 0.0;
-  const double beta = continua64(cc, p, t, ds, qh, uco2, uh2o, kw);
-  const double B = planck64(prow, t);
+                                        double qh, double tau_gas, int& kj, double& rad, double& tau,
+                                        double* eb = nullptr, bool use_eb = false) {
   const bool scpt = A.scattering > 0 && af > 0.0;
   const int ai = scpt || (A.sca_n > 0 && af != 0.0) ? (int)rec(ip, REC_AI) : 0;
+  double beta = 0.0, B, E = 0.0;
+  if (PM == 2 && use_eb) {
+    E = eb[0];
+    B = eb[1];
+  } else {
+    const double uco2 = cc.co2_on ? rec(ip, uo + cc.co2_ig) : 0.0;
+    const double uh2o = cc.h2o_on ? rec(ip, uo + cc.h2o_ig) : 0.0;
+    const double kw = A.nw > 0 ? rec(ip, REC_K + cc.window) : 0.0;
+    beta = continua64(cc, p, t, ds, qh, uco2, uh2o, kw);
+    B = planck64(prow, t);
+    if (PM != 0 && !scpt) {
+      double bx = beta;
+      if (A.sca_n > 0 && af != 0.0)
+        bx += af * (A.ext_beta_e ? A.beta_e64[ai * A.nd + id] : A.beta_a64[ai * A.nd + id]);
+      E = exp(-bx * ds);
+    }
+    if (PM == 1) {
+      eb[0] = E;
+      eb[1] = B;
+    }
+  }
   if (scpt) {
     // scattering source (forwardmodel.c:317-360); a zero-length segment
     // adds nothing (the reference forms -log(1)/0 there)
@@ -817,9 +846,13 @@ __device__ __forceinline__ void point64(const PathArgs& A, const ChanConst64& cc
     kj++;
   } else if (tau_gas > 0.0) {
     // gas, continua and aerosol extinction (forwardmodel.c:363-390)
-    double bx = beta;
-    if (A.sca_n > 0 && af != 0.0) bx += af * (A.ext_beta_e ? A.beta_e64[ai * A.nd + id] : A.beta_a64[ai * A.nd + id]);
-    const double eps = 1.0 - tau_gas * exp(-bx * ds);
+    if (PM == 0) {
+      double bx = beta;
+      if (A.sca_n > 0 && af != 0.0)
+        bx += af * (A.ext_beta_e ? A.beta_e64[ai * A.nd + id] : A.beta_a64[ai * A.nd + id]);
+      E = exp(-bx * ds);
+    }
+    const double eps = 1.0 - tau_gas * E;
     rad += B * eps * tau;
     tau *= 1.0 - eps;
   }
@@ -838,8 +871,11 @@ __device__ __forceinline__ void finish64(const PathArgs& A, const double* prow, 
   A.tau[o] = (float)tau;
 }
 
-template <int NGT, bool HINT, bool R64>
+// MODE 0: plain; 1: write the snapshots of base rays (snap_mode 1); 2: reuse
+// base rays (snap_mode 2).  Modes 1 and 2 read FP64 records.
+template <int NGT, bool HINT, bool R64, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_path64(PathArgs A) {
+  constexpr bool REUSE = MODE == 2, SNAP = MODE == 1;
   const int lane = threadIdx.x & 63;
   int chunk, group;
   if (!map_work(A, 1, chunk, group)) return;
@@ -854,8 +890,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
   double tau_path[NGT];
   // bracket hints of the previous point, 7 per emitter (p, T at both
   // pressures, four corners), in LDS
-  __shared__ int hint_lds[HINT ? 4 : 1][HINT ? NGT * 7 : 1][64];
-  int* hbase = &hint_lds[HINT ? (threadIdx.x >> 6) : 0][0][lane];
+  __shared__ short hint_lds[HINT ? 4 : 1][HINT ? NGT * 7 : 1][64];
+  short* hbase = &hint_lds[HINT ? (threadIdx.x >> 6) : 0][0][lane];
 #pragma unroll
   for (int g = 0; g < NGT; g++) {
     tau_path[g] = 1.0;
@@ -864,20 +900,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
   }
   double rad = 0.0, tau = 1.0;
   int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
-  // prefix reuse: the state of the base ray before point ip0
-  const int snw = A.ng + 2;
+  // reuse of a base ray's path (snap_mode 2): its state before point ip0,
+  // and per emitter the first point fd[g] whose inputs to that emitter's
+  // chain differ from the base's (before it, the base's factor 1 - eps)
+  const int snw = 2 * A.ng + 4;
   double* sp = nullptr;
-  int ip0 = 0;
-  if (A.snap_mode && valid) {
+  int ip0 = 0, fc = 0;
+  int fd[NGT];
+#pragma unroll
+  for (int g = 0; g < NGT; g++) fd[g] = 0;
+  if (MODE != 0 && valid) {
     sp = A.snap + ((size_t)(ray % A.pnr) * A.nd + id) * (size_t)(A.cap + 1) * snw;
-    if (A.snap_mode == 2) {
+    if (REUSE) {
       ip0 = A.pstart[ray];
       const double* st = sp + (size_t)ip0 * snw;
 #pragma unroll
       for (int g = 0; g < NGT; g++)
-        if (g < A.ng) tau_path[g] = st[g];
+        if (g < A.ng) {
+          tau_path[g] = st[g];
+          fd[g] = A.fdiff[(size_t)ray * A.ng + g];
+        }
       rad = st[A.ng];
       tau = st[A.ng + 1];
+      fc = A.fcont[ray];
     }
   }
   auto save = [&](int ip) {
@@ -888,26 +933,52 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     st[A.ng] = rad;
     st[A.ng + 1] = tau;
   };
+  // copies of one base ray share a wave (sorted so): its lanes walk the same
+  // point index, each from its own first point
+  int lo = ip0, hi = npts;
+  if (REUSE) {
+    lo = valid ? ip0 : 0x7fffffff;
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = min(lo, __shfl_xor(lo, o, 64));
+      hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+  }
   // point records: the FP64 copy when the trace wrote one (R64)
   const size_t rbase = (size_t)ray * A.cap * A.R;
   auto rec = [&](int ip, int f) -> double {
     const size_t i = rbase + (size_t)ip * A.R + f;
     return R64 ? A.rec64[i] : (double)A.rec[i];
   };
-  for (int ip = ip0; ip < npts; ip++) {
-    if (A.snap_mode == 1) save(ip);
+  for (int ip = lo; ip < hi; ip++) {
+    if (REUSE && (ip < ip0 || ip >= npts)) continue;
+    if (SNAP) save(ip);
     const double p = rec(ip, REC_P), t = rec(ip, REC_T), ds = rec(ip, REC_DS), af = rec(ip, REC_AF);
     const double qh = rec(ip, REC_QH2O);
+    const double* st = REUSE ? sp + (size_t)ip * snw + A.ng + 2 : nullptr;  // the base's factors here
     double tau_gas = 1.0;
 #pragma unroll
     for (int g = 0; g < NGT; g++)
-      if (g < A.ng)
-        tau_gas *= 1.0 - ega64<HINT>(A.T, id * A.ng + g, p, t, rec(ip, uo + g), tau_path[g],
-                                     LaneHints{hbase + g * 7 * 64});
-    point64(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau);
+      if (g < A.ng) {
+        double f;
+        if (REUSE && ip < fd[g]) {
+          // the base's chain: the same factor, the same path transmittance
+          f = st[g];
+          tau_path[g] *= f;
+        } else {
+          f = 1.0 - ega64<HINT>(A.T, id * A.ng + g, p, t, rec(ip, uo + g), tau_path[g],
+                                LaneHints{hbase + g * 7 * 64});
+        }
+        if (SNAP) sp[(size_t)ip * snw + A.ng + 2 + g] = f;
+        tau_gas *= f;
+      }
+    if (MODE == 0)
+      point64(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau);
+    else
+      point64<MODE>(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau,
+                    sp + (size_t)ip * snw + 2 * A.ng + 2, ip < fc);
   }
   if (!valid) return;
-  if (A.snap_mode == 1) save(npts);
+  if (SNAP) save(npts);
   finish64(A, prow, id, ray, rad, tau);
 }
 
@@ -945,39 +1016,138 @@ __global__ void __launch_bounds__(256) k_path64e(PathArgs A) {
     return A.rec64 ? A.rec64[i] : (double)A.rec[i];
   };
   const int gbase = lane & ~7;
+  // snap_mode 1: the path state before every point and after the last, and
+  // each emitter's factor 1 - eps (the layout of k_path64)
+  const int snw = 2 * A.ng + 4;
+  double* sp = (A.snap_mode == 1 && valid) ? A.snap + ((size_t)(ray % A.pnr) * A.nd + id) * (size_t)(A.cap + 1) * snw
+                                           : nullptr;
+  auto save = [&](int ip) {
+    double* st = sp + (size_t)ip * snw;
+    if (gl) st[sub] = tau_path;
+    if (sub == 0) {
+      st[A.ng] = rad;
+      st[A.ng + 1] = tau;
+    }
+  };
   for (int ip = 0; ip < npts; ip++) {
+    if (sp) save(ip);
     const double p = rec(ip, REC_P), t = rec(ip, REC_T), ds = rec(ip, REC_DS), af = rec(ip, REC_AF);
     const double qh = rec(ip, REC_QH2O);
     double f = 1.0;
     if (gl) f = 1.0 - ega64<HINT>(A.T, id * A.ng + sub, p, t, rec(ip, uo + sub), tau_path, RegHints{hv});
+    if (sp && gl) sp[(size_t)ip * snw + A.ng + 2 + sub] = f;
     double tau_gas = 1.0;
     for (int g = 0; g < A.ng; g++) tau_gas *= __shfl(f, gbase + g, 64);
-    point64(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau);
+    if (sp) {
+      // every lane of the ray computes E and B; lane 0 writes them
+      double eb[2];
+      point64<1>(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau, eb);
+      if (sub == 0) {
+        sp[(size_t)ip * snw + 2 * A.ng + 2] = eb[0];
+        sp[(size_t)ip * snw + 2 * A.ng + 3] = eb[1];
+      }
+    } else {
+      point64(A, cc, prow, id, uo, rec, ip, p, t, ds, af, qh, tau_gas, kj, rad, tau);
+    }
   }
+  if (sp) save(npts);
   if (!valid || sub != 0) return;
   finish64(A, prow, id, ray, rad, tau);
 }
 
-__global__ void k_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,
-                               int* rem) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Reuse of base rays (ray i >= pnr repeats ray i % pnr up to where its FP64
+// records differ): one wave per ray compares 64 points at a time.  start[i]:
+// the first point with any field different (or the shorter point count);
+// fdiff[i][g]: the first point where emitter g's inputs (p, T, u_g) differ;
+// rem[i] = np[i] - start[i]; evals[i] = sum_g (np[i] - fdiff[i][g]).  Bases:
+// start = fdiff = np, rem = evals = 0.  key[i] (sort key, descending):
+// keyed != 0: ((class * pnr + base) << 12) | min(start, 4095), class 0 when
+// no emitter differs, 1 + g when only emitter g does, ng + 1 otherwise, so
+// that a wave holds copies of one base ray that compute the same emitters;
+// keyed == 0: rem.
+__global__ void __launch_bounds__(256) k_reuse_start(int nray, int pnr, int R, int ng, int uo, int cap,
+                                                     unsigned long long cmask, const double* rec64, const int* np,
+                                                     int* start, int* rem, int* fdiff, int* fcont, int* evals,
+                                                     int* key, int keyed) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= nray) return;
   const int n = np[i];
-  int f = n;
-  if (i >= pnr) {
-    const int b = i % pnr, nb = np[b], nmin = n < nb ? n : nb;
-    const double* ri = rec64 + (size_t)i * cap * R;
-    const double* rb = rec64 + (size_t)b * cap * R;
-    f = 0;
-    for (; f < nmin; f++) {
-      bool same = true;
-      for (int k = 0; k < R && same; k++)
-        same = __double_as_longlong(ri[(size_t)f * R + k]) == __double_as_longlong(rb[(size_t)f * R + k]);
-      if (!same) break;
+  if (i < pnr) {
+    if (lane < ng) fdiff[(size_t)i * ng + lane] = n;
+    if (lane == 0) {
+      start[i] = n;
+      fcont[i] = n;
+      rem[i] = 0;
+      evals[i] = 0;
+      key[i] = keyed ? ((i << 12) | min(n, 4095)) : 0;
+    }
+    return;
+  }
+  const int b = i % pnr, nb = np[b], nmin = n < nb ? n : nb;
+  const double* ri = rec64 + (size_t)i * cap * R;
+  const double* rb = rec64 + (size_t)b * cap * R;
+  int f_any = nmin, f_c = nmin;
+  int f_g = nmin;  // lane g < ng: emitter g's first differing point
+  unsigned long long open = (ng < 64 ? (1ull << ng) - 1 : ~0ull);  // emitters not yet found
+  bool any_open = true, c_open = true;
+  for (int f0 = 0; f0 < nmin && (any_open || c_open || open); f0 += 64) {
+    const int f = f0 + lane;
+    bool d_any = false, d_pt = false, d_c = false;
+    unsigned long long d_u = 0;  // bit g: u_g differs
+    if (f < nmin) {
+      const double* a = ri + (size_t)f * R;
+      const double* c = rb + (size_t)f * R;
+      for (int k = 0; k < R; k++) {
+        const bool d = __double_as_longlong(a[k]) != __double_as_longlong(c[k]);
+        d_any |= d;
+        if (k == REC_P || k == REC_T) d_pt |= d;
+        if (k >= uo && d) d_u |= 1ull << (k - uo);
+        // the point's clear-air optics: every field but the column
+        // amounts of emitters without a continuum
+        if (k < uo || ((cmask >> (k - uo)) & 1ull)) d_c |= d;
+      }
+    }
+    if (any_open) {
+      const unsigned long long m = __ballot(d_any);
+      if (m) {
+        f_any = f0 + __builtin_ctzll(m);
+        any_open = false;
+      }
+    }
+    if (c_open) {
+      const unsigned long long m = __ballot(d_c);
+      if (m) {
+        f_c = f0 + __builtin_ctzll(m);
+        c_open = false;
+      }
+    }
+    for (int g = 0; g < ng; g++) {
+      if (!((open >> g) & 1ull)) continue;
+      const unsigned long long m = __ballot(d_pt || ((d_u >> g) & 1ull));
+      if (m) {
+        if (lane == g) f_g = f0 + __builtin_ctzll(m);
+        open &= ~(1ull << g);
+      }
     }
   }
-  start[i] = f;
-  rem[i] = n - f;
+  // a differing emitter input is a differing field: f_g >= f_any
+  int ev = lane < ng ? n - f_g : 0, cnt = lane < ng && f_g < n ? 1 : 0;
+  int who = lane < ng && f_g < n ? lane : 0;
+  for (int o = 32; o > 0; o >>= 1) {
+    ev += __shfl_xor(ev, o, 64);
+    cnt += __shfl_xor(cnt, o, 64);
+    who = max(who, __shfl_xor(who, o, 64));
+  }
+  if (lane < ng) fdiff[(size_t)i * ng + lane] = f_g;
+  if (lane == 0) {
+    start[i] = f_any;
+    fcont[i] = f_c;
+    rem[i] = n - f_any;
+    evals[i] = ev;
+    const int cls = cnt == 0 ? 0 : (cnt == 1 ? 1 + who : ng + 1);
+    key[i] = keyed ? (((cls * pnr + b) << 12) | min(f_any, 4095)) : n - f_any;
+  }
 }
 
 void launch_64e(const PathArgs& A, int wpb, hipStream_t s) {
@@ -995,16 +1165,28 @@ void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
   long nb = (long)A.nd * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;
   const bool r64 = A.rec64 != nullptr;
-  if (A.T.shared && NGT <= 8) {
-    if (r64)
-      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, true>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  const bool hint = A.T.shared && NGT <= 8;
+  if (A.snap_mode == 2) {
+    // reuse of base rays (FP64 records only)
+    if (hint)
+      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, true, 2>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
     else
-      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, false>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+      hipLaunchKernelGGL((k_path64<NGT, false, true, 2>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  } else if (A.snap_mode == 1) {
+    if (hint)
+      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, true, 1>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_path64<NGT, false, true, 1>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+  } else if (hint) {
+    if (r64)
+      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, true, 0>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_path64<NGT, NGT <= 8, false, 0>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
   } else {
     if (r64)
-      hipLaunchKernelGGL((k_path64<NGT, false, true>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+      hipLaunchKernelGGL((k_path64<NGT, false, true, 0>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
     else
-      hipLaunchKernelGGL((k_path64<NGT, false, false>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
+      hipLaunchKernelGGL((k_path64<NGT, false, false, 0>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
   }
 }
 
@@ -1044,7 +1226,7 @@ void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
     // emitters on separate lanes when one lane per ray leaves the GPU
     // mostly idle (fewer than 2048 waves: 8 per CU)
     const long waves = (long)A.nd * ((A.nray + 63) / 64);
-    if (A.ng <= 8 && A.snap_mode == 0 && (A.fp64_split > 0 || (A.fp64_split < 0 && waves < 2048))) {
+    if (A.ng <= 8 && A.snap_mode != 2 && (A.fp64_split > 0 || (A.fp64_split < 0 && waves < 2048))) {
       launch_64e(A, 4, s);
       return;
     }
@@ -1095,13 +1277,16 @@ __global__ void k_sum_points(const int* np, int n, unsigned long long* acc) {
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
 }
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
 
 void launch_sum_points(const int* np, int n, unsigned long long* acc, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_sum_points, dim3((n + 255) / 256), dim3(256), 0, s, np, n, acc);
 }
-void launch_zero_u64(unsigned long long* p, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, p);
+void launch_zero_u64(unsigned long long* p, hipStream_t s, int n) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, n);
 }
 // Counting sort of rays by descending LOS point count (keys 0..nbins-1), for
 // captured graphs: three kernels, no library temporaries or memsets.  The
@@ -1163,9 +1348,60 @@ void launch_fill_f64(double* p, size_t n, double v, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
 
-void launch_prefix_start(int nray, int pnr, int R, int cap, const double* rec64, const int* np, int* start,
-                         int* rem, hipStream_t s) {
+// Work of the reuse pass as the waves see it (lanes = 64 consecutive rays of
+// the sorted order): per wave the point iterations max(np) - min(start), the
+// emitter iterations sum_g max(np) - min(fdiff_g) over lanes still computing
+// g, and the clear-air optics iterations, summed into acc[0..2]; waves and
+// their point iterations by the number k of emitters computed (k = 0 .. 8,
+// 8 for 8 or more) into acc[3 + 2 k], acc[4 + 2 k].
+__global__ void __launch_bounds__(256) k_reuse_waves(const int* order, const int* start, const int* np,
+                                                     const int* fdiff, const int* fcont, int nray, int ng,
+                                                     unsigned long long* acc) {
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w * 64 >= nray) return;
+  const int slot = w * 64 + lane;
+  const bool v = slot < nray;
+  const int r = v ? order[slot] : 0;
+  const int n = v ? np[r] : 0;
+  int lo = v ? start[r] : 0x7fffffff, hi = n, clo = v && fcont[r] < n ? fcont[r] : 0x7fffffff;
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+    clo = min(clo, __shfl_xor(clo, o, 64));
+  }
+  long ge = 0;
+  int ne = 0;
+  for (int g = 0; g < ng; g++) {
+    int f = v && fdiff[(size_t)r * ng + g] < n ? fdiff[(size_t)r * ng + g] : 0x7fffffff;
+    for (int o = 32; o > 0; o >>= 1) f = min(f, __shfl_xor(f, o, 64));
+    if (f < hi) {
+      ge += hi - f;
+      ne++;
+    }
+  }
+  if (lane == 0) {
+    const int span = hi > lo ? hi - lo : 0;
+    atomicAdd(&acc[0], (unsigned long long)span);
+    atomicAdd(&acc[1], (unsigned long long)ge);
+    atomicAdd(&acc[2], (unsigned long long)(clo < hi ? hi - clo : 0));
+    // by the number of emitters the wave computes (0 .. 8+)
+    const int k = ne < 8 ? ne : 8;
+    atomicAdd(&acc[3 + 2 * k], 1ull);
+    atomicAdd(&acc[4 + 2 * k], (unsigned long long)span);
+  }
+}
+void launch_reuse_waves(const int* order, const int* start, const int* np, const int* fdiff, const int* fcont,
+                        int nray, int ng, unsigned long long* acc, hipStream_t s) {
+  const int waves = (nray + 63) / 64;
+  if (waves > 0)
+    hipLaunchKernelGGL(k_reuse_waves, dim3((waves + 3) / 4), dim3(256), 0, s, order, start, np, fdiff, fcont, nray,
+                       ng, acc);
+}
+void launch_reuse_start(int nray, int pnr, int R, int ng, int nw, int cap, unsigned long long cmask,
+                        const double* rec64, const int* np, int* start, int* rem, int* fdiff, int* fcont, int* evals,
+                        int* key, int keyed, hipStream_t s) {
   if (nray > 0)
-    hipLaunchKernelGGL(k_prefix_start, dim3((nray + 127) / 128), dim3(128), 0, s, nray, pnr, R, cap, rec64, np,
-                       start, rem);
+    hipLaunchKernelGGL(k_reuse_start, dim3((nray + 3) / 4), dim3(256), 0, s, nray, pnr, R, ng, rec_u(nw), cap,
+                       cmask, rec64, np, start, rem, fdiff, fcont, evals, key, keyed);
 }

@@ -193,24 +193,37 @@ def test_fp64_emitter_lanes_bit_identical(glib, in_examples, case):
     np.testing.assert_array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("group", [1, 0])
 @pytest.mark.parametrize("name", ["clear_t_p_q", "clear_hydz_bbt", "ext1_aerosol"])
-def test_prefix_reuse_bit_identical(glib, in_examples, name):
+def test_prefix_reuse_bit_identical(glib, in_examples, name, group):
     """A perturbed copy's ray repeats copy 0's up to its first differing LOS
     point (FP64 records compared bit for bit) and is integrated from there,
-    starting from copy 0's path state: the Jacobian is bit-identical to the
-    one integrated from the observer (option kernel_prefix 0)."""
+    starting from copy 0's path state; each emitter's factor 1 - eps and the
+    point's clear-air optics come from copy 0 up to the first point whose
+    inputs to them differ.  The Jacobian is bit-identical to the one
+    integrated from the observer (option kernel_prefix 0), with the copies
+    grouped by the emitters they compute or in plain order."""
     case, keys, _ = CASES[name]
     ks = []
-    for pre in (0, 1):
-        glib.jur_set_option(b"kernel_prefix", pre)
-        try:
+    glib.jur_set_option(b"reuse_group", group)
+    try:
+        for pre in (0, 1):
+            glib.jur_set_option(b"kernel_prefix", pre)
             k, _, _ = jacobian(glib, "", case, keys)
-            pts = glib.jur_stat(b"integrated_points")
-        finally:
-            glib.jur_set_option(b"kernel_prefix", 0)
-        ks.append((k, pts))
-    print("%s: LOS points integrated %d (prefix reuse) vs %d" % (name, ks[1][1], ks[0][1]), flush=True)
+            ks.append((k, glib.jur_stat(b"integrated_points"), glib.jur_stat(b"emitter_points")))
+    finally:
+        glib.jur_set_option(b"kernel_prefix", 1)
+        glib.jur_set_option(b"reuse_group", 1)
+    ng = read_example(glib, "", case)[0].ng
+    print("%s: LOS points integrated %d (reuse) vs %d, emitter evaluations %g vs %g"
+          % (name, ks[1][1], ks[0][1], ks[1][2], ng * ks[0][1]), flush=True)
     np.testing.assert_array_equal(ks[1][0], ks[0][0])
-    # the statistic is of the last batched call: an aerosol column (its own
+    # the statistics are of the last batched call: an aerosol column (its own
     # call, no copies to reuse) in ext1_aerosol
-    assert ks[1][1] < ks[0][1] or name == "ext1_aerosol"
+    assert np.isnan(ks[0][2])
+    if name != "ext1_aerosol":
+        assert ks[1][1] < ks[0][1]
+        assert ks[1][2] <= ng * ks[1][1]
+    if name == "clear_t_p_q":
+        # O3 copies take the CO2 and H2O chains from copy 0
+        assert ks[1][2] < ng * ks[1][1]
