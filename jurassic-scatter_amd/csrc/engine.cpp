@@ -2,6 +2,7 @@
 // source-function tables and continuum constants resident in HBM, aerosol
 // layer optics, work buffers, and the clear-air / extinction pipeline
 // (ray tracing -> sort by path length -> path integration).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -29,7 +30,7 @@ struct Engine {
   DBuf t_np, t_p, t_nt, t_t, t_ncol, t_col, t_ue, planck, chan, nu_d;
   DBuf t_p64, t_t64, planck64, chan64;  // FP64 grids, Planck table, channel data
   DBuf g_np, g_p, g_nt, g_t, t_meta;
-  int shared = 0;
+  int shared = 0, gnp_max = 0, gnt_max = 0;
   unsigned ue_bytes = 0, meta_bytes = 0;
   // aerosol layers
   DBuf beta_e, beta_s, beta_a, alti;
@@ -336,6 +337,11 @@ void engine_prepare(const ctl_t* ctl) {
     std::vector<int2> meta(ncol.size());
     for (size_t c = 0; c < ncol.size(); c++) meta[c] = make_int2(ncol[c], col[c]);
     e.shared = shared ? 1 : 0;
+    e.gnp_max = e.gnt_max = 0;
+    for (int ig = 0; ig < ng; ig++) {
+      e.gnp_max = std::max(e.gnp_max, gnp[ig]);
+      for (int ip = 0; ip < gnp[ig]; ip++) e.gnt_max = std::max(e.gnt_max, gnt[(size_t)ig * TBLNPMAX + ip]);
+    }
     upload(e.g_np, gnp, s);
     upload(e.g_p, gp, s);
     upload(e.g_nt, gnt, s);
@@ -1196,6 +1202,8 @@ DevTables engine_tables() {
   T.gt = e.g_t.as<float>();
   T.meta = e.t_meta.as<int2>();
   T.meta_bytes = e.meta_bytes;
+  T.gnp_max = e.gnp_max;
+  T.gnt_max = e.gnt_max;
   T.p64 = e.t_p64.as<double>();
   T.t64 = e.t_t64.as<double>();
   return T;

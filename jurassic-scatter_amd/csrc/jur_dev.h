@@ -38,6 +38,7 @@ struct DevTables {
   const float* gt;     // [ng][TBLNPMAX][TBLNTMAX]
   const int2* meta;    // [ntbl][TBLNPMAX][TBLNTMAX] = (ncol, col)
   unsigned meta_bytes;
+  int gnp_max, gnt_max;  // largest np and nt of the shared grids
   // the pressure and temperature grids as read (FP64), for the FP64 path
   const double* p64;   // [ntbl][TBLNPMAX]
   const double* t64;   // [ntbl][TBLNPMAX][TBLNTMAX]

@@ -612,6 +612,326 @@ __global__ void k_zero_int(int* p, int n) {
 }
 
 // ---------------------------------------------------------------------------
+// Staged path kernel: the fast kernel's arithmetic (same helpers, same
+// operation order: bit-identical radiances) with its memory accesses
+// reorganised for latency.  The shared (p, T) grids of all emitters are
+// staged in LDS once per block; a column's two bracket nodes are one 16-byte
+// load; every corner of an emitter keeps its own bracket hint (16 bits in
+// LDS, the forward bracket of the previous point, where the next inverse
+// lookup lands 94 % of the time and otherwise one node off); the metadata
+// and node loads of all corners and channels of an emitter are issued
+// together, and the four corners search in lockstep: one load round per
+// step for all of them, where the fast kernel walks each corner's search
+// chain in turn (a wave pays for the slowest lane of each).  Searches from
+// a hint return the same bracket as the reference's bisection on strictly
+// increasing columns.  The next point's record is loaded one iteration
+// ahead.
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// nodes h and h + 1 of the column at element b: (u0, eps0, u1, eps1)
+__device__ inline float4 ld_pair(__amdgpu_buffer_rsrc_t r, int b, int h) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (b + h) * 8, 0, 0);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+// two neighbouring (ncol, col) entries
+__device__ inline int4 ld_meta2(__amdgpu_buffer_rsrc_t r, int i) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, i * 8, 0, 0);
+  return make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
+}
+
+// A contiguous column read through the table's buffer resource: pairs are
+// one 16-byte load.
+struct ColP : ColB {
+  __device__ float4 pair(int h) const { return ld_pair(r, base, h); }
+};
+// Step of a column search: -1 / +1 when x lies below / above the bracket
+// (h, q), 0 inside it (or NaN).
+template <bool KEY_EPS>
+__device__ inline int step_dir(float4 q, int h, int n, float x) {
+  const float lo = KEY_EPS ? q.y : q.x, hi = KEY_EPS ? q.w : q.z;
+  return (h > 0 && x < lo) ? -1 : ((h < n - 2 && hi <= x) ? 1 : 0);
+}
+
+// Brackets of x[k] in the four corner columns (element base b[k], length
+// n[k]) from the hints h[k] with their pairs q[k] loaded: all corners step
+// together, one load round per step for the four of them (a wave takes a
+// step when any of its lanes needs it), three steps at most; what is still
+// off then, and NaN, is found by bisection.  Returns the reference's
+// bracket (the unique one in a strictly increasing column).
+template <bool KEY_EPS, class Col>
+__device__ inline void lockstep(const Col (&col)[4], const int (&n)[4], const float (&x)[4], bool live, int (&h)[4],
+                                float4 (&q)[4]) {
+  int d[4];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    d[k] = live ? step_dir<KEY_EPS>(q[k], h[k], n[k], x[k]) : 0;
+    any = any || d[k] != 0;
+  }
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    if (!__any(any)) break;
+    // every corner reloads (a corner that stays reloads its own pair): no
+    // per-corner branches, the four loads of a round issue together
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      h[k] += d[k];
+      q[k] = col[k].pair(h[k]);
+    }
+    any = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      d[k] = live ? step_dir<KEY_EPS>(q[k], h[k], n[k], x[k]) : 0;
+      any = any || d[k] != 0;
+    }
+  }
+  bool slow = any;
+#pragma unroll
+  for (int k = 0; k < 4; k++) slow = slow || (live && !(x[k] == x[k]));
+  if (__any(slow)) {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (live && (d[k] != 0 || !(x[k] == x[k]))) {
+        h[k] = locate_col<KEY_EPS>(col[k], n[k], x[k]);
+        q[k] = col[k].pair(h[k]);
+      }
+  }
+}
+
+// Bracket in a strictly increasing grid held in LDS, from hint h.
+__device__ inline int grid_bracket(const float* g, int n, float x, int h) {
+  h = h < 0 ? 0 : (h > n - 2 ? n - 2 : h);
+  const float a = g[h], b = g[h + 1];
+  if ((h == 0 || a <= x) && (h == n - 2 || x < b)) return h;
+  return locate_mono<float>(g, n, x);
+}
+
+template <int NGT, int CPW, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) k_path_stage(PathArgs A) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const DevTables& T = A.T;
+  const int NPS = T.gnp_max, NTS = T.gnt_max;
+  float* sgp = (float*)lds_raw;                 // [NGT][NPS] pressure grids
+  int* sgnt = (int*)(sgp + NGT * NPS);          // [NGT][NPS] temperatures per pressure
+  float* sgt = (float*)(sgnt + NGT * NPS);      // [NGT][NPS][NTS] temperature grids
+  const int goff = (2 * NGT * NPS + NGT * NPS * NTS + 1) & ~1;
+  for (int i = threadIdx.x; i < NGT * NPS; i += blockDim.x) {
+    const int g = i / NPS, ip = i - g * NPS;
+    const bool in = g < A.ng;
+    sgp[i] = in ? T.gp[g * TBLNPMAX + ip] : 0.f;
+    sgnt[i] = in ? T.gnt[g * TBLNPMAX + ip] : 0;
+  }
+  for (int i = threadIdx.x; i < NGT * NPS * NTS; i += blockDim.x) {
+    const int g = i / (NPS * NTS), r = i - g * NPS * NTS, ip = r / NTS, it = r - ip * NTS;
+    sgt[i] = g < A.ng ? T.gt[(size_t)(g * TBLNPMAX + ip) * TBLNTMAX + it] : 0.f;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  int chunk, group;
+  if (!map_work(A, CPW, chunk, group)) return;
+  // per-corner bracket hints: [wave][channel x emitter][lane], 4 x 16 bits
+  uint2* hints = (uint2*)((float*)lds_raw + goff) + (threadIdx.x >> 6) * (CPW * NGT * 64) + lane;
+  const int slot = group * 64 + lane;
+  const bool valid = slot < A.nray;
+  const int ray = valid ? A.order[slot] : 0;
+  const int npts = valid ? A.np[ray] : 0;
+  const int c0 = chunk * CPW;
+  const int uo = rec_u(A.nw);
+  const __amdgpu_buffer_rsrc_t ur = __builtin_amdgcn_make_buffer_rsrc((void*)T.ue, (short)0, (int)T.ue_bytes,
+                                                                      0x00020000);
+  const __amdgpu_buffer_rsrc_t mr = __builtin_amdgcn_make_buffer_rsrc((void*)T.meta, (short)0,
+                                                                      (int)T.meta_bytes, 0x00020000);
+  int gnp[NGT];
+#pragma unroll
+  for (int g = 0; g < NGT; g++) gnp[g] = g < A.ng ? T.gnp[g] : 0;
+
+  float ep[CPW][NGT], tp[CPW][NGT], radv[CPW], comp[CPW], tau[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; c++) {
+#pragma unroll
+    for (int g = 0; g < NGT; g++) {
+      ep[c][g] = 0.f;
+      tp[c][g] = 1.f;
+      hints[(c * NGT + g) * 64] = make_uint2(0u, 0u);
+    }
+    radv[c] = 0.f;
+    comp[c] = 0.f;
+    tau[c] = 1.f;
+  }
+  int hpt[NGT];  // packed (ipr, it0, it1) brackets of the previous point
+#pragma unroll
+  for (int g = 0; g < NGT; g++) hpt[g] = 0;
+  int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
+  const float* rp = A.rec + (size_t)ray * A.cap * A.R;
+
+  // the next point's record, loaded one iteration ahead
+  float np_p, np_t, np_ds, np_af, np_qh, np_ai, np_du[NGT];
+  {
+    const float* pr = rp;
+    np_p = pr[REC_P];
+    np_t = pr[REC_T];
+    np_ds = pr[REC_DS];
+    np_af = pr[REC_AF];
+    np_qh = pr[REC_QH2O];
+    np_ai = pr[REC_AI];
+#pragma unroll
+    for (int g = 0; g < NGT; g++) np_du[g] = g < A.ng ? pr[uo + g] : 0.f;
+  }
+
+  for (int ip = 0;; ip++) {
+    const bool act = ip < npts;
+    if (!__any(act)) break;
+    const float p = np_p, t = np_t, ds = np_ds, af = np_af, qh = np_qh, aif = np_ai;
+    float dus[NGT];
+#pragma unroll
+    for (int g = 0; g < NGT; g++) dus[g] = np_du[g];
+    {
+      // prefetch (the last point of the ray reloads itself)
+      const int in = ip + 1 < npts ? ip + 1 : (npts > 0 ? npts - 1 : 0);
+      const float* pn = rp + (size_t)in * A.R;
+      np_p = pn[REC_P];
+      np_t = pn[REC_T];
+      np_ds = pn[REC_DS];
+      np_af = pn[REC_AF];
+      np_qh = pn[REC_QH2O];
+      np_ai = pn[REC_AI];
+#pragma unroll
+      for (int g = 0; g < NGT; g++) np_du[g] = g < A.ng ? pn[uo + g] : 0.f;
+    }
+    if (act) {
+      const float* pr = rp + (size_t)ip * A.R;
+      int itp;
+      float dtp;
+      planck_index(t, itp, dtp);
+      const bool scpt = A.scattering > 0 && af > 0.f;
+      const int ai = scpt || (A.sca_n > 0 && af != 0.f) ? (int)aif : 0;
+      const float rtp = __builtin_amdgcn_rcpf(t), rdsp = __builtin_amdgcn_rcpf(ds);
+      float tg[CPW], et[CPW];
+#pragma unroll
+      for (int c = 0; c < CPW; c++) {
+        tg[c] = 1.f;
+        et[c] = 0.f;
+      }
+#pragma unroll
+      for (int g = 0; g < NGT; g++) {
+        if (g >= A.ng) continue;
+        const int np_ = gnp[g];
+        if (np_ < 2) continue;
+        const float du = dus[g];
+        // (p, T) brackets and weights from the staged grids
+        const float* pg = sgp + g * NPS;
+        const int ipr = grid_bracket(pg, np_, p, hpt[g] & 0xff);
+        const int nt0 = sgnt[g * NPS + ipr], nt1 = sgnt[g * NPS + ipr + 1];
+        const bool gok = nt0 >= 2 && nt1 >= 2;
+        const float* t0g = sgt + (g * NPS + ipr) * NTS;
+        const float* t1g = t0g + NTS;
+        const int it0 = grid_bracket(t0g, gok ? nt0 : 2, t, (hpt[g] >> 8) & 0xff);
+        const int it1 = grid_bracket(t1g, gok ? nt1 : 2, t, (hpt[g] >> 16) & 0xff);
+        if (gok) hpt[g] = ipr | (it0 << 8) | (it1 << 16);
+        const float wt0 = fdiv(t - t0g[it0], t0g[it0 + 1] - t0g[it0]);
+        const float wt1 = fdiv(t - t1g[it1], t1g[it1 + 1] - t1g[it1]);
+        const float wp = fdiv(p - pg[ipr], pg[ipr + 1] - pg[ipr]);
+        const int gb0 = ipr * TBLNTMAX + it0, gb1 = (ipr + 1) * TBLNTMAX + it1;
+        // all channels: metadata, hints and the hinted bracket nodes
+        int nn[CPW][4], bb[CPW][4], hh[CPW][4];
+        float4 qa[CPW][4];
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+          const int id = c0 + c < A.nd ? c0 + c : A.nd - 1;
+          const int tb = (id * A.ng + g) * TBLNPMAX * TBLNTMAX;
+          const int4 m0 = ld_meta2(mr, tb + gb0), m1 = ld_meta2(mr, tb + gb1);
+          const uint2 hv = hints[(c * NGT + g) * 64];
+          nn[c][0] = m0.x; bb[c][0] = m0.y; nn[c][1] = m0.z; bb[c][1] = m0.w;
+          nn[c][2] = m1.x; bb[c][2] = m1.y; nn[c][3] = m1.z; bb[c][3] = m1.w;
+          hh[c][0] = (int)(hv.x & 0xffffu); hh[c][1] = (int)(hv.x >> 16);
+          hh[c][2] = (int)(hv.y & 0xffffu); hh[c][3] = (int)(hv.y >> 16);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const int hm = nn[c][k] - 2;
+            hh[c][k] = hh[c][k] > hm ? hm : hh[c][k];
+            hh[c][k] = hh[c][k] < 0 ? 0 : hh[c][k];
+            qa[c][k] = ld_pair(ur, bb[c][k], hh[c][k]);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+          const int id = c0 + c;
+          if (id >= A.nd) continue;
+          const bool sat = tp[c][g] < 1e-9f;
+          const bool live = gok && !sat && nn[c][0] >= 2 && nn[c][1] >= 2 && nn[c][2] >= 2 && nn[c][3] >= 2;
+          const float e = ep[c][g];
+          const int h0[4] = {hh[c][0], hh[c][1], hh[c][2], hh[c][3]};
+          const float ex[4] = {e, e, e, e};
+          const ColP cols[4] = {{{ur, bb[c][0]}}, {{ur, bb[c][1]}}, {{ur, bb[c][2]}}, {{ur, bb[c][3]}}};
+          lockstep<true>(cols, nn[c], ex, live, hh[c], qa[c]);
+          float ux[4];
+          float4 qf[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const float4 q = qa[c][k];
+            ux[k] = lin_r(q.y, q.x, q.w, q.z, e) + du;
+            qf[k] = q;
+          }
+          lockstep<false>(cols, nn[c], ux, live, hh[c], qf);
+          if (hh[c][0] != h0[0] || hh[c][1] != h0[1] || hh[c][2] != h0[2] || hh[c][3] != h0[3])
+            hints[(c * NGT + g) * 64] = make_uint2((unsigned)hh[c][0] | ((unsigned)hh[c][1] << 16),
+                                                   (unsigned)hh[c][2] | ((unsigned)hh[c][3] << 16));
+          EC ec[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const float4 q = qf[k];
+            const float s = fdiv(q.w - q.y, q.z - q.x), dx = ux[k] - q.x;
+            ec[k] = clamp_ec(q.y + s * dx, (1.f - q.y) - s * dx);
+          }
+          EC e00 = mix_ec(ec[0], ec[1], wt0);
+          const EC e11 = mix_ec(ec[2], ec[3], wt1);
+          e00 = mix_ec(e00, e11, wp);
+          const float rt = __builtin_amdgcn_rcpf(tp[c][g]);
+          const bool tr = e00.e < 0.5f;
+          const float s1 = (e00.e - e) * rt, f1 = e00.c * rt;
+          float s = tr ? s1 : 1.f - f1;
+          float f = tr ? 1.f - s1 : f1;
+          if (sat) {
+            s = 1.f;
+            f = 0.f;
+            ep[c][g] = 1.f;
+            tp[c][g] = 0.f;
+          } else if (live) {
+            ep[c][g] = e00.e;
+            tp[c][g] = e00.c;
+          } else {
+            s = 0.f;
+            f = 1.f;
+          }
+          et[c] = et[c] + s - et[c] * s;
+          tg[c] *= f;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CPW; c++) {
+        const int id = c0 + c;
+        if (id < A.nd)
+          point_radiance(A, A.cc[id], id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg[c], et[c], rtp, rdsp,
+                         radv[c], comp[c], tau[c]);
+      }
+      if (scpt) kj++;
+    }
+  }
+  if (!valid) return;
+  finish_ray<CPW>(A, ray, c0, radv, comp, tau);
+}
+
+// LDS bytes of k_path_stage: staged grids + hints; 0 if the grids are too
+// large to stage (the fast kernel runs instead)
+size_t stage_lds_bytes(const PathArgs& A, int ngt, int cpw, int wpb) {
+  const size_t gw = (size_t)2 * ngt * A.T.gnp_max + (size_t)ngt * A.T.gnp_max * A.T.gnt_max;
+  if (gw * 4 > 24576 || A.T.gnp_max < 2 || A.T.gnt_max < 2) return 0;
+  return ((gw + 1) & ~(size_t)1) * 4 + (size_t)wpb * cpw * ngt * 64 * 8;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 path integration: the reference's arithmetic and operation order
 // (forwardmodel.c:303-399 with intpol_tbl :409-502 and formod_continua
 // :120-152) in double precision, on the FP32 (u, eps) columns the reference
@@ -1190,19 +1510,24 @@ void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
   }
 }
 
-template <int NGT, int CPW, int OCC>
+template <int NGT, int CPW, int OCC, bool STAGE = false>
 void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   int nchunk = (A.nd + CPW - 1) / CPW;
   int ngroups = (A.nray + 63) / 64;
   long nb = (long)nchunk * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;  // a multiple of the XCD count for the logical remap
-  if (A.T.shared)
+  const size_t lds = STAGE && A.T.shared ? stage_lds_bytes(A, NGT, CPW, wpb) : 0;
+  if (lds > 0)
+    hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC>), dim3((unsigned)nb), dim3(64 * wpb), (unsigned)lds, s, A);
+  else if (A.T.shared)
     hipLaunchKernelGGL((k_path_fast<NGT, CPW, OCC>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
   else
     hipLaunchKernelGGL((k_path<NGT, CPW>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
 }
 
-// Variants (channels per wave x occupancy target) picked by the tuner.
+// Variants (kernel, channels per wave, occupancy target) picked by the
+// tuner: 0-6 the fast kernel, 7-8 the staged kernel (the fast one where the
+// grids are too large to stage).
 template <int NGT>
 void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
   switch (variant) {
@@ -1212,13 +1537,15 @@ void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
     case 3: launch_t<NGT, 4, 2>(A, wpb, s); break;
     case 4: launch_t<NGT, 4, 4>(A, wpb, s); break;
     case 5: launch_t<NGT, 2, 3>(A, wpb, s); break;
+    case 7: launch_t<NGT, 1, 4, true>(A, wpb, s); break;
+    case 8: launch_t<NGT, 2, 3, true>(A, wpb, s); break;
     default: launch_t<NGT, 2, 5>(A, wpb, s); break;
   }
 }
 
 }  // namespace
 
-int path_variants() { return 7; }
+int path_variants() { return 9; }
 
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;
