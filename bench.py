@@ -69,7 +69,13 @@ CPU_SAMPLE_NADIR = ("cirrus_4096",)
 MEASURED_TRAFFIC = {
     ("clear_batch_65536", 65536, 6, 2): (1.9978e11, "profiles/k_path_fast_clear_batch_65536.md"),
     ("clear_batch_65536", 65536, 6, 4): (1.9911e+11, "profiles/k_path_fast_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 7, 1): (1.9045e11, "profiles/k_path_stage_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 7, 2): (1.8952e11, "profiles/k_path_stage_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 7, 4): (1.8896e11, "profiles/k_path_stage_clear_batch_65536.md"),
 }
+# The FP32 path kernel of each autotuner variant (7-8: the staged kernel,
+# grids in LDS, lockstep bracket searches; 0-6: the fast kernel).
+PATH_KERNEL = {7: "k_path_stage", 8: "k_path_stage"}
 # The same for the FP64 path kernel of kernel() on the Jacobian workload.
 # HBM bytes per kernel() call of the path kernels (FETCH_SIZE / WRITE_SIZE
 # passes): with base-ray reuse (the default), and integrating every copy
@@ -863,7 +869,7 @@ def main():
         "graph_call_ms": float(np.mean(gcall)) if graph else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "k_path_fast", "kernel_ms": path_avg,
+                     "kernel": PATH_KERNEL.get(variant, "k_path_fast"), "kernel_ms": path_avg,
                      "kernel_ms_source": "events inside the replayed graph" if path_timed is gpath
                      else "events around the eager launches",
                      "kernel_ms_eager": float(np.mean(path_ms)),
