@@ -647,10 +647,25 @@ struct ColP : ColB {
 };
 // Step of a column search: -1 / +1 when x lies below / above the bracket
 // (h, q), 0 inside it (or NaN).
-template <bool KEY_EPS>
-__device__ inline int step_dir(float4 q, int h, int n, float x) {
-  const float lo = KEY_EPS ? q.y : q.x, hi = KEY_EPS ? q.w : q.z;
+// X: float (FP32 path) or double (FP64 path: FP32 nodes compared in double,
+// as the reference's locate on the table's float columns does).
+template <bool KEY_EPS, class X>
+__device__ inline int step_dir(float4 q, int h, int n, X x) {
+  const X lo = KEY_EPS ? q.y : q.x, hi = KEY_EPS ? q.w : q.z;
   return (h > 0 && x < lo) ? -1 : ((h < n - 2 && hi <= x) ? 1 : 0);
+}
+
+// Bisection of a column for key x (locate, misc.c:237-265, for increasing
+// data; NaN gives the last bracket).
+template <bool KEY_EPS, class C, class X>
+__device__ inline int locate_colx(const C& c, int n, X x) {
+  int ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    const int i = (ihi + ilo) >> 1;
+    const X v = KEY_EPS ? c[i].y : c[i].x;
+    if (v > x) ihi = i; else ilo = i;
+  }
+  return ilo;
 }
 
 // Brackets of x[k] in the four corner columns (element base b[k], length
@@ -659,8 +674,8 @@ __device__ inline int step_dir(float4 q, int h, int n, float x) {
 // step when any of its lanes needs it), three steps at most; what is still
 // off then, and NaN, is found by bisection.  Returns the reference's
 // bracket (the unique one in a strictly increasing column).
-template <bool KEY_EPS, class Col>
-__device__ inline void lockstep(const Col (&col)[4], const int (&n)[4], const float (&x)[4], bool live, int (&h)[4],
+template <bool KEY_EPS, class Col, class X>
+__device__ inline void lockstep(const Col (&col)[4], const int (&n)[4], const X (&x)[4], bool live, int (&h)[4],
                                 float4 (&q)[4]) {
   int d[4];
   bool any = false;
@@ -693,7 +708,7 @@ __device__ inline void lockstep(const Col (&col)[4], const int (&n)[4], const fl
 #pragma unroll
     for (int k = 0; k < 4; k++)
       if (live && (d[k] != 0 || !(x[k] == x[k]))) {
-        h[k] = locate_col<KEY_EPS>(col[k], n[k], x[k]);
+        h[k] = locate_colx<KEY_EPS>(col[k], n[k], x[k]);
         q[k] = col[k].pair(h[k]);
       }
   }
@@ -1024,6 +1039,7 @@ struct RegHints {
   __device__ int& operator[](int k) const { return h[k]; }
 };
 
+
 template <bool HINT, class H>
 __device__ inline double corner64(const float2* col, int n, double e_path, double du, H&& h) {
   const int i = HINT ? near_col64<true>(col, n, e_path, h) : locate_col64<true>(col, n, e_path);
@@ -1062,10 +1078,45 @@ __device__ inline double ega64(const DevTables& T, int tb, double p, double t, d
       const int n00 = T.ncol[k00], n01 = T.ncol[k00 + 1], n10 = T.ncol[k10], n11 = T.ncol[k10 + 1];
       if (n00 >= 2 && n01 >= 2 && n10 >= 2 && n11 >= 2) {
         const double ep = 1.0 - tau_path;
-        double e00 = corner64<HINT>(T.ue + T.col[k00], n00, ep, du, hint[3]);
-        double e01 = corner64<HINT>(T.ue + T.col[k00 + 1], n01, ep, du, hint[4]);
-        double e10 = corner64<HINT>(T.ue + T.col[k10], n10, ep, du, hint[5]);
-        double e11 = corner64<HINT>(T.ue + T.col[k10 + 1], n11, ep, du, hint[6]);
+        double e00, e01, e10, e11;
+        if (HINT) {
+          // the four corners searched in lockstep from their hints (the
+          // previous point's forward brackets), as in the FP32 staged kernel
+          const __amdgpu_buffer_rsrc_t ur =
+              __builtin_amdgcn_make_buffer_rsrc((void*)T.ue, (short)0, (int)T.ue_bytes, 0x00020000);
+          const ColP col[4] = {{{ur, T.col[k00]}}, {{ur, T.col[k00 + 1]}}, {{ur, T.col[k10]}},
+                               {{ur, T.col[k10 + 1]}}};
+          const int n[4] = {n00, n01, n10, n11};
+          int h[4];
+          float4 q[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const int hk = hint[3 + k];
+            h[k] = hk < 0 ? 0 : (hk > n[k] - 2 ? n[k] - 2 : hk);
+            q[k] = col[k].pair(h[k]);
+          }
+          const double ex[4] = {ep, ep, ep, ep};
+          lockstep<true>(col, n, ex, true, h, q);
+          double uu[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) uu[k] = lin_f(q[k].y, q[k].x, q[k].w, q[k].z, ep) + du;
+          lockstep<false>(col, n, uu, true, h, q);
+          double ek[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            hint[3 + k] = h[k];
+            ek[k] = clamp01d(lin_f(q[k].x, q[k].y, q[k].z, q[k].w, uu[k]));
+          }
+          e00 = ek[0];
+          e01 = ek[1];
+          e10 = ek[2];
+          e11 = ek[3];
+        } else {
+          e00 = corner64<HINT>(T.ue + T.col[k00], n00, ep, du, hint[3]);
+          e01 = corner64<HINT>(T.ue + T.col[k00 + 1], n01, ep, du, hint[4]);
+          e10 = corner64<HINT>(T.ue + T.col[k10], n10, ep, du, hint[5]);
+          e11 = corner64<HINT>(T.ue + T.col[k10 + 1], n11, ep, du, hint[6]);
+        }
         e00 = clamp01d(lin<double>(t0g[it0], e00, t0g[it0 + 1], e01, t));
         e11 = clamp01d(lin<double>(t1g[it1], e10, t1g[it1 + 1], e11, t));
         e00 = clamp01d(lin<double>(pg[ipr], e00, pg[ipr + 1], e11, p));
