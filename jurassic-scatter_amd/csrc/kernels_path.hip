@@ -1245,7 +1245,7 @@ __device__ __forceinline__ void finish64(const PathArgs& A, const double* prow, 
 // MODE 0: plain; 1: write the snapshots of base rays (snap_mode 1); 2: reuse
 // base rays (snap_mode 2).  Modes 1 and 2 read FP64 records.
 template <int NGT, bool HINT, bool R64, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_path64(PathArgs A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) k_path64(PathArgs A) {
   constexpr bool REUSE = MODE == 2, SNAP = MODE == 1;
   const int lane = threadIdx.x & 63;
   int chunk, group;
