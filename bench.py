@@ -80,8 +80,8 @@ PATH_KERNEL = {7: "k_path_stage", 8: "k_path_stage"}
 # HBM bytes per kernel() call of the path kernels (FETCH_SIZE / WRITE_SIZE
 # passes): with base-ray reuse (the default), and integrating every copy
 # from the observer (option kernel_prefix 0)
-MEASURED_TRAFFIC_JACOBIAN = {1: (3.072e+10, "profiles/k_path64_jacobian_limb_17.md"),
-                             0: (1.0259e+11, "profiles/k_path64_jacobian_limb_17.md")}
+MEASURED_TRAFFIC_JACOBIAN = {1: (1.900e+10, "profiles/k_path64_jacobian_limb_17.md"),
+                             0: (7.323e+10, "profiles/k_path64_jacobian_limb_17.md")}
 
 
 def parse():
@@ -683,7 +683,9 @@ def bench_scatter3d(args, lib):
                    "profiles": 2, "levels": w.nlev, "scattering_points": npts, "pencils": pencils,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_path_fast",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": PATH_KERNEL.get(int(lib.jur_stat(b"tuned_variant")), "k_path_fast")
+                     if np.isfinite(lib.jur_stat(b"tuned_variant")) else "k_path_fast",
                      "kernel_ms": path_ms, "kernel_share": path_ms / (1e3 * dt / steps),
                      "alg_bytes_per_call": alg_bytes,
                      "note": "path kernel time and algorithmic bytes summed over the call's launches"},

@@ -882,21 +882,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
           const ColP cols[4] = {{{ur, bb[c][0]}}, {{ur, bb[c][1]}}, {{ur, bb[c][2]}}, {{ur, bb[c][3]}}};
           lockstep<true>(cols, nn[c], ex, live, hh[c], qa[c]);
           float ux[4];
-          float4 qf[4];
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const float4 q = qa[c][k];
             ux[k] = lin_r(q.y, q.x, q.w, q.z, e) + du;
-            qf[k] = q;
           }
-          lockstep<false>(cols, nn[c], ux, live, hh[c], qf);
+          // the forward search starts in the inverse bracket, in place
+          lockstep<false>(cols, nn[c], ux, live, hh[c], qa[c]);
           if (hh[c][0] != h0[0] || hh[c][1] != h0[1] || hh[c][2] != h0[2] || hh[c][3] != h0[3])
             hints[(c * NGT + g) * 64] = make_uint2((unsigned)hh[c][0] | ((unsigned)hh[c][1] << 16),
                                                    (unsigned)hh[c][2] | ((unsigned)hh[c][3] << 16));
           EC ec[4];
 #pragma unroll
           for (int k = 0; k < 4; k++) {
-            const float4 q = qf[k];
+            const float4 q = qa[c][k];
             const float s = fdiv(q.w - q.y, q.z - q.x), dx = ux[k] - q.x;
             ec[k] = clamp_ec(q.y + s * dx, (1.f - q.y) - s * dx);
           }
@@ -1705,10 +1704,55 @@ __global__ void k_scatter_order(const int* np, int n, int nbins, int* cursor, in
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) order[atomicAdd(&cursor[min(max(np[i], 0), nbins - 1)], 1)] = i;
 }
+// The same two passes with a block-local histogram in LDS (nbins <=
+// SORT_LDS_BINS): rays of equal count are frequent, and one global atomic
+// per (block, bin) replaces one per ray.  A block takes SORT_PER rays per
+// thread.
+constexpr int SORT_LDS_BINS = 4096, SORT_PER = 8;
+__global__ void __launch_bounds__(256) k_count_hist_lds(const int* np, int n, int nbins, int* hist) {
+  __shared__ int lh[SORT_LDS_BINS];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const int i0 = blockIdx.x * blockDim.x * SORT_PER + threadIdx.x;
+  for (int j = 0; j < SORT_PER; j++) {
+    const int i = i0 + j * blockDim.x;
+    if (i < n) atomicAdd(&lh[min(max(np[i], 0), nbins - 1)], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (lh[b]) atomicAdd(&hist[b], lh[b]);
+}
+__global__ void __launch_bounds__(256) k_scatter_order_lds(const int* np, int n, int nbins, int* cursor,
+                                                           int* order) {
+  __shared__ int lh[SORT_LDS_BINS];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) lh[b] = 0;
+  __syncthreads();
+  const int i0 = blockIdx.x * blockDim.x * SORT_PER + threadIdx.x;
+  int bin[SORT_PER], rank[SORT_PER];
+  for (int j = 0; j < SORT_PER; j++) {
+    const int i = i0 + j * blockDim.x;
+    bin[j] = i < n ? min(max(np[i], 0), nbins - 1) : -1;
+    rank[j] = bin[j] >= 0 ? atomicAdd(&lh[bin[j]], 1) : 0;
+  }
+  __syncthreads();
+  // the block's range of each bin, reserved with one global atomic
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (lh[b]) lh[b] = atomicAdd(&cursor[b], lh[b]);
+  __syncthreads();
+  for (int j = 0; j < SORT_PER; j++)
+    if (bin[j] >= 0) order[lh[bin[j]] + rank[j]] = i0 + j * blockDim.x;
+}
 
 void launch_count_sort(const int* np, int n, int nbins, int* hist, int* order, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_zero_int, dim3((nbins + 255) / 256), dim3(256), 0, s, hist, nbins);
+  if (nbins <= SORT_LDS_BINS) {
+    const int nb = (n + 256 * SORT_PER - 1) / (256 * SORT_PER);
+    hipLaunchKernelGGL(k_count_hist_lds, dim3(nb), dim3(256), 0, s, np, n, nbins, hist);
+    hipLaunchKernelGGL(k_hist_offsets, dim3(1), dim3(1024), 0, s, hist, nbins);
+    hipLaunchKernelGGL(k_scatter_order_lds, dim3(nb), dim3(256), 0, s, np, n, nbins, hist, order);
+    return;
+  }
   hipLaunchKernelGGL(k_count_hist, dim3((n + 255) / 256), dim3(256), 0, s, np, n, nbins, hist);
   hipLaunchKernelGGL(k_hist_offsets, dim3(1), dim3(1024), 0, s, hist, nbins);
   hipLaunchKernelGGL(k_scatter_order, dim3((n + 255) / 256), dim3(256), 0, s, np, n, nbins, hist, order);
