@@ -185,6 +185,28 @@ __device__ inline bool map_work(const PathArgs& A, int cpw, int& chunk, int& gro
   return chunk < nchunk && group < ngroups;
 }
 
+// Tiled variant: each XCD's contiguous range of logical blocks cycles
+// through TC = nchunk / 8 channel chunks for one ray group before the next
+// group, so the point records a ray group reads are shared in the XCD's L2
+// by its TC channel chunks (read once per XCD instead of once per chunk),
+// while the XCD's table working set grows to TC chunks' tables.
+__device__ inline bool map_work_tiled(const PathArgs& A, int cpw, int& chunk, int& group) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  const int per = gridDim.x >> 3;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int ngroups = (A.nray + 63) >> 6;
+  const int bpc = (ngroups + wpb - 1) / wpb;  // blocks per channel chunk
+  const int nchunk = (A.nd + cpw - 1) / cpw;
+  const int TC = nchunk >= 16 ? nchunk >> 3 : 1;
+  const int tile = L / (TC * bpc), r = L - tile * TC * bpc;
+  const int tc = min(TC, nchunk - tile * TC);  // chunks in this tile
+  if (tc <= 0) return false;
+  chunk = tile * TC + r % tc;
+  group = (r / tc) * wpb + wv;
+  return chunk < nchunk && group < ngroups && r < tc * bpc;
+}
+
 // Radiance and transmittance update of one ray x channel at one LOS point
 // (forwardmodel.c:332-389) given the gas segment transmittance tg = tau_gas
 // and its complement et = 1 - tau_gas.
@@ -722,7 +744,7 @@ __device__ inline int grid_bracket(const float* g, int n, float x, int h) {
   return locate_mono<float>(g, n, x);
 }
 
-template <int NGT, int CPW, int OCC>
+template <int NGT, int CPW, int OCC, bool TILE = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) k_path_stage(PathArgs A) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const DevTables& T = A.T;
@@ -745,7 +767,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 
   const int lane = threadIdx.x & 63;
   int chunk, group;
-  if (!map_work(A, CPW, chunk, group)) return;
+  if (!(TILE ? map_work_tiled(A, CPW, chunk, group) : map_work(A, CPW, chunk, group))) return;
   // per-corner bracket hints: [wave][channel x emitter][lane], 4 x 16 bits
   uint2* hints = (uint2*)((float*)lds_raw + goff) + (threadIdx.x >> 6) * (CPW * NGT * 64) + lane;
   const int slot = group * 64 + lane;
@@ -1560,7 +1582,7 @@ void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
   }
 }
 
-template <int NGT, int CPW, int OCC, bool STAGE = false>
+template <int NGT, int CPW, int OCC, bool STAGE = false, bool TILE = false>
 void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   int nchunk = (A.nd + CPW - 1) / CPW;
   int ngroups = (A.nray + 63) / 64;
@@ -1568,7 +1590,8 @@ void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   nb = (nb + 7) / 8 * 8;  // a multiple of the XCD count for the logical remap
   const size_t lds = STAGE && A.T.shared ? stage_lds_bytes(A, NGT, CPW, wpb) : 0;
   if (lds > 0)
-    hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC>), dim3((unsigned)nb), dim3(64 * wpb), (unsigned)lds, s, A);
+    hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC, TILE>), dim3((unsigned)nb), dim3(64 * wpb), (unsigned)lds, s,
+                       A);
   else if (A.T.shared)
     hipLaunchKernelGGL((k_path_fast<NGT, CPW, OCC>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
   else
@@ -1589,13 +1612,14 @@ void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
     case 5: launch_t<NGT, 2, 3>(A, wpb, s); break;
     case 7: launch_t<NGT, 1, 4, true>(A, wpb, s); break;
     case 8: launch_t<NGT, 2, 3, true>(A, wpb, s); break;
+    case 9: launch_t<NGT, 1, 4, true, true>(A, wpb, s); break;
     default: launch_t<NGT, 2, 5>(A, wpb, s); break;
   }
 }
 
 }  // namespace
 
-int path_variants() { return 9; }
+int path_variants() { return 10; }
 
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;

@@ -1,7 +1,7 @@
 """Every variant of the FP32 path kernel gives the same radiances, bit for bit.
 
 The variants differ in work mapping (channels per wave, waves per block,
-occupancy target) and, for the staged kernel (variants 7-8), in how the
+occupancy target) and, for the staged kernel (variants 7-9), in how the
 table brackets are found (grids in LDS, per-corner hints, the four corners
 searched in lockstep); the arithmetic is the same, so radiances and
 transmittances must be identical to the default fast kernel's (variant 6)
@@ -22,7 +22,7 @@ from pyjurassic import synth
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = list(range(9))
+VARIANTS = list(range(10))
 
 
 @pytest.fixture(scope="module")
