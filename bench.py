@@ -72,10 +72,14 @@ MEASURED_TRAFFIC = {
     ("clear_batch_65536", 65536, 7, 1): (1.9045e11, "profiles/k_path_stage_clear_batch_65536.md"),
     ("clear_batch_65536", 65536, 7, 2): (1.8952e11, "profiles/k_path_stage_clear_batch_65536.md"),
     ("clear_batch_65536", 65536, 7, 4): (1.8896e11, "profiles/k_path_stage_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 9, 1): (1.1773e10, "profiles/k_path_stage_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 9, 2): (1.5405e10, "profiles/k_path_stage_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 9, 4): (2.4268e10, "profiles/k_path_stage_clear_batch_65536.md"),
 }
-# The FP32 path kernel of each autotuner variant (7-8: the staged kernel,
-# grids in LDS, lockstep bracket searches; 0-6: the fast kernel).
-PATH_KERNEL = {7: "k_path_stage", 8: "k_path_stage"}
+# The FP32 path kernel of each autotuner variant (7-9: the staged kernel,
+# grids in LDS, lockstep bracket searches, 9 with the tiled block mapping;
+# 0-6: the fast kernel).
+PATH_KERNEL = {7: "k_path_stage", 8: "k_path_stage", 9: "k_path_stage"}
 # The same for the FP64 path kernel of kernel() on the Jacobian workload.
 # HBM bytes per kernel() call of the path kernels (FETCH_SIZE / WRITE_SIZE
 # passes): with base-ray reuse (the default), and integrating every copy
