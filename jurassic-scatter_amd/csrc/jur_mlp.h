@@ -66,7 +66,7 @@ void launch_out_fwd(const uint16_t* act, const uint16_t* w5, const float* b5, in
 void launch_loss_grad(const float* y, const float* t, int B, float* dy, float* loss_hist, const int* step,
                       int mask, hipStream_t s);
 void launch_out_bwd(const uint16_t* act4, const float* z4, const float* dy, const uint16_t* w5, int B,
-                    float* gw5, float* gb5, uint16_t* dz4, hipStream_t s);
+                    float* gw5, float* gb5, uint16_t* dz4, float* part, hipStream_t s);
 void launch_silu_back(const float* da, const float* z, long n, uint16_t* dz, hipStream_t s);
 void launch_colsum(const uint16_t* dz, int B, int N, float* out, hipStream_t s);
 void launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wbf, long n, const int* step, float lr,

@@ -104,10 +104,114 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   }
 }
 
+// The same GEMM with 16-byte global loads and a double-buffered 64-deep K
+// stage (the next stage's loads in flight during the current stage's matrix
+// operations).  Operands whose contiguous dimension is K are staged as read;
+// those contiguous in M or N are transposed on the way into LDS (a thread's
+// 8 elements to 8 rows; consecutive threads take consecutive k, so the 2-byte
+// LDS writes of a wave fall in distinct banks).  The K chunks of 16 reach
+// the matrix core in the same order as in k_gemm: identical results.
+// Requires M and N multiples of 8, K per split a multiple of TK2 and
+// 16-byte aligned rows (checked by the launcher).
+constexpr int TK2 = 64;
+constexpr int LDK2 = TK2 + 8;  // 144-byte LDS rows
+typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_gemm_v(GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[2][TS][LDK2];
+  __shared__ __align__(16) uint16_t Bs[2][TS][LDK2];  // [n][k]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int m0 = blockIdx.y * TS, n0 = blockIdx.x * TS;
+  const int kper = g.K / g.ksplit;
+  const int kbeg = blockIdx.z * kper, kend = kbeg + kper;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+  const bool active = (m0 + wm < g.M) && (n0 + wn < g.N);
+  const bool adirect = g.sak == 1, bdirect = g.sbk == 1;
+  u32x4v ra[2], rb[2];
+  // direct: vector v = (row, 8 k); transposed: v = (8 rows, k) with k fastest
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      const int idx = tid + v * 256;
+      const u32x4v z = {0u, 0u, 0u, 0u};
+      if (adirect) {
+        const int r = idx >> 3, c = (idx & 7) * 8, m = m0 + r;
+        ra[v] = m < g.M ? *(const u32x4v*)&g.A[(long)m * g.sam + k0 + c] : z;
+      } else {
+        const int k = idx & 63, mc = (idx >> 6) * 8;
+        ra[v] = m0 + mc < g.M ? *(const u32x4v*)&g.A[(long)(m0 + mc) + (long)(k0 + k) * g.sak] : z;
+      }
+      if (bdirect) {
+        const int r = idx >> 3, c = (idx & 7) * 8, n = n0 + r;
+        rb[v] = n < g.N ? *(const u32x4v*)&g.B[(long)n * g.sbn + k0 + c] : z;
+      } else {
+        const int k = idx & 63, nc = (idx >> 6) * 8;
+        rb[v] = n0 + nc < g.N ? *(const u32x4v*)&g.B[(long)(k0 + k) * g.sbk + n0 + nc] : z;
+      }
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      const int idx = tid + v * 256;
+      if (adirect) {
+        *(u32x4v*)&As[buf][idx >> 3][(idx & 7) * 8] = ra[v];
+      } else {
+        const int k = idx & 63, mc = (idx >> 6) * 8;
+#pragma unroll
+        for (int j = 0; j < 8; j++) As[buf][mc + j][k] = (uint16_t)(ra[v][j >> 1] >> (16 * (j & 1)));
+      }
+      if (bdirect) {
+        *(u32x4v*)&Bs[buf][idx >> 3][(idx & 7) * 8] = rb[v];
+      } else {
+        const int k = idx & 63, nc = (idx >> 6) * 8;
+#pragma unroll
+        for (int j = 0; j < 8; j++) Bs[buf][nc + j][k] = (uint16_t)(rb[v][j >> 1] >> (16 * (j & 1)));
+      }
+    }
+  };
+  f32x16 acc = zero16();
+  gload(kbeg);
+  lstore(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += TK2) {
+    const bool more = k0 + TK2 < kend;
+    if (more) gload(k0 + TK2);
+    if (active) {
+      const int h = lane >> 5, r = lane & 31;
+#pragma unroll
+      for (int kk = 0; kk < TK2; kk += 16) {
+        const bf16x8 a = *(const bf16x8*)&As[cur][wm + r][kk + 8 * h];
+        const bf16x8 b = *(const bf16x8*)&Bs[cur][wn + r][kk + 8 * h];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      }
+    }
+    if (more) lstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (!active) return;
+  const int j = n0 + wn + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int i = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (g.epi == 0) {
+      g.C[(long)blockIdx.z * g.cslab + (long)i * g.ldc + j] = acc[r];
+    } else {
+      const float z = acc[r] + g.bias[j];
+      g.C[(long)i * g.ldc + j] = z;
+      g.act[(long)i * g.ldact + j] = f2bf(silu(z));
+    }
+  }
+}
+
 __global__ void k_splitk_sum(const float* part, int nslab, long slab, long n, float* out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // slabs summed in order; unrolled so that their loads are in flight together
   float s = 0.f;
+#pragma unroll 16
   for (int z = 0; z < nslab; z++) s += part[(long)z * slab + i];
   out[i] = s;
 }
@@ -170,21 +274,22 @@ __global__ void k_out_dz(const float* z4, const float* dy, const uint16_t* w5, i
 }
 
 // gW5[j] = sum_b act4[b][j] dy[b] (block j < 256), gb5 = sum_b dy[b] (block 256).
-__global__ void __launch_bounds__(256) k_out_wgrad(const uint16_t* act4, const float* dy, int B, float* gw5,
-                                                   float* gb5) {
-  __shared__ float red[4];
-  const int j = blockIdx.x;
-  float s = 0.f;
-  for (int b = threadIdx.x; b < B; b += blockDim.x)
-    s += (j < MLP_H ? bf2f(act4[(long)b * MLP_H + j]) : 1.f) * dy[b];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const float tot = red[0] + red[1] + red[2] + red[3];
-    if (j < MLP_H) gw5[j] = tot;
-    else gb5[0] = tot;
+// Output-layer weight and bias gradients, part 1: block z sums rows
+// 64 z .. 64 z + 63 of act4 x dy per column (column MLP_H: dy itself, the
+// bias); the blocks' partial sums are added in order by k_splitk_sum.
+__global__ void __launch_bounds__(256) k_out_wgrad_part(const uint16_t* act4, const float* dy, int B,
+                                                        float* part) {
+  const int j = threadIdx.x, r0 = blockIdx.x * 64;
+  const int r1 = r0 + 64 < B ? r0 + 64 : B;
+  float s = 0.f, sb = 0.f;
+#pragma unroll 16
+  for (int r = r0; r < r1; r++) {
+    const float d = dy[r];
+    s += bf2f(act4[(long)r * MLP_H + j]) * d;
+    sb += d;
   }
+  part[(long)blockIdx.x * (MLP_H + 1) + j] = s;
+  if (j == 0) part[(long)blockIdx.x * (MLP_H + 1) + MLP_H] = sb;
 }
 
 __global__ void k_silu_back(const float* da, const float* z, long n, uint16_t* dz) {
@@ -196,7 +301,9 @@ __global__ void k_silu_back(const float* da, const float* z, long n, uint16_t* d
 __global__ void __launch_bounds__(256) k_colsum_part(const uint16_t* dz, int B, float* part) {
   const int j = threadIdx.x, r0 = blockIdx.x * 64;
   float s = 0.f;
-  for (int r = r0; r < r0 + 64 && r < B; r++) s += bf2f(dz[(long)r * MLP_H + j]);
+  const int r1 = r0 + 64 < B ? r0 + 64 : B;
+#pragma unroll 16
+  for (int r = r0; r < r1; r++) s += bf2f(dz[(long)r * MLP_H + j]);
   part[(long)blockIdx.x * MLP_H + j] = s;
 }
 
@@ -347,7 +454,14 @@ static inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
 
 void launch_gemm(const GemmArgs& g, hipStream_t s) {
   dim3 grid((g.N + TS - 1) / TS, (g.M + TS - 1) / TS, g.ksplit);
-  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
+  // the vectorised kernel where its layout conditions hold
+  const bool al = ((uintptr_t)g.A % 16 == 0) && ((uintptr_t)g.B % 16 == 0);
+  const bool la = g.sak == 1 ? g.sam % 8 == 0 : (g.sam == 1 && g.sak % 8 == 0);
+  const bool lb = g.sbk == 1 ? g.sbn % 8 == 0 : (g.sbn == 1 && g.sbk % 8 == 0);
+  if (al && la && lb && g.M % 8 == 0 && g.N % 8 == 0 && g.K % g.ksplit == 0 && (g.K / g.ksplit) % TK2 == 0)
+    hipLaunchKernelGGL(k_gemm_v, grid, dim3(256), 0, s, g);
+  else
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
 }
 void launch_splitk_sum(const float* part, int nslab, long slab, long n, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_splitk_sum, dim3(nb256(n)), dim3(256), 0, s, part, nslab, slab, n, out);
@@ -365,9 +479,14 @@ void launch_loss_grad(const float* y, const float* t, int B, float* dy, float* l
   hipLaunchKernelGGL(k_loss_grad, dim3(1), dim3(1024), 0, s, y, t, B, dy, loss_hist, step, mask);
 }
 void launch_out_bwd(const uint16_t* act4, const float* z4, const float* dy, const uint16_t* w5, int B,
-                    float* gw5, float* gb5, uint16_t* dz4, hipStream_t s) {
+                    float* gw5, float* gb5, uint16_t* dz4, float* part, hipStream_t s) {
   hipLaunchKernelGGL(k_out_dz, dim3(nb256((long)B * MLP_H)), dim3(256), 0, s, z4, dy, w5, B, dz4);
-  hipLaunchKernelGGL(k_out_wgrad, dim3(MLP_H + 1), dim3(256), 0, s, act4, dy, B, gw5, gb5);
+  // gw5 and gb5 are neighbours in the parameter layout (MLP_B5 = MLP_W5 + MLP_H)
+  const int nblk = (B + 63) / 64;
+  hipLaunchKernelGGL(k_out_wgrad_part, dim3(nblk), dim3(256), 0, s, act4, dy, B, part);
+  hipLaunchKernelGGL(k_splitk_sum, dim3(nb256(MLP_H + 1)), dim3(256), 0, s, part, nblk, (long)(MLP_H + 1),
+                     (long)(MLP_H + 1), gw5);
+  (void)gb5;
 }
 void launch_silu_back(const float* da, const float* z, long n, uint16_t* dz, hipStream_t s) {
   hipLaunchKernelGGL(k_silu_back, dim3(nb256(n)), dim3(256), 0, s, da, z, n, dz);

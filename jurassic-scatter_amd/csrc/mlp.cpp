@@ -82,7 +82,7 @@ void ensure_buffers(int B) {
   m.y.need((size_t)B * 4);
   m.dy.need((size_t)B * 4);
   const size_t slab = (size_t)MLP_H * MLP_H;
-  m.part.need(std::max((size_t)16 * slab, (size_t)((B + 63) / 64) * MLP_H) * 4);
+  m.part.need(std::max((size_t)16 * slab, (size_t)((B + 63) / 64) * (MLP_H + 1)) * 4);
   m.loss.need(LOSS_CAP * 4);
   for (int i = 0; i < 16; i++)
     if (all[i]->p != before[i]) {
@@ -131,7 +131,7 @@ void enqueue_step(int B, float lr, bool gather, bool update, hipStream_t s) {
                    (int)(LOSS_CAP - 1), s);
   // backward
   launch_out_bwd(m.act[3].as<uint16_t>(), m.z[3].as<float>(), m.dy.as<float>(), W + MLP_W5, B, G + MLP_W5,
-                 G + MLP_B5, m.dz.as<uint16_t>(), s);
+                 G + MLP_B5, m.dz.as<uint16_t>(), m.part.as<float>(), s);
   const int ks = ksplit_for(B);
   const long slab = (long)MLP_H * MLP_H;
   for (int l = MLP_NHID - 1; l >= 0; l--) {
