@@ -78,8 +78,9 @@ MEASURED_TRAFFIC = {
 }
 # The FP32 path kernel of each autotuner variant (7-9: the staged kernel,
 # grids in LDS, lockstep bracket searches, 9 with the tiled block mapping;
-# 0-6: the fast kernel).
-PATH_KERNEL = {7: "k_path_stage", 8: "k_path_stage", 9: "k_path_stage"}
+# 10: the same on the packed stream of k_pack (lane-contiguous records, (p, T)
+# brackets precomputed per point and emitter), whose time is included; 0-6: the fast kernel).
+PATH_KERNEL = {7: "k_path_stage", 8: "k_path_stage", 9: "k_path_stage", 10: "k_pack + k_path_stage"}
 # The same for the FP64 path kernel of kernel() on the Jacobian workload.
 # HBM bytes per kernel() call of the path kernels (FETCH_SIZE / WRITE_SIZE
 # passes): with base-ray reuse (the default), and integrating every copy

@@ -192,11 +192,15 @@ int jur_tab2bin(ctl_t *ctl);
 /* Select the GPU (call before anything else; default device 0). */
 int jur_init(int device);
 
-/* Tuning.  Options: "variant" path-kernel variant (channels per wave x
-   occupancy target, 0-4; default: autotuned), "wpb" waves per block (1, 2,
-   4; default: autotuned), "autotune" (0/1, default 1), "fast" (0: always use
-   the generic table search), "cap" initial LOS points per ray, "chunk_mb"
-   trace-buffer budget of the scattering levels in MiB (default 6144),
+/* Tuning.  Options: "variant" FP32 path-kernel variant (0-6 the fast
+   kernel at several channels per wave x occupancy targets, 7-9 the staged
+   kernel, 10 the staged kernel on a packed lane-contiguous copy of the
+   point records with the (p, T) brackets precomputed once per LOS point and
+   emitter; all give bit-identical radiances; default: autotuned),
+   "wpb" waves per block (1, 2, 4; default: autotuned), "autotune" (0/1,
+   default 1), "fast" (0: always use the generic table search), "cap"
+   initial LOS points per ray, "chunk_mb" trace-buffer budget of the
+   scattering levels in MiB (default 13312),
    "graph" (1: a clear-air call is recorded once as a hipGraph -- trace,
    sort, path integration -- and replayed while the inputs, outputs and
    work buffers keep their addresses; a replay whose rays overflow the

@@ -41,6 +41,7 @@ struct Engine {
   double top[NLMAX] = {0}, bottom[NLMAX] = {0}, trans[NLMAX] = {0};
   // work buffers of the primary pass
   DBuf geo, geo2, rec, npts, tsurf, status, ids, order, keys, sorttmp;
+  DBuf pack;  // packed point records and (p, T) brackets of the path integration
   int cap = 512;
   // FP64 point records and surface temperatures of the last trace (option
   // "fp64"); the next path integration reads them
@@ -685,14 +686,14 @@ std::string graph_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& r
   Engine& e = E();
   char k[2048];
   int n = snprintf(k, sizeof k, "%s|%s|%d %d %d %d|%p %p %p %p %p|%p %p %p %p %p %p %p %p|%p %p %p %p %p %p|"
-                   "%p %p %p %p %p %p %p %p %p|%p|%d %.17g %.17g %.17g %d %d|%s|%d %d %d|",
+                   "%p %p %p %p %p %p %p %p %p %p|%p|%d %.17g %.17g %.17g %d %d|%s|%d %d %d|",
                    e.tbl_key.c_str(), e.chan_key.c_str(), rays.nray, atm.natm, atm.nlev, scattering,
                    (const void*)atm.z, (const void*)atm.p, (const void*)atm.t, (const void*)atm.q,
                    (const void*)atm.k, (const void*)rays.time, (const void*)rays.obsz, (const void*)rays.obslon,
                    (const void*)rays.obslat, (const void*)rays.vpz, (const void*)rays.vplon,
                    (const void*)rays.vplat, (const void*)rays.atm_idx, (void*)out.rad, (void*)out.tau,
                    (void*)out.tpz, (void*)out.tplon, (void*)out.tplat, (void*)out.npts, e.geo.p, e.geo2.p, e.rec.p,
-                   e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, (void*)s, ctl->sca_n,
+                   e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, e.pack.p, (void*)s, ctl->sca_n,
                    ctl->rayds, ctl->raydz, ctl->transs, ctl->refrac, ctl->nw + 1000 * out.ld, ctl->sca_ext,
                    engine_option_int("fast", 1), engine_option_int("variant", -1), engine_option_int("wpb", -1));
   std::string key(k, n < (int)sizeof k ? n : (int)sizeof k - 1);
@@ -955,6 +956,10 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   PathArgs P = path_args(ctl, nray, cap, rec, npts, tsurf, order, scattering, J, jbase, rad, tau, out_ld, out_off,
                          rad64, tau64);
   P.nJ = J ? nJ : 0;
+  // packed point records and (p, T) brackets for the FP32 kernel variant
+  // that reads them (R + 4 ng floats per LOS point slot of every ray group)
+  if (!P.fp64 && P.T.shared && ctl->ng > 0)
+    P.pack = (float*)e.pack.need((size_t)(nray + 63) / 64 * 64 * cap * (P.R + 4 * ctl->ng) * sizeof(float));
   int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
   if (P.fp64) {
     // the FP64 kernels have one configuration (variant and waves per block

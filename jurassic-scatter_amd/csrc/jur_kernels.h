@@ -87,6 +87,11 @@ struct PathArgs {
   const double* tsurf64;
   const ChanConst64* cc64; // [nd] (fp64)
   const double* planck64;  // [nd][JUR_NPLANCK] (fp64)
+  // packed records and (p, T) brackets per ray group (64 sorted rays) and
+  // point slot, [ngroups][cap][R + 4 ng][64] floats, written by the path
+  // launch itself for variant 10 (optional: without it variant 10 runs as
+  // variant 9)
+  float* pack;
 };
 int path_variants();
 // Launch the path integration with the given waves per block.

@@ -209,14 +209,15 @@ __device__ inline bool map_work_tiled(const PathArgs& A, int cpw, int& chunk, in
 
 // Radiance and transmittance update of one ray x channel at one LOS point
 // (forwardmodel.c:332-389) given the gas segment transmittance tg = tau_gas
-// and its complement et = 1 - tau_gas.
+// and its complement et = 1 - tau_gas.  pr: the point's record, fields rs
+// floats apart.
 __device__ inline void point_radiance(const PathArgs& A, const ChanConst& cc, int id, const float* pr, int uo,
                                       float p, float t, float ds, float qh, float af, int ai, bool scpt, int kj,
                                       int itp, float dtp, float tg, float et, float rt, float rds, float& radv,
-                                      float& comp, float& tau) {
-  const float uco2 = cc.co2_on ? pr[uo + cc.co2_ig] : 0.f;
-  const float uh2o = cc.h2o_on ? pr[uo + cc.h2o_ig] : 0.f;
-  const float kw = A.nw > 0 ? pr[REC_K + cc.window] : 0.f;
+                                      float& comp, float& tau, int rs = 1) {
+  const float uco2 = cc.co2_on ? pr[(uo + cc.co2_ig) * rs] : 0.f;
+  const float uh2o = cc.h2o_on ? pr[(uo + cc.h2o_ig) * rs] : 0.f;
+  const float kw = A.nw > 0 ? pr[(REC_K + cc.window) * rs] : 0.f;
   const float beta = continua(cc, p, t, ds, qh, uco2, uh2o, kw, rt, rds);
   const float B = planck_at(A.planck + (size_t)id * JUR_NPLANCK, itp, dtp);
   if (!(tg > 0.f)) return;
@@ -744,26 +745,33 @@ __device__ inline int grid_bracket(const float* g, int n, float x, int h) {
   return locate_mono<float>(g, n, x);
 }
 
-template <int NGT, int CPW, int OCC, bool TILE = false>
+// PACK: the point records come from the ray group's packed stream written
+// by k_pack (A.pack: lane-contiguous, so every per-point load of the wave is
+// one contiguous 256-byte or 1-KiB access instead of 64 scattered ones), and
+// the (p, T) brackets and weights of every (point, emitter) come precomputed
+// in that stream instead of from the grids in LDS.
+template <int NGT, int CPW, int OCC, bool TILE = false, bool PACK = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) k_path_stage(PathArgs A) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const DevTables& T = A.T;
-  const int NPS = T.gnp_max, NTS = T.gnt_max;
+  const int NPS = PACK ? 0 : T.gnp_max, NTS = PACK ? 0 : T.gnt_max;
   float* sgp = (float*)lds_raw;                 // [NGT][NPS] pressure grids
   int* sgnt = (int*)(sgp + NGT * NPS);          // [NGT][NPS] temperatures per pressure
   float* sgt = (float*)(sgnt + NGT * NPS);      // [NGT][NPS][NTS] temperature grids
   const int goff = (2 * NGT * NPS + NGT * NPS * NTS + 1) & ~1;
-  for (int i = threadIdx.x; i < NGT * NPS; i += blockDim.x) {
-    const int g = i / NPS, ip = i - g * NPS;
-    const bool in = g < A.ng;
-    sgp[i] = in ? T.gp[g * TBLNPMAX + ip] : 0.f;
-    sgnt[i] = in ? T.gnt[g * TBLNPMAX + ip] : 0;
+  if (!PACK) {
+    for (int i = threadIdx.x; i < NGT * NPS; i += blockDim.x) {
+      const int g = i / NPS, ip = i - g * NPS;
+      const bool in = g < A.ng;
+      sgp[i] = in ? T.gp[g * TBLNPMAX + ip] : 0.f;
+      sgnt[i] = in ? T.gnt[g * TBLNPMAX + ip] : 0;
+    }
+    for (int i = threadIdx.x; i < NGT * NPS * NTS; i += blockDim.x) {
+      const int g = i / (NPS * NTS), r = i - g * NPS * NTS, ip = r / NTS, it = r - ip * NTS;
+      sgt[i] = g < A.ng ? T.gt[(size_t)(g * TBLNPMAX + ip) * TBLNTMAX + it] : 0.f;
+    }
+    __syncthreads();
   }
-  for (int i = threadIdx.x; i < NGT * NPS * NTS; i += blockDim.x) {
-    const int g = i / (NPS * NTS), r = i - g * NPS * NTS, ip = r / NTS, it = r - ip * NTS;
-    sgt[i] = g < A.ng ? T.gt[(size_t)(g * TBLNPMAX + ip) * TBLNTMAX + it] : 0.f;
-  }
-  __syncthreads();
 
   const int lane = threadIdx.x & 63;
   int chunk, group;
@@ -801,20 +809,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 #pragma unroll
   for (int g = 0; g < NGT; g++) hpt[g] = 0;
   int kj = (valid && A.scattering > 0) ? A.jbase[ray] : 0;
-  const float* rp = A.rec + (size_t)ray * A.cap * A.R;
+  // point records: the ray's own [cap][R] (stride 1 between fields), or
+  // the group's packed stream [cap][R + 4 ng][64] (stride 64)
+  const int RS = PACK ? 64 : 1;
+  const size_t PF = PACK ? (size_t)(A.R + 4 * A.ng) * 64 : (size_t)A.R;  // floats per point slot
+  const float* rp = PACK ? A.pack + (size_t)group * A.cap * PF + lane : A.rec + (size_t)ray * A.cap * A.R;
 
   // the next point's record, loaded one iteration ahead
   float np_p, np_t, np_ds, np_af, np_qh, np_ai, np_du[NGT];
   {
     const float* pr = rp;
-    np_p = pr[REC_P];
-    np_t = pr[REC_T];
-    np_ds = pr[REC_DS];
-    np_af = pr[REC_AF];
-    np_qh = pr[REC_QH2O];
-    np_ai = pr[REC_AI];
+    np_p = pr[REC_P * RS];
+    np_t = pr[REC_T * RS];
+    np_ds = pr[REC_DS * RS];
+    np_af = pr[REC_AF * RS];
+    np_qh = pr[REC_QH2O * RS];
+    np_ai = pr[REC_AI * RS];
 #pragma unroll
-    for (int g = 0; g < NGT; g++) np_du[g] = g < A.ng ? pr[uo + g] : 0.f;
+    for (int g = 0; g < NGT; g++) np_du[g] = g < A.ng ? pr[(uo + g) * RS] : 0.f;
   }
 
   for (int ip = 0;; ip++) {
@@ -822,23 +834,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
     if (!__any(act)) break;
     const float p = np_p, t = np_t, ds = np_ds, af = np_af, qh = np_qh, aif = np_ai;
     float dus[NGT];
+    int4 bks[NGT];
+    // precomputed brackets of this point (ip < cap: some lane of the wave
+    // still has points), loaded before the next point's record so that
+    // waiting for them does not wait for the prefetch
+    const int4* bq = PACK ? (const int4*)(rp - lane + (size_t)ip * PF + (size_t)A.R * 64) + lane : nullptr;
 #pragma unroll
-    for (int g = 0; g < NGT; g++) dus[g] = np_du[g];
+    for (int g = 0; g < NGT; g++) {
+      dus[g] = np_du[g];
+      bks[g] = PACK && g < A.ng ? bq[g * 64] : make_int4(0, 0, 0, 0);
+    }
     {
       // prefetch (the last point of the ray reloads itself)
       const int in = ip + 1 < npts ? ip + 1 : (npts > 0 ? npts - 1 : 0);
-      const float* pn = rp + (size_t)in * A.R;
-      np_p = pn[REC_P];
-      np_t = pn[REC_T];
-      np_ds = pn[REC_DS];
-      np_af = pn[REC_AF];
-      np_qh = pn[REC_QH2O];
-      np_ai = pn[REC_AI];
+      const float* pn = rp + (size_t)in * PF;
+      np_p = pn[REC_P * RS];
+      np_t = pn[REC_T * RS];
+      np_ds = pn[REC_DS * RS];
+      np_af = pn[REC_AF * RS];
+      np_qh = pn[REC_QH2O * RS];
+      np_ai = pn[REC_AI * RS];
 #pragma unroll
-      for (int g = 0; g < NGT; g++) np_du[g] = g < A.ng ? pn[uo + g] : 0.f;
+      for (int g = 0; g < NGT; g++) np_du[g] = g < A.ng ? pn[(uo + g) * RS] : 0.f;
     }
     if (act) {
-      const float* pr = rp + (size_t)ip * A.R;
+      const float* pr = rp + (size_t)ip * PF;
       int itp;
       float dtp;
       planck_index(t, itp, dtp);
@@ -854,22 +874,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 #pragma unroll
       for (int g = 0; g < NGT; g++) {
         if (g >= A.ng) continue;
+        const int4 bc = bks[g];
         const int np_ = gnp[g];
         if (np_ < 2) continue;
         const float du = dus[g];
-        // (p, T) brackets and weights from the staged grids
-        const float* pg = sgp + g * NPS;
-        const int ipr = grid_bracket(pg, np_, p, hpt[g] & 0xff);
-        const int nt0 = sgnt[g * NPS + ipr], nt1 = sgnt[g * NPS + ipr + 1];
-        const bool gok = nt0 >= 2 && nt1 >= 2;
-        const float* t0g = sgt + (g * NPS + ipr) * NTS;
-        const float* t1g = t0g + NTS;
-        const int it0 = grid_bracket(t0g, gok ? nt0 : 2, t, (hpt[g] >> 8) & 0xff);
-        const int it1 = grid_bracket(t1g, gok ? nt1 : 2, t, (hpt[g] >> 16) & 0xff);
-        if (gok) hpt[g] = ipr | (it0 << 8) | (it1 << 16);
-        const float wt0 = fdiv(t - t0g[it0], t0g[it0 + 1] - t0g[it0]);
-        const float wt1 = fdiv(t - t1g[it1], t1g[it1 + 1] - t1g[it1]);
-        const float wp = fdiv(p - pg[ipr], pg[ipr + 1] - pg[ipr]);
+        int ipr, it0, it1;
+        bool gok;
+        float wt0, wt1, wp;
+        if (PACK) {
+          ipr = bc.x & 0xff;
+          it0 = (bc.x >> 8) & 0xff;
+          it1 = (bc.x >> 16) & 0xff;
+          gok = (bc.x >> 24) != 0;
+          wt0 = __int_as_float(bc.y);
+          wt1 = __int_as_float(bc.z);
+          wp = __int_as_float(bc.w);
+        } else {
+          // (p, T) brackets and weights from the staged grids
+          const float* pg = sgp + g * NPS;
+          ipr = grid_bracket(pg, np_, p, hpt[g] & 0xff);
+          const int nt0 = sgnt[g * NPS + ipr], nt1 = sgnt[g * NPS + ipr + 1];
+          gok = nt0 >= 2 && nt1 >= 2;
+          const float* t0g = sgt + (g * NPS + ipr) * NTS;
+          const float* t1g = t0g + NTS;
+          it0 = grid_bracket(t0g, gok ? nt0 : 2, t, (hpt[g] >> 8) & 0xff);
+          it1 = grid_bracket(t1g, gok ? nt1 : 2, t, (hpt[g] >> 16) & 0xff);
+          if (gok) hpt[g] = ipr | (it0 << 8) | (it1 << 16);
+          wt0 = fdiv(t - t0g[it0], t0g[it0 + 1] - t0g[it0]);
+          wt1 = fdiv(t - t1g[it1], t1g[it1 + 1] - t1g[it1]);
+          wp = fdiv(p - pg[ipr], pg[ipr + 1] - pg[ipr]);
+        }
         const int gb0 = ipr * TBLNTMAX + it0, gb1 = (ipr + 1) * TBLNTMAX + it1;
         // all channels: metadata, hints and the hinted bracket nodes
         int nn[CPW][4], bb[CPW][4], hh[CPW][4];
@@ -950,7 +984,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
         const int id = c0 + c;
         if (id < A.nd)
           point_radiance(A, A.cc[id], id, pr, uo, p, t, ds, qh, af, ai, scpt, kj, itp, dtp, tg[c], et[c], rtp, rdsp,
-                         radv[c], comp[c], tau[c]);
+                         radv[c], comp[c], tau[c], RS);
       }
       if (scpt) kj++;
     }
@@ -960,11 +994,66 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 }
 
 // LDS bytes of k_path_stage: staged grids + hints; 0 if the grids are too
-// large to stage (the fast kernel runs instead)
-size_t stage_lds_bytes(const PathArgs& A, int ngt, int cpw, int wpb) {
+// large to stage (the fast kernel runs instead).  With the packed stream
+// (precomputed brackets) the hints only.
+size_t stage_lds_bytes(const PathArgs& A, int ngt, int cpw, int wpb, bool pack = false) {
+  if (A.T.gnp_max < 2 || A.T.gnt_max < 2) return 0;
+  const size_t hints = (size_t)wpb * cpw * ngt * 64 * 8;
+  if (pack) return hints;
   const size_t gw = (size_t)2 * ngt * A.T.gnp_max + (size_t)ngt * A.T.gnp_max * A.T.gnt_max;
-  if (gw * 4 > 24576 || A.T.gnp_max < 2 || A.T.gnt_max < 2) return 0;
-  return ((gw + 1) & ~(size_t)1) * 4 + (size_t)wpb * cpw * ngt * 64 * 8;
+  if (gw * 4 > 24576) return 0;
+  return ((gw + 1) & ~(size_t)1) * 4 + hints;
+}
+
+// Packed stream of a ray group for k_path_stage<..., PACK>: per group of 64
+// consecutive rays of the sorted list and LOS point slot, the rays' point
+// records lane-contiguous ([R][64] floats) followed by the (p, T) brackets
+// and interpolation weights of every emitter ([ng][64] int4: ipr | it0 << 8
+// | it1 << 16 | gok << 24, wt0, wt1, wp; gok = both pressure rows hold >= 2
+// temperatures) on the shared table grids (forwardmodel.c:442-446 and the
+// weights of the LIN calls at :477-491), computed once per (point, emitter)
+// for all channels.  The searches are the staged kernel's own (grid_bracket:
+// on a strictly increasing grid the unique bracket, whatever the hint) and
+// the quotients the same, so radiances are bit-identical.  One thread per
+// (group, point slot, lane): a wave writes one slot's 64 lanes.
+static_assert(TBLNPMAX <= 256 && TBLNTMAX <= 256, "bracket indices are packed in 8 bits");
+__global__ void __launch_bounds__(256) k_pack(PathArgs A) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ngroups = (A.nray + 63) >> 6;
+  if (t >= (long)ngroups * A.cap * 64) return;
+  const int lane = (int)(t & 63);
+  const long q = t >> 6;
+  const int group = (int)(q / A.cap), ip = (int)(q - (long)group * A.cap);
+  const int slot = group * 64 + lane;
+  if (slot >= A.nray) return;
+  const int ray = A.order[slot];
+  if (ip >= A.np[ray]) return;
+  const DevTables& T = A.T;
+  const float* pr = A.rec + ((size_t)ray * A.cap + ip) * A.R;
+  float* out = A.pack + ((size_t)group * A.cap + ip) * (size_t)(A.R + 4 * A.ng) * 64 + lane;
+  for (int f = 0; f < A.R; f++) out[f * 64] = pr[f];
+  int4* ob = (int4*)(out - lane + (size_t)A.R * 64) + lane;
+  const float p = pr[REC_P], tt = pr[REC_T];
+  for (int g = 0; g < A.ng; g++) {
+    const int np_ = T.gnp[g];
+    if (np_ < 2) {
+      ob[g * 64] = make_int4(0, 0, 0, 0);
+      continue;
+    }
+    const float* pg = T.gp + g * TBLNPMAX;
+    const int ipr = grid_bracket(pg, np_, p, 0);
+    const int nt0 = T.gnt[g * TBLNPMAX + ipr], nt1 = T.gnt[g * TBLNPMAX + ipr + 1];
+    const bool gok = nt0 >= 2 && nt1 >= 2;
+    const float* t0g = T.gt + (size_t)(g * TBLNPMAX + ipr) * TBLNTMAX;
+    const float* t1g = t0g + TBLNTMAX;
+    const int it0 = grid_bracket(t0g, gok ? nt0 : 2, tt, 0);
+    const int it1 = grid_bracket(t1g, gok ? nt1 : 2, tt, 0);
+    const float wt0 = fdiv(tt - t0g[it0], t0g[it0 + 1] - t0g[it0]);
+    const float wt1 = fdiv(tt - t1g[it1], t1g[it1 + 1] - t1g[it1]);
+    const float wp = fdiv(p - pg[ipr], pg[ipr + 1] - pg[ipr]);
+    ob[g * 64] = make_int4(ipr | (it0 << 8) | (it1 << 16) | ((gok ? 1 : 0) << 24), __float_as_int(wt0),
+                           __float_as_int(wt1), __float_as_int(wp));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1582,12 +1671,22 @@ void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
   }
 }
 
-template <int NGT, int CPW, int OCC, bool STAGE = false, bool TILE = false>
+template <int NGT, int CPW, int OCC, bool STAGE = false, bool TILE = false, bool PACK = false>
 void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   int nchunk = (A.nd + CPW - 1) / CPW;
   int ngroups = (A.nray + 63) / 64;
   long nb = (long)nchunk * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;  // a multiple of the XCD count for the logical remap
+  if (PACK && A.T.shared && A.pack) {
+    const size_t lds = stage_lds_bytes(A, NGT, CPW, wpb, true);
+    if (lds > 0) {
+      const long nt = (long)ngroups * A.cap * 64;
+      hipLaunchKernelGGL(k_pack, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, A);
+      hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC, TILE, true>), dim3((unsigned)nb), dim3(64 * wpb),
+                         (unsigned)lds, s, A);
+      return;
+    }
+  }
   const size_t lds = STAGE && A.T.shared ? stage_lds_bytes(A, NGT, CPW, wpb) : 0;
   if (lds > 0)
     hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC, TILE>), dim3((unsigned)nb), dim3(64 * wpb), (unsigned)lds, s,
@@ -1599,8 +1698,9 @@ void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
 }
 
 // Variants (kernel, channels per wave, occupancy target) picked by the
-// tuner: 0-6 the fast kernel, 7-8 the staged kernel (the fast one where the
-// grids are too large to stage).
+// tuner: 0-6 the fast kernel, 7-9 the staged kernel (the fast one where the
+// grids are too large to stage), 10 the staged kernel on the packed stream
+// of k_pack (variant 9 where no stream buffer is given).
 template <int NGT>
 void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
   switch (variant) {
@@ -1613,13 +1713,14 @@ void launch_ng(const PathArgs& A, int wpb, int variant, hipStream_t s) {
     case 7: launch_t<NGT, 1, 4, true>(A, wpb, s); break;
     case 8: launch_t<NGT, 2, 3, true>(A, wpb, s); break;
     case 9: launch_t<NGT, 1, 4, true, true>(A, wpb, s); break;
+    case 10: launch_t<NGT, 1, 4, true, true, true>(A, wpb, s); break;
     default: launch_t<NGT, 2, 5>(A, wpb, s); break;
   }
 }
 
 }  // namespace
 
-int path_variants() { return 10; }
+int path_variants() { return 11; }
 
 void launch_path(const PathArgs& A, int wpb, hipStream_t s) {
   if (A.nray <= 0 || A.nd <= 0) return;

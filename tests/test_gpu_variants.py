@@ -1,9 +1,10 @@
 """Every variant of the FP32 path kernel gives the same radiances, bit for bit.
 
 The variants differ in work mapping (channels per wave, waves per block,
-occupancy target) and, for the staged kernel (variants 7-9), in how the
+occupancy target) and, for the staged kernel (variants 7-10), in how the
 table brackets are found (grids in LDS, per-corner hints, the four corners
-searched in lockstep); the arithmetic is the same, so radiances and
+searched in lockstep; variant 10: the (p, T) brackets and weights computed
+once per LOS point and emitter by k_brackets); the arithmetic is the same, so radiances and
 transmittances must be identical to the default fast kernel's (variant 6)
 on every ray and channel.  Cases: a
 4,096-ray slice of the 65,536-ray batch (synthetic tables, 5 emitters, 64
@@ -22,7 +23,7 @@ from pyjurassic import synth
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = list(range(10))
+VARIANTS = list(range(11))
 
 
 @pytest.fixture(scope="module")
