@@ -200,7 +200,8 @@ int jur_init(int device);
    "wpb" waves per block (1, 2, 4; default: autotuned), "autotune" (0/1,
    default 1), "fast" (0: always use the generic table search), "cap"
    initial LOS points per ray, "chunk_mb" trace-buffer budget of the
-   scattering levels in MiB (default 13312),
+   scattering levels in MiB (default: a sixth of the device's memory,
+   at most 48 GiB),
    "graph" (1: a clear-air call is recorded once as a hipGraph -- trace,
    sort, path integration -- and replayed while the inputs, outputs and
    work buffers keep their addresses; a replay whose rays overflow the

@@ -68,6 +68,7 @@ struct Engine {
   std::string g_key;
   hipGraph_t g_graph = nullptr;
   hipGraphExec_t g_exec = nullptr;
+  size_t hbm_bytes = 0;  // the device's memory
 };
 
 Engine& E() {
@@ -78,6 +79,8 @@ Engine& E() {
       JUR_FAIL("No GPU found: the forward model needs an MI355X (gfx950) device.");
     HIP_OK(hipSetDevice(e.device));
     HIP_OK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    size_t fr = 0;
+    HIP_OK(hipMemGetInfo(&fr, &e.hbm_bytes));
     e.ready = true;
   }
   return e;
@@ -212,6 +215,16 @@ ChanConst64 make_chan_const64(const ctl_t* ctl, int id) {
 
 hipStream_t engine_stream() { return E().stream; }
 
+// Trace-buffer budget of a scattering level's chunks: option "chunk_mb", by
+// default a sixth of the device's memory, at most 48 GiB (48 GiB of 288 on
+// an MI355X: ~400,000 pencils of 512 LOS points per chunk, several waves
+// per SIMD for the thread-per-pencil march).
+size_t engine_chunk_bytes() {
+  const int mb = engine_option_int("chunk_mb", 0);
+  if (mb > 0) return (size_t)mb << 20;
+  const size_t cap = (size_t)48 << 30, sixth = E().hbm_bytes / 6;
+  return sixth > 0 && sixth < cap ? sixth : cap;
+}
 int engine_option_int(const char* key, int dflt) {
   auto& m = E().opt;
   auto it = m.find(key);
@@ -1169,6 +1182,8 @@ extern "C" int jur_init(int device) {
     HIP_OK(hipSetDevice(device));
     HIP_OK(hipStreamDestroy(e.stream));
     HIP_OK(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    size_t fr = 0;
+    HIP_OK(hipMemGetInfo(&fr, &e.hbm_bytes));
     e.tbl_key.clear();
     e.chan_key.clear();
     e.aero_ready = false;

@@ -109,6 +109,7 @@ void engine_forward(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& ray
                     int scattering, const OutDev& out, hipStream_t s);
 hipStream_t engine_stream();
 int engine_option_int(const char* key, int dflt);
+size_t engine_chunk_bytes();
 // FP64 clear-air calls whose rays are copies of the first nr rays (the
 // Jacobian's perturbed states): each ray is integrated from its first LOS
 // point that differs from its base ray (0: off).

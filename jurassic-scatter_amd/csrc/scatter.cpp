@@ -134,7 +134,7 @@ RaySetDev sub(const RaySetDev& R, int r0, int n) {
 int chunk_rays(const ctl_t* ctl, long n, int cap) {
   const int R = rec_len(ctl->ng, ctl->nw);
   const size_t per = (size_t)cap * (2 * sizeof(double4) + 2 * R * sizeof(float) + ctl->ng * sizeof(int4)) + 64;
-  const size_t budget = (size_t)engine_option_int("chunk_mb", 13312) << 20;
+  const size_t budget = engine_chunk_bytes();
   size_t c = budget / per;
   if (c < 64) c = 64;
   return (int)(c < (size_t)n ? c : (size_t)n);
