@@ -90,5 +90,6 @@ struct InferArgs {
   const float* params;   // FP32 master (biases, b5)
   const uint16_t* wbf;   // BF16 copy (W5)
   float* y;
+  int variant;           // 1: a wave takes all 128 samples of its block x 64 neurons; 0: 64 samples x 128 neurons
 };
 void launch_mlp_infer(const InferArgs& a, hipStream_t s);

@@ -344,10 +344,15 @@ __global__ void k_fill_f32(float* p, long n, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: 128 samples per block, 4 waves; wave w computes samples
-// 64 (w & 1) .. +64 (two 32-sample tiles) and neurons 128 (w >> 1) .. +128
-// (four 32-neuron tiles) of every hidden layer, so each 16-byte weight
-// fragment read from L2 feeds two matrix-core operations.  The weights are
+// Fused inference: 128 samples per block, 4 waves.  Default (variant 1,
+// hidden_layer_w below): wave w computes all 128 samples (four 32-sample
+// tiles) for neurons 64 w .. +64 (two 32-neuron tiles) of every hidden
+// layer, so each 16-byte weight fragment is read from L2 once per block and
+// feeds four matrix-core operations.  Variant 0 (hidden_layer): wave w
+// computes samples 64 (w & 1) .. +64 and neurons 128 (w >> 1) .. +128, each
+// fragment read by two waves and feeding two operations.  The layers are
+// inlined into the kernel (as out-of-line calls they cost a scratch stack
+// and 40 % of the time).  The weights are
 // the A operand and the activations the B operand, so a lane's accumulator
 // holds four consecutive neurons of one sample: the epilogue converts them
 // with the packed BF16 instruction and stores 8 bytes at a time.  The
@@ -369,8 +374,8 @@ __device__ inline uint32_t pack_bf16(float lo, float hi) {
 }
 
 template <int K>
-__device__ inline void hidden_layer(uint16_t (*act)[LDH], const uint16_t* wt, const float* bias, int wm, int nb,
-                                    int lane) {
+__device__ __forceinline__ void hidden_layer(uint16_t (*act)[LDH], const uint16_t* wt, const float* bias, int wm,
+                                             int nb, int lane) {
   const int h = lane >> 5, r = lane & 31;
   f32x16 acc[2][4];  // [sample tile][neuron tile]
 #pragma unroll
@@ -418,10 +423,74 @@ __device__ inline void hidden_layer(uint16_t (*act)[LDH], const uint16_t* wt, co
     }
   __syncthreads();
 }
+// The same layer with wave w computing every sample of the block (four
+// 32-sample tiles) for neurons 64 w .. +64 (two 32-neuron tiles): each
+// weight fragment is read from L2 once per block (not once per wave pair)
+// and feeds four matrix-core operations; the weight fragments are loaded
+// two K steps ahead.
+template <int K>
+__device__ __forceinline__ void hidden_layer_w(uint16_t (*act)[LDH], const uint16_t* wt, const float* bias, int nb,
+                                               int lane) {
+  const int h = lane >> 5, r = lane & 31;
+  f32x16 acc[4][2];  // [sample tile][neuron tile]
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int t = 0; t < 2; t++) acc[u][t] = zero16();
+  const uint16_t* wrow = wt + (long)(nb + r) * K + 8 * h;
+  bf16x8 w1[2], w2[2];
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    w1[t] = *(const bf16x8*)&wrow[(long)32 * t * K];
+    if (16 < K) w2[t] = *(const bf16x8*)&wrow[(long)32 * t * K + 16];
+  }
+  for (int kk = 0; kk < K; kk += 16) {
+    bf16x8 wc[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      wc[t] = w1[t];
+      w1[t] = w2[t];
+    }
+    if (kk + 32 < K) {
+#pragma unroll
+      for (int t = 0; t < 2; t++) w2[t] = *(const bf16x8*)&wrow[(long)32 * t * K + kk + 32];
+    }
+    bf16x8 x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) x[u] = *(const bf16x8*)&act[32 * u + r][kk + 8 * h];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+        acc[u][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[t], x[u], acc[u][t], 0, 0, 0);
+  }
+  __syncthreads();  // every wave has read this layer's inputs
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int n0 = nb + 32 * t + 8 * g + 4 * h;
+      const float4 bj = *(const float4*)&bias[n0];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const f32x16& c = acc[u][t];
+        uint2 v;
+        v.x = pack_bf16(silu(c[4 * g] + bj.x), silu(c[4 * g + 1] + bj.y));
+        v.y = pack_bf16(silu(c[4 * g + 2] + bj.z), silu(c[4 * g + 3] + bj.w));
+        *(uint2*)&act[32 * u + r][n0] = v;
+      }
+    }
+  __syncthreads();
+}
 }  // namespace
 
 // two waves per SIMD (at most 256 registers per lane): two blocks per CU
-// hide each other's weight loads
+// hide each other's weight loads.  (Measured and dropped: 64-sample blocks
+// at four per CU, 6.9 ms against 4.8 on the 8.3 M-sample call, the weight
+// reads per sample doubled; a ping-pong schedule over two 64-sample halves
+// with one half's SiLU epilogue between the other's matrix operations,
+// 6.9 ms: nine barriers per block and no weight prefetch.)
+template <bool W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_mlp_infer(InferArgs a) {
   __shared__ __align__(16) uint16_t A[INF_M][LDH];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -436,11 +505,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     *(bf16x8*)&A[r][c] = v;
   }
   __syncthreads();
-  const int wm = (w & 1) * 64, nb = (w >> 1) * 128;
-  hidden_layer<MLP_INP>(A, a.w1t, a.params + MLP_B1, wm, nb, lane);
-  hidden_layer<MLP_H>(A, a.w2t, a.params + MLP_B2, wm, nb, lane);
-  hidden_layer<MLP_H>(A, a.w3t, a.params + MLP_B3, wm, nb, lane);
-  hidden_layer<MLP_H>(A, a.w4t, a.params + MLP_B4, wm, nb, lane);
+  if (W) {
+    const int nb = w * 64;
+    hidden_layer_w<MLP_INP>(A, a.w1t, a.params + MLP_B1, nb, lane);
+    hidden_layer_w<MLP_H>(A, a.w2t, a.params + MLP_B2, nb, lane);
+    hidden_layer_w<MLP_H>(A, a.w3t, a.params + MLP_B3, nb, lane);
+    hidden_layer_w<MLP_H>(A, a.w4t, a.params + MLP_B4, nb, lane);
+  } else {
+    const int wm = (w & 1) * 64, nb = (w >> 1) * 128;
+    hidden_layer<MLP_INP>(A, a.w1t, a.params + MLP_B1, wm, nb, lane);
+    hidden_layer<MLP_H>(A, a.w2t, a.params + MLP_B2, wm, nb, lane);
+    hidden_layer<MLP_H>(A, a.w3t, a.params + MLP_B3, wm, nb, lane);
+    hidden_layer<MLP_H>(A, a.w4t, a.params + MLP_B4, wm, nb, lane);
+  }
   // output: 2 threads per sample, 128 columns each
   const int smp = tid >> 1, part = tid & 1;
   float s = 0.f;
@@ -510,5 +587,9 @@ void launch_fill_f32(float* p, long n, float v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_f32, dim3(nb256(n)), dim3(256), 0, s, p, n, v);
 }
 void launch_mlp_infer(const InferArgs& a, hipStream_t s) {
-  if (a.n > 0) hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((a.n + INF_M - 1) / INF_M)), dim3(256), 0, s, a);
+  if (a.n <= 0) return;
+  if (a.variant == 1)
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3((unsigned)((a.n + INF_M - 1) / INF_M)), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_mlp_infer<false>, dim3((unsigned)((a.n + INF_M - 1) / INF_M)), dim3(256), 0, s, a);
 }

@@ -221,6 +221,7 @@ void mlp_infer_dev(const uint16_t* x, long n, float* y, hipStream_t s) {
   a.params = m.params.as<float>();
   a.wbf = m.wbf.as<uint16_t>();
   a.y = y;
+  a.variant = engine_option_int("mlp_infer", 1);
   launch_mlp_infer(a, s);
 }
 bool mlp_trained() { return M().init && M().norm.trained; }

@@ -38,19 +38,27 @@ def get_params(glib):
     return p
 
 
-def test_forward_matches_oracle(glib):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_forward_matches_oracle(glib, variant):
+    """Both fused inference kernels (option mlp_infer: 0 a wave per 64
+    samples x 128 neurons, 1 per 128 samples x 64 neurons, the default)."""
     glib.jur_mlp_init(11)  # identity normalisation
     p = M.init_params(21)
     set_params(glib, p)
     rng = np.random.default_rng(0)
-    for n in [1, 63, 64, 1000]:
-        x = rng.normal(size=(n, M.IN)).astype(np.float32)
-        y = np.zeros(n, dtype=np.float32)
-        glib.jur_mlp_forward(fp(x), n, y.ctypes.data_as(F))
-        ref = M.forward(p, x.astype(np.float64), True)[0]
-        err = np.abs(y - ref)
-        print("n=%d forward max err %.3e (scale %.3e)" % (n, err.max(), np.abs(ref).max()), flush=True)
-        assert err.max() <= 1e-2 * (np.abs(ref).max() + 1e-3)
+    glib.jur_set_option(b"mlp_infer", float(variant))
+    try:
+        for n in [1, 63, 64, 65, 127, 128, 129, 1000]:
+            x = rng.normal(size=(n, M.IN)).astype(np.float32)
+            y = np.zeros(n, dtype=np.float32)
+            glib.jur_mlp_forward(fp(x), n, y.ctypes.data_as(F))
+            ref = M.forward(p, x.astype(np.float64), True)[0]
+            err = np.abs(y - ref)
+            print("variant %d n=%d forward max err %.3e (scale %.3e)" % (variant, n, err.max(), np.abs(ref).max()),
+                  flush=True)
+            assert err.max() <= 1e-2 * (np.abs(ref).max() + 1e-3)
+    finally:
+        glib.jur_set_option(b"mlp_infer", 1.0)
 
 
 def test_gradients_match_oracle(glib):
