@@ -42,7 +42,6 @@ struct Engine {
   // work buffers of the primary pass
   DBuf geo, geo2, rec, npts, tsurf, status, ids, order, keys, sorttmp;
   DBuf pack;  // packed point records and (p, T) brackets of the path integration
-  DBuf pack64;  // packed FP64 point records of the FP64 path integration
   int cap = 512;
   // FP64 point records and surface temperatures of the last trace (option
   // "fp64"); the next path integration reads them
@@ -974,9 +973,6 @@ void engine_integrate(const ctl_t* ctl, int nray, int cap, const float* rec, con
   // that reads them (R + 4 ng floats per LOS point slot of every ray group)
   if (!P.fp64 && P.T.shared && ctl->ng > 0)
     P.pack = (float*)e.pack.need((size_t)(nray + 63) / 64 * 64 * cap * (P.R + 4 * ctl->ng) * sizeof(float));
-  // the same for the FP64 records of the FP64 kernel (option "pack64", default 1)
-  if (P.fp64 && P.rec64 && engine_option_int("pack64", 1))
-    P.pack64 = (double*)e.pack64.need((size_t)(nray + 63) / 64 * 64 * cap * P.R * sizeof(double));
   int variant = engine_option_int("variant", -1), wpb = engine_option_int("wpb", -1);
   if (P.fp64) {
     // the FP64 kernels have one configuration (variant and waves per block

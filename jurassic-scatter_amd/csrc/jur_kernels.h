@@ -92,9 +92,6 @@ struct PathArgs {
   // launch itself for variant 10 (optional: without it variant 10 runs as
   // variant 9)
   float* pack;
-  // FP64 records of every ray group lane-contiguous, [ngroups][cap][R][64],
-  // written by the FP64 path launch itself for k_path64 (optional)
-  double* pack64;
 };
 int path_variants();
 // Launch the path integration with the given waves per block.

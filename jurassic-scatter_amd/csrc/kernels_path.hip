@@ -1424,13 +1424,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       hi = max(hi, __shfl_xor(hi, o, 64));
     }
   }
-  // point records: the FP64 copy when the trace wrote one (R64), from the
-  // ray group's lane-contiguous packed copy when the launch made one
-  // (A.pack64: every field load of the wave one contiguous 512-byte access)
+  // point records: the FP64 copy when the trace wrote one (R64)
   const size_t rbase = (size_t)ray * A.cap * A.R;
-  const double* pk = R64 && A.pack64 ? A.pack64 + (size_t)group * A.cap * A.R * 64 + lane : nullptr;
   auto rec = [&](int ip, int f) -> double {
-    if (R64 && pk) return pk[((size_t)ip * A.R + f) * 64];
     const size_t i = rbase + (size_t)ip * A.R + f;
     return R64 ? A.rec64[i] : (double)A.rec[i];
   };
@@ -1538,25 +1534,6 @@ __global__ void __launch_bounds__(256) k_path64e(PathArgs A) {
   if (sp) save(npts);
   if (!valid || sub != 0) return;
   finish64(A, prow, id, ray, rad, tau);
-}
-
-// FP64 point records of a ray group (64 consecutive rays of the sorted list)
-// copied lane-contiguous, [group][slot][field][64], for k_path64: one thread
-// per (group, point slot, lane), a wave writes one slot's 64 lanes.
-__global__ void __launch_bounds__(256) k_pack64(PathArgs A) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ngroups = (A.nray + 63) >> 6;
-  if (t >= (long)ngroups * A.cap * 64) return;
-  const int lane = (int)(t & 63);
-  const long q = t >> 6;
-  const int group = (int)(q / A.cap), ip = (int)(q - (long)group * A.cap);
-  const int slot = group * 64 + lane;
-  if (slot >= A.nray) return;
-  const int ray = A.order[slot];
-  if (ip >= A.np[ray]) return;
-  const double* pr = A.rec64 + ((size_t)ray * A.cap + ip) * A.R;
-  double* out = A.pack64 + ((size_t)group * A.cap + ip) * (size_t)A.R * 64 + lane;
-  for (int f = 0; f < A.R; f++) out[f * 64] = pr[f];
 }
 
 // Reuse of base rays (ray i >= pnr repeats ray i % pnr up to where its FP64
@@ -1669,10 +1646,6 @@ void launch_64(const PathArgs& A, int wpb, hipStream_t s) {
   long nb = (long)A.nd * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;
   const bool r64 = A.rec64 != nullptr;
-  if (r64 && A.pack64) {
-    const long nt = (long)ngroups * A.cap * 64;
-    hipLaunchKernelGGL(k_pack64, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, A);
-  }
   const bool hint = A.T.shared && NGT <= 8;
   if (A.snap_mode == 2) {
     // reuse of base rays (FP64 records only)
