@@ -1014,20 +1014,25 @@ size_t stage_lds_bytes(const PathArgs& A, int ngt, int cpw, int wpb, bool pack =
 // weights of the LIN calls at :477-491), computed once per (point, emitter)
 // for all channels.  The searches are the staged kernel's own (grid_bracket:
 // on a strictly increasing grid the unique bracket, whatever the hint) and
-// the quotients the same, so radiances are bit-identical.  One thread per
-// (group, point slot, lane): a wave writes one slot's 64 lanes.
+// the quotients the same, so radiances are bit-identical.  Block (group,
+// y): its 4 waves write point slots 32 y .. 32 y + 31 of the group, a wave
+// one slot's 64 lanes per step; blocks past the group's longest ray leave
+// at once.
 static_assert(TBLNPMAX <= 256 && TBLNTMAX <= 256, "bracket indices are packed in 8 bits");
+constexpr int PACK_SLOTS = 32;  // point slots per block of k_pack
+__device__ inline void pack_point(const PathArgs& A, int group, int lane, int ray, int ip);
 __global__ void __launch_bounds__(256) k_pack(PathArgs A) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ngroups = (A.nray + 63) >> 6;
-  if (t >= (long)ngroups * A.cap * 64) return;
-  const int lane = (int)(t & 63);
-  const long q = t >> 6;
-  const int group = (int)(q / A.cap), ip = (int)(q - (long)group * A.cap);
+  const int group = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int slot = group * 64 + lane;
-  if (slot >= A.nray) return;
-  const int ray = A.order[slot];
-  if (ip >= A.np[ray]) return;
+  const int ray = slot < A.nray ? A.order[slot] : -1;
+  const int np = ray >= 0 ? A.np[ray] : 0;
+  int mx = np;
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+  const int ip1 = min(mx, (int)(blockIdx.y + 1) * PACK_SLOTS);
+  for (int ip = blockIdx.y * PACK_SLOTS + wv; ip < ip1; ip += 4)
+    if (ip < np) pack_point(A, group, lane, ray, ip);
+}
+__device__ inline void pack_point(const PathArgs& A, int group, int lane, int ray, int ip) {
   const DevTables& T = A.T;
   const float* pr = A.rec + ((size_t)ray * A.cap + ip) * A.R;
   float* out = A.pack + ((size_t)group * A.cap + ip) * (size_t)(A.R + 4 * A.ng) * 64 + lane;
@@ -1680,8 +1685,8 @@ void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   if (PACK && A.T.shared && A.pack) {
     const size_t lds = stage_lds_bytes(A, NGT, CPW, wpb, true);
     if (lds > 0) {
-      const long nt = (long)ngroups * A.cap * 64;
-      hipLaunchKernelGGL(k_pack, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, A);
+      hipLaunchKernelGGL(k_pack, dim3((unsigned)ngroups, (unsigned)((A.cap + PACK_SLOTS - 1) / PACK_SLOTS)),
+                         dim3(256), 0, s, A);
       hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC, TILE, true>), dim3((unsigned)nb), dim3(64 * wpb),
                          (unsigned)lds, s, A);
       return;

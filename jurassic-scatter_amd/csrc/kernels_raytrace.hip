@@ -139,17 +139,12 @@ __global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
   if (status) atomicOr(A.status, status);
 }
 
-// One thread per (ray, final point): with aerosol layers the segment length
-// from Cartesian half-chords (lineofsight.c:310-324) and the layer
+// Point ip of ray r (nf final points): with aerosol layers the segment
+// length from Cartesian half-chords (lineofsight.c:310-324) and the layer
 // membership / transition factor (:334-363); then the atmosphere at the
 // point's altitude and the FP32 record with the column densities
 // u = 10 q p / (k_B T) ds (:182-185, :330-332).
-__global__ void k_records(TraceArgs A) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)A.nray * A.cap) return;
-  const int r = (int)(t / A.cap), ip = (int)(t % A.cap);
-  const int nf = A.np_out[r];
-  if (ip >= nf) return;
+__device__ inline void record_point(const TraceArgs& A, int r, int ip, int nf) {
   const int a = A.atm_idx ? A.atm_idx[r] : 0;
   const DevAtm& M = A.atm;
   // k_raytrace inserted layer-edge points (into geo2) for every ray with
@@ -231,10 +226,19 @@ __global__ void k_records(TraceArgs A) {
   }
 }
 
+// One wave per ray, its lanes over the ray's final points (a thread per
+// point up to the LOS capacity left most threads without a point: the
+// capacity is sized for the longest ray of the batch).
+__global__ void __launch_bounds__(256) k_records(TraceArgs A) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= A.nray) return;
+  const int nf = A.np_out[r];
+  for (int ip = threadIdx.x & 63; ip < nf; ip += 64) record_point(A, r, ip, nf);
+}
+
 void launch_raytrace(const TraceArgs& A, hipStream_t s) {
   if (A.nray <= 0) return;
   int bs = 128;
   hipLaunchKernelGGL(k_raytrace, dim3((A.nray + bs - 1) / bs), dim3(bs), 0, s, A);
-  const long npt = (long)A.nray * A.cap;
-  hipLaunchKernelGGL(k_records, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, s, A);
+  hipLaunchKernelGGL(k_records, dim3((unsigned)((A.nray + 3) / 4)), dim3(256), 0, s, A);
 }

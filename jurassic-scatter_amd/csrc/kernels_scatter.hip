@@ -237,6 +237,7 @@ template <int NA>
 __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
   __shared__ double Aw[180][NA];
   __shared__ double alpha[NA];
+  __shared__ double sphi[180], cphi[180];  // sin and cos of the 180 azimuths
   const int k = blockIdx.x;
   if (G.d_npt && k >= *G.d_npt) return;  // beyond the device-side count of the points
   const ScaPt P = G.pts[k];
@@ -262,20 +263,29 @@ __global__ void __launch_bounds__(256) k_angular(AngArgs G) {
   }
   for (int i = threadIdx.x; i < 180 * NA; i += blockDim.x) (&Aw[0][0])[i] = 0.0;
   if (threadIdx.x < NA) alpha[threadIdx.x] = G.alpha[threadIdx.x];
+  for (int i = threadIdx.x; i < 180; i += blockDim.x) {
+    const double phi = 2. * (0.5 + i) * M_PI / 180.;
+    sphi[i] = sin(phi);
+    cphi[i] = cos(phi);
+  }
   __syncthreads();
   double lx[3], ly[3], lz[3], sx[3], sy[3], sz[3];
   local_frame(P.x, lx, ly, lz);
   bascoord(P.dx, P.x, sx, sy, sz);
   const double mlz[3] = {-lz[0], -lz[1], -lz[2]};
+  const double nmlz = norm3(mlz);
   for (int it = threadIdx.x; it < 180; it += blockDim.x) {
     const double th = (0.5 + it) * M_PI / 180.;
     const double st = sin(th), ct = cos(th);
     for (int iphi = 0; iphi < 180; iphi++) {
-      const double phi = 2. * (0.5 + iphi) * M_PI / 180.;
       double ek[3];
-      const double sp = sin(phi), cp = cos(phi);
+      const double sp = sphi[iphi], cp = cphi[iphi];
       for (int i = 0; i < 3; i++) ek[i] = st * sp * sx[i] + st * cp * sy[i] + ct * sz[i];
-      const double a2 = angle3(mlz, ek);
+      // angle3(mlz, ek) with the fan's constant norm hoisted
+      double c = dot3(mlz, ek) / nmlz / norm3(ek);
+      c = (c > -1) ? c : -1;
+      c = (c < 1) ? c : 1;
+      const double a2 = acos(c);
       const int idx = locate_mono<double>(alpha, NA, a2);
       const double f = (a2 - alpha[idx]) / (alpha[idx + 1] - alpha[idx]);
       Aw[it][idx] += 1.0 - f;
