@@ -212,6 +212,24 @@ __device__ inline int atm_locate_h(const DevAtm& A, int a, double z0, int h) {
   }
   return locate_mono<double>(z, n, z0);
 }
+// locate on an increasing grid from a guess by the grid's mean spacing:
+// the guessed bracket is checked and one step tried either way before the
+// bisection, so near-uniform grids need one or two loads instead of
+// log2(n); the bracket z[h] <= z0 < z[h + 1] is unique, so this returns
+// locate_mono's index.
+__device__ inline int locate_guess(const double* z, int n, double z0) {
+  if (n < 3 || !(z[0] < z[1]) || !(z[0] < z[n - 1]) || !(z0 == z0)) return locate_mono<double>(z, n, z0);
+  const double g = (z0 - z[0]) / (z[n - 1] - z[0]) * (n - 1);
+  int h = g < 0 ? 0 : (g >= n - 2 ? n - 2 : (int)g);
+  const bool okl = (h == 0) || (z[h] <= z0), okh = (h == n - 2) || (z0 < z[h + 1]);
+  if (okl && okh) return h;
+  if (okl) {
+    if (h + 1 == n - 2 || z0 < z[h + 2]) return h + 1;
+  } else {
+    if (h - 1 == 0 || z[h - 1] <= z0) return h - 1;
+  }
+  return locate_mono<double>(z, n, z0);
+}
 __device__ inline int atm_locate(const DevAtm& A, int a, double z0) {
   return locate_mono<double>(A.z + (size_t)a * A.nlev, A.nlev, z0);
 }
@@ -262,7 +280,7 @@ __device__ inline GeoLoc geo_locate(const DevAtm& A, int a, double z0, double lo
   if (A.geo.ip >= 2) geo2cart(0, lon0, lat0, L.x0);
   const double* z = A.z + (size_t)a * A.nlev;
   if (A.geo.ip <= 1) {
-    L.j0 = L.j1 = locate_mono<double>(z, A.nlev, z0);
+    L.j0 = L.j1 = locate_guess(z, A.nlev, z0);
   } else if (A.geo.ip == 2) {
     double d0 = 1e99, d1 = 1e99;
     int i0 = 0, i1 = 0;
@@ -279,8 +297,8 @@ __device__ inline GeoLoc geo_locate(const DevAtm& A, int a, double z0, double lo
       }
     }
     const int b0 = A.geo.pidx[i0], b1 = A.geo.pidx[i1];
-    L.j0 = b0 + locate_mono<double>(z + b0, A.geo.pnz[i0], z0);
-    L.j1 = b1 + locate_mono<double>(z + b1, A.geo.pnz[i1], z0);
+    L.j0 = b0 + locate_guess(z + b0, A.geo.pnz[i0], z0);
+    L.j1 = b1 + locate_guess(z + b1, A.geo.pnz[i1], z0);
     const double x2 = dist2_3(A.geo.px + 3 * i0, A.geo.px + 3 * i1), x = sqrt(x2);
     const double r0 = (d0 - d1 + x2) / (2 * x), r1 = x - r0;
     L.r = r0 <= 0 ? 0.0 : (r1 <= 0 ? 1.0 : r0 / (r0 + r1));
