@@ -207,6 +207,9 @@ int jur_init(int device);
    work buffers keep their addresses; a replay whose rays overflow the
    recorded LOS capacity is detected and the call re-run), "surrogate",
    "hull_margin", "collect" (learned scattering source, below),
+   "mlp_infer" (the network's fused inference kernel: 1, the default, a
+   wave per 128 samples x 64 neurons of a layer; 0 a wave per 64 samples x
+   128 neurons),
    "fan" (secondary pencils per scattering point, even, 10-32; default the
    reference's 28; 10-28 use the reference's zenith-angle construction, 30
    and 32 keep its horizon band and first steps and continue in equal steps
