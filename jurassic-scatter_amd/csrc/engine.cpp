@@ -42,6 +42,7 @@ struct Engine {
   // work buffers of the primary pass
   DBuf geo, geo2, rec, npts, tsurf, status, ids, order, keys, sorttmp;
   DBuf pack;  // packed point records and (p, T) brackets of the path integration
+  DBuf lps;   // log-slopes of the traced atmosphere set's pressure
   int cap = 512;
   // FP64 point records and surface temperatures of the last trace (option
   // "fp64"); the next path integration reads them
@@ -605,6 +606,11 @@ void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays
     A.status = (int*)e.status.need(sizeof(int) * 4);
     launch_zero_int(A.status, 1, s);
   }
+  // the pressure interpolation's log-slopes of this atmosphere set
+  const long nlv = (long)(atm.natm > 0 ? atm.natm : 1) * atm.nlev;
+  double* lps = (double*)e.lps.need((size_t)nlv * sizeof(double));
+  launch_log_slopes(atm.z, atm.p, nlv, lps, s);
+  A.atm.lps = lps;
   launch_raytrace(A, s);
 }
 }  // namespace
@@ -699,14 +705,14 @@ std::string graph_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& r
   Engine& e = E();
   char k[2048];
   int n = snprintf(k, sizeof k, "%s|%s|%d %d %d %d|%p %p %p %p %p|%p %p %p %p %p %p %p %p|%p %p %p %p %p %p|"
-                   "%p %p %p %p %p %p %p %p %p %p|%p|%d %.17g %.17g %.17g %d %d|%s|%d %d %d|",
+                   "%p %p %p %p %p %p %p %p %p %p %p|%p|%d %.17g %.17g %.17g %d %d|%s|%d %d %d|",
                    e.tbl_key.c_str(), e.chan_key.c_str(), rays.nray, atm.natm, atm.nlev, scattering,
                    (const void*)atm.z, (const void*)atm.p, (const void*)atm.t, (const void*)atm.q,
                    (const void*)atm.k, (const void*)rays.time, (const void*)rays.obsz, (const void*)rays.obslon,
                    (const void*)rays.obslat, (const void*)rays.vpz, (const void*)rays.vplon,
                    (const void*)rays.vplat, (const void*)rays.atm_idx, (void*)out.rad, (void*)out.tau,
                    (void*)out.tpz, (void*)out.tplon, (void*)out.tplat, (void*)out.npts, e.geo.p, e.geo2.p, e.rec.p,
-                   e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, e.pack.p, (void*)s, ctl->sca_n,
+                   e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, e.pack.p, e.lps.p, (void*)s, ctl->sca_n,
                    ctl->rayds, ctl->raydz, ctl->transs, ctl->refrac, ctl->nw + 1000 * out.ld, ctl->sca_ext,
                    engine_option_int("fast", 1), engine_option_int("variant", -1), engine_option_int("wpb", -1));
   std::string key(k, n < (int)sizeof k ? n : (int)sizeof k - 1);

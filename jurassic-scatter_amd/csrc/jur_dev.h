@@ -187,6 +187,9 @@ struct DevAtm {
   const double* k;
   int nlev, ng, nw;
   DevGeo geo;
+  // optional: log(p[j + 1] / p[j]) / (z[j + 1] - z[j]) of every interval
+  // (k_log_slopes; the same operations as the interpolation's), [a][lev]
+  const double* lps = nullptr;
 };
 
 // intpol_atm_1d (atmosphere.c:298-320): p exponential (EXP, jurassic.h:55-58),
@@ -240,7 +243,7 @@ __device__ inline AtmPoint atm_pt(const DevAtm& A, int a, double z0, int ip) {
   AtmPoint r;
   double y0 = p[ip], y1 = p[ip + 1];
   if (y0 > 0 && y1 > 0)
-    r.p = y0 * exp(log(y1 / y0) / (z[ip + 1] - z[ip]) * (z0 - z[ip]));
+    r.p = y0 * exp((A.lps ? A.lps[(size_t)a * A.nlev + ip] : log(y1 / y0) / (z[ip + 1] - z[ip])) * (z0 - z[ip]));
   else
     r.p = lin(z[ip], y0, z[ip + 1], y1, z0);
   r.t = lin(z[ip], t[ip], z[ip + 1], t[ip + 1], z0);
@@ -308,18 +311,20 @@ __device__ inline GeoLoc geo_locate(const DevAtm& A, int a, double z0, double lo
 // A field of copy a (f points at its first value) at the located point:
 // exponential in altitude for the pressure (EXP, jurassic.h:55-58), linear
 // otherwise.
+// lp: the copy's precomputed log-slopes (pressure only; may be null).
 template <bool EXPO>
-__device__ inline double geo_1d(const double* z, const double* f, int j, double z0) {
+__device__ inline double geo_1d(const double* z, const double* f, int j, double z0, const double* lp) {
   const double y0 = f[j], y1 = f[j + 1];
-  if (EXPO && y0 > 0 && y1 > 0) return y0 * exp(log(y1 / y0) / (z[j + 1] - z[j]) * (z0 - z[j]));
+  if (EXPO && y0 > 0 && y1 > 0) return y0 * exp((lp ? lp[j] : log(y1 / y0) / (z[j + 1] - z[j])) * (z0 - z[j]));
   return lin(z[j], y0, z[j + 1], y1, z0);
 }
 template <bool EXPO>
 __device__ inline double geo_field(const DevAtm& A, int a, const GeoLoc& L, const double* f) {
   const double* z = A.z + (size_t)a * A.nlev;
-  if (A.geo.ip <= 1) return geo_1d<EXPO>(z, f, L.j0, L.z0);
+  const double* lp = EXPO && A.lps ? A.lps + (size_t)a * A.nlev : nullptr;
+  if (A.geo.ip <= 1) return geo_1d<EXPO>(z, f, L.j0, L.z0, lp);
   if (A.geo.ip == 2)
-    return (1 - L.r) * geo_1d<EXPO>(z, f, L.j0, L.z0) + L.r * geo_1d<EXPO>(z, f, L.j1, L.z0);
+    return (1 - L.r) * geo_1d<EXPO>(z, f, L.j0, L.z0, lp) + L.r * geo_1d<EXPO>(z, f, L.j1, L.z0, lp);
   // IP 3: weights (1 - dz/CZ)(1 - dh/CX) of the points within range, in
   // the reference's point order
   double v = 0, wsum = 0;

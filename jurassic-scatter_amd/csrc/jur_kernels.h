@@ -44,6 +44,8 @@ struct TraceArgs {
   int nvalid_off;
 };
 void launch_raytrace(const TraceArgs& A, hipStream_t s);
+// Log-slopes of the pressure of n consecutive levels (k_log_slopes).
+void launch_log_slopes(const double* z, const double* p, long n, double* lps, hipStream_t s);
 
 struct PathArgs {
   int nray, nd, ng, nw, R, cap;

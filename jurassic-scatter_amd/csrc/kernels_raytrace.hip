@@ -236,6 +236,20 @@ __global__ void __launch_bounds__(256) k_records(TraceArgs A) {
   for (int ip = threadIdx.x & 63; ip < nf; ip += 64) record_point(A, r, ip, nf);
 }
 
+// log(p[j + 1] / p[j]) / (z[j + 1] - z[j]) of every level interval of every
+// copy (0 where the pressure interpolation is linear), the factor of the
+// exponential interpolation (atmosphere.c:298-320 with EXP, jurassic.h:55-58)
+// computed once per interval instead of once per evaluation.
+__global__ void k_log_slopes(const double* z, const double* p, long n, double* lps) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= n) return;
+  const double y0 = p[i], y1 = p[i + 1];
+  lps[i] = (y0 > 0 && y1 > 0) ? log(y1 / y0) / (z[i + 1] - z[i]) : 0.0;
+}
+void launch_log_slopes(const double* z, const double* p, long n, double* lps, hipStream_t s) {
+  if (n > 1) hipLaunchKernelGGL(k_log_slopes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z, p, n, lps);
+}
+
 void launch_raytrace(const TraceArgs& A, hipStream_t s) {
   if (A.nray <= 0) return;
   int bs = 128;
