@@ -207,6 +207,9 @@ int jur_init(int device);
    work buffers keep their addresses; a replay whose rays overflow the
    recorded LOS capacity is detected and the call re-run), "surrogate",
    "hull_margin", "collect" (learned scattering source, below),
+   "trace_sort" (1, the default: the rays of a trace of at least 262,144
+   rays are marched in the order of their straight-line tangent altitude,
+   so that a wave's rays march about as long; results are identical),
    "mlp_infer" (the network's fused inference kernel: 1, the default, a
    wave per 128 samples x 64 neurons of a layer; 0 a wave per 64 samples x
    128 neurons),

@@ -43,6 +43,7 @@ struct Engine {
   DBuf geo, geo2, rec, npts, tsurf, status, ids, order, keys, sorttmp;
   DBuf pack;  // packed point records and (p, T) brackets of the path integration
   DBuf lps;   // log-slopes of the traced atmosphere set's pressure
+  DBuf tkey, torder, thist;  // march order of the traced rays
   int cap = 512;
   // FP64 point records and surface temperatures of the last trace (option
   // "fp64"); the next path integration reads them
@@ -611,6 +612,17 @@ void enqueue_trace(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& rays
   double* lps = (double*)e.lps.need((size_t)nlv * sizeof(double));
   launch_log_slopes(atm.z, atm.p, nlv, lps, s);
   A.atm.lps = lps;
+  // march order: rays of similar straight-line length in one wave (option
+  // "trace_sort", default 1), for calls of at least four waves per SIMD
+  // (262,144 rays): with fewer every wave marches at once and the call
+  // lasts as long as its longest ray whatever the order
+  if (engine_option_int("trace_sort", 1) && rays.nray >= 262144) {
+    int* key = (int*)e.tkey.need((size_t)rays.nray * sizeof(int));
+    int* ord = (int*)e.torder.need((size_t)rays.nray * sizeof(int));
+    launch_trace_key(A, key, s);
+    launch_count_sort(key, rays.nray, TRACE_KEY_BINS, (int*)e.thist.need(TRACE_KEY_BINS * sizeof(int)), ord, s);
+    A.order = ord;
+  }
   launch_raytrace(A, s);
 }
 }  // namespace
@@ -705,14 +717,14 @@ std::string graph_key(const ctl_t* ctl, const AtmSetDev& atm, const RaySetDev& r
   Engine& e = E();
   char k[2048];
   int n = snprintf(k, sizeof k, "%s|%s|%d %d %d %d|%p %p %p %p %p|%p %p %p %p %p %p %p %p|%p %p %p %p %p %p|"
-                   "%p %p %p %p %p %p %p %p %p %p %p|%p|%d %.17g %.17g %.17g %d %d|%s|%d %d %d|",
+                   "%p %p %p %p %p %p %p %p %p %p %p %p|%p|%d %.17g %.17g %.17g %d %d|%s|%d %d %d|",
                    e.tbl_key.c_str(), e.chan_key.c_str(), rays.nray, atm.natm, atm.nlev, scattering,
                    (const void*)atm.z, (const void*)atm.p, (const void*)atm.t, (const void*)atm.q,
                    (const void*)atm.k, (const void*)rays.time, (const void*)rays.obsz, (const void*)rays.obslon,
                    (const void*)rays.obslat, (const void*)rays.vpz, (const void*)rays.vplon,
                    (const void*)rays.vplat, (const void*)rays.atm_idx, (void*)out.rad, (void*)out.tau,
                    (void*)out.tpz, (void*)out.tplon, (void*)out.tplat, (void*)out.npts, e.geo.p, e.geo2.p, e.rec.p,
-                   e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, e.pack.p, e.lps.p, (void*)s, ctl->sca_n,
+                   e.npts.p, e.tsurf.p, e.order.p, e.hist.p, e.status.p, e.pts_acc.p, e.pack.p, e.lps.p, e.torder.p, (void*)s, ctl->sca_n,
                    ctl->rayds, ctl->raydz, ctl->transs, ctl->refrac, ctl->nw + 1000 * out.ld, ctl->sca_ext,
                    engine_option_int("fast", 1), engine_option_int("variant", -1), engine_option_int("wpb", -1));
   std::string key(k, n < (int)sizeof k ? n : (int)sizeof k - 1);

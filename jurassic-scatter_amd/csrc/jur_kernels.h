@@ -42,8 +42,16 @@ struct TraceArgs {
   // capacity: ray r is empty when r + nvalid_off >= *nvalid
   const int* nvalid;
   int nvalid_off;
+  // optional march order: thread i of k_raytrace traces ray order[i]
+  // (rays of similar straight-line length share a wave)
+  const int* order;
 };
 void launch_raytrace(const TraceArgs& A, hipStream_t s);
+// Straight-line length class of every ray for the march order
+// (k_trace_key): 0 for lines that reach the ground or leave upwards, else
+// 1000 - the tangent altitude in km (longer marches larger); keys < 1001.
+constexpr int TRACE_KEY_BINS = 1001;
+void launch_trace_key(const TraceArgs& A, int* key, hipStream_t s);
 // Log-slopes of the pressure of n consecutive levels (k_log_slopes).
 void launch_log_slopes(const double* z, const double* p, long n, double* lps, hipStream_t s);
 

@@ -40,8 +40,9 @@ struct InsertSink {
 }  // namespace
 
 __global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= A.nray) return;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= A.nray) return;
+  const int r = A.order ? A.order[tid] : tid;
   if (A.nvalid && r + A.nvalid_off >= *A.nvalid) {
     // beyond the device-side count of a batch sized by its capacity: an
     // empty ray (no LOS points, no surface)
@@ -234,6 +235,33 @@ __global__ void __launch_bounds__(256) k_records(TraceArgs A) {
   if (r >= A.nray) return;
   const int nf = A.np_out[r];
   for (int ip = threadIdx.x & 63; ip < nf; ip += 64) record_point(A, r, ip, nf);
+}
+
+// March-order key of each ray from its straight line observer -> view
+// point: the march of a wave lasts as long as its longest ray, and rays in
+// input order mix limb paths of several hundred steps with nadir paths of
+// about a hundred.
+__global__ void k_trace_key(TraceArgs A, int* key) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.nray) return;
+  double xo[3], xv[3], e[3], c[3];
+  geo2cart(A.obsz[r], A.obslon[r], A.obslat[r], xo);
+  geo2cart(A.vpz[r], A.vplon[r], A.vplat[r], xv);
+  for (int i = 0; i < 3; i++) e[i] = xv[i] - xo[i];
+  const double n = norm3(e);
+  for (int i = 0; i < 3; i++) e[i] /= n;
+  const double sd = -dot3(xo, e);  // distance to the closest approach
+  double ht = A.obsz[r];
+  if (sd > 0) {
+    for (int i = 0; i < 3; i++) c[i] = xo[i] + sd * e[i];
+    ht = norm3(c) - RE;
+  }
+  int k = 0;
+  if (sd > 0 && ht >= 0 && ht == ht) k = 1000 - (int)(ht < 999.0 ? ht : 999.0);
+  key[r] = k;
+}
+void launch_trace_key(const TraceArgs& A, int* key, hipStream_t s) {
+  if (A.nray > 0) hipLaunchKernelGGL(k_trace_key, dim3((A.nray + 255) / 256), dim3(256), 0, s, A, key);
 }
 
 // log(p[j + 1] / p[j]) / (z[j + 1] - z[j]) of every level interval of every
