@@ -78,6 +78,17 @@ MEASURED_TRAFFIC = {
     # variant 10: k_pack and k_path_stage together (the launch the HIP events time)
     ("clear_batch_65536", 65536, 10, 1): (4.1996e10, "profiles/k_path_stage_clear_batch_65536.md"),
 }
+# What binds the path kernel, from rocprofv3 counters of the same
+# configuration (the profile named beside each entry holds the passes):
+# VALU issue (SQ_INSTS_VALU x 4 cycles / (SIMDs x kernel cycles)), the
+# vector-memory data-return path (TD_TD_BUSY_sum / (CUs x cycles)), the
+# address path (TA) and the L2 hit rate.
+MEASURED_BINDING = {
+    ("clear_batch_65536", 65536, 10, 1): ({"valu_issue": 1.03, "td_busy": 0.94, "ta_busy": 0.71, "l2_hit": 0.76},
+                                          "profiles/k_path_stage_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 9, 1): ({"valu_issue": 0.88, "td_busy": 0.92, "ta_busy": 0.66, "l2_hit": 0.96},
+                                         "profiles/k_path_stage_clear_batch_65536.md"),
+}
 # The FP32 path kernel of each autotuner variant (7-9: the staged kernel,
 # grids in LDS, lockstep bracket searches, 9 with the tiled block mapping;
 # 10: the same on the packed stream of k_pack (lane-contiguous records, (p, T)
@@ -854,6 +865,7 @@ def main():
     variant = args.variant if args.variant is not None else as_int(lib.jur_stat(b"tuned_variant"))
     wpb = args.wpb if args.wpb is not None else as_int(lib.jur_stat(b"tuned_wpb"))
     traffic, traffic_src = MEASURED_TRAFFIC.get((args.workload, nray, variant, wpb), (None, None))
+    binding, binding_src = MEASURED_BINDING.get((args.workload, nray, variant, wpb), (None, None))
 
     line = {
         "metric": "ray x channel radiances/s (FP32 EGA + Kahan, FP64 geometry)",
@@ -884,7 +896,8 @@ def main():
                      "kernel_ms_eager": float(np.mean(path_ms)),
                      "trace_ms": float(np.mean(trace_ms)) if not scattering else None,
                      "launches_per_call": int(launches[-1]),
-                     "alg_bytes_per_call": alg_bytes, "variant": variant, "wpb": wpb},
+                     "alg_bytes_per_call": alg_bytes, "variant": variant, "wpb": wpb,
+                     "binding": binding, "binding_source": binding_src},
     }
 
     if not args.no_cpu:
