@@ -3,8 +3,9 @@
 The variants differ in work mapping (channels per wave, waves per block,
 occupancy target) and, for the staged kernel (variants 7-10), in how the
 table brackets are found (grids in LDS, per-corner hints, the four corners
-searched in lockstep; variant 10: the (p, T) brackets and weights computed
-once per LOS point and emitter by k_brackets); the arithmetic is the same, so radiances and
+searched in lockstep; variant 10: k_pack copies each 64-ray group's point
+records lane-contiguously and computes the (p, T) brackets and weights once
+per LOS point and emitter); the arithmetic is the same, so radiances and
 transmittances must be identical to the default fast kernel's (variant 6)
 on every ray and channel.  Cases: a
 4,096-ray slice of the 65,536-ray batch (synthetic tables, 5 emitters, 64
