@@ -1682,8 +1682,10 @@ void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
   int ngroups = (A.nray + 63) / 64;
   long nb = (long)nchunk * ((ngroups + wpb - 1) / wpb);
   nb = (nb + 7) / 8 * 8;  // a multiple of the XCD count for the logical remap
-  if (PACK && A.T.shared && A.pack) {
-    const size_t lds = stage_lds_bytes(A, NGT, CPW, wpb, true);
+  // compile-time branches: only the staged variants instantiate the staged
+  // kernel, and only the packed one its packed form
+  if constexpr (PACK) {
+    const size_t lds = A.T.shared && A.pack ? stage_lds_bytes(A, NGT, CPW, wpb, true) : 0;
     if (lds > 0) {
       hipLaunchKernelGGL(k_pack, dim3((unsigned)ngroups, (unsigned)((A.cap + PACK_SLOTS - 1) / PACK_SLOTS)),
                          dim3(256), 0, s, A);
@@ -1692,11 +1694,15 @@ void launch_t(const PathArgs& A, int wpb, hipStream_t s) {
       return;
     }
   }
-  const size_t lds = STAGE && A.T.shared ? stage_lds_bytes(A, NGT, CPW, wpb) : 0;
-  if (lds > 0)
-    hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC, TILE>), dim3((unsigned)nb), dim3(64 * wpb), (unsigned)lds, s,
-                       A);
-  else if (A.T.shared)
+  if constexpr (STAGE) {
+    const size_t lds = A.T.shared ? stage_lds_bytes(A, NGT, CPW, wpb) : 0;
+    if (lds > 0) {
+      hipLaunchKernelGGL((k_path_stage<NGT, CPW, OCC, TILE>), dim3((unsigned)nb), dim3(64 * wpb), (unsigned)lds,
+                         s, A);
+      return;
+    }
+  }
+  if (A.T.shared)
     hipLaunchKernelGGL((k_path_fast<NGT, CPW, OCC>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
   else
     hipLaunchKernelGGL((k_path<NGT, CPW>), dim3((unsigned)nb), dim3(64 * wpb), 0, s, A);
