@@ -140,12 +140,20 @@ __global__ void __launch_bounds__(128) k_raytrace(TraceArgs A) {
   if (status) atomicOr(A.status, status);
 }
 
+// Cartesian positions of a point and its two neighbours on the ray, and the
+// previous point's altitude (aerosol layers only).
+struct Nbr {
+  double xo[3], xp[3], xn[3];
+  double zp;
+};
+
 // Point ip of ray r (nf final points): with aerosol layers the segment
-// length from Cartesian half-chords (lineofsight.c:310-324) and the layer
+// length from Cartesian half-chords (lineofsight.c:310-324; the positions
+// come in nb, computed once per point by k_records) and the layer
 // membership / transition factor (:334-363); then the atmosphere at the
 // point's altitude and the FP32 record with the column densities
 // u = 10 q p / (k_B T) ds (:182-185, :330-332).
-__device__ inline void record_point(const TraceArgs& A, int r, int ip, int nf) {
+__device__ inline void record_point(const TraceArgs& A, int r, int ip, int nf, const Nbr& nb) {
   const int a = A.atm_idx ? A.atm_idx[r] : 0;
   const DevAtm& M = A.atm;
   // k_raytrace inserted layer-edge points (into geo2) for every ray with
@@ -158,26 +166,16 @@ __device__ inline void record_point(const TraceArgs& A, int r, int ip, int nf) {
   float af = 0.f, ai = -999.f;
   double af64 = 0.0;
   if (ins) {
-    double xa[3], xb[3], xc[3];
-    // the neighbours' positions only: each thread rewrites its own point's
-    // segment length (.w) below
-    auto pos = [F](int i, double* x) { geo2cart(F[i].x, F[i].y, F[i].z, x); };
     if (nf >= 2) {
-      if (ip == 0 || ip == nf - 1) {
-        geo2cart(g.x, g.y, g.z, xa);
-        pos(ip == 0 ? 1 : nf - 2, xb);
-        ds = 0.5 * dist3(xa, xb);
-      } else {
-        pos(ip - 1, xa);
-        geo2cart(g.x, g.y, g.z, xb);
-        pos(ip + 1, xc);
-        ds = 0.5 * (dist3(xa, xb) + dist3(xb, xc));
-      }
+      if (ip == 0 || ip == nf - 1)
+        ds = 0.5 * dist3(nb.xo, ip == 0 ? nb.xn : nb.xp);
+      else
+        ds = 0.5 * (dist3(nb.xp, nb.xo) + dist3(nb.xo, nb.xn));
       F[ip].w = ds;
     }
     const double zi = g.x;
     const double altimax = A.nalti > 0 ? A.altimax : NAN, altimin = A.nalti > 0 ? A.altimin : NAN;
-    const bool pre = (ip == 0) || ((F[ip - 1].x < altimax || zi < altimax) && (F[ip - 1].x > altimin || zi > altimin));
+    const bool pre = (ip == 0) || ((nb.zp < altimax || zi < altimax) && (nb.zp > altimin || zi > altimin));
     double afd = 0;
     int aii = -999;
     if (pre) {
@@ -229,12 +227,38 @@ __device__ inline void record_point(const TraceArgs& A, int r, int ip, int nf) {
 
 // One wave per ray, its lanes over the ray's final points (a thread per
 // point up to the LOS capacity left most threads without a point: the
-// capacity is sized for the longest ray of the batch).
+// capacity is sized for the longest ray of the batch).  With aerosol layers
+// the wave walks the ray in windows of 62 points with a halo point on each
+// side: every lane converts its own point to Cartesian coordinates once and
+// takes its neighbours' from the adjacent lanes, instead of converting all
+// three points itself.
 __global__ void __launch_bounds__(256) k_records(TraceArgs A) {
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= A.nray) return;
   const int nf = A.np_out[r];
-  for (int ip = threadIdx.x & 63; ip < nf; ip += 64) record_point(A, r, ip, nf);
+  const int lane = threadIdx.x & 63;
+  if (!(A.sca_n > 0 && A.geo2 != nullptr) || nf < 2) {
+    const Nbr none{};
+    for (int ip = lane; ip < nf; ip += 64) record_point(A, r, ip, nf, none);
+    return;
+  }
+  const double4* F = A.geo2 + (size_t)r * A.cap;
+  for (int base = 0; base < nf; base += 62) {  // uniform across the wave
+    const int ip = base - 1 + lane;
+    Nbr nb{};
+    double z = 0;
+    if (ip >= 0 && ip < nf) {
+      const double4 g = F[ip];
+      geo2cart(g.x, g.y, g.z, nb.xo);
+      z = g.x;
+    }
+    for (int i = 0; i < 3; i++) {
+      nb.xp[i] = __shfl_up(nb.xo[i], 1, 64);
+      nb.xn[i] = __shfl_down(nb.xo[i], 1, 64);
+    }
+    nb.zp = __shfl_up(z, 1, 64);
+    if (lane >= 1 && lane <= 62 && ip < nf) record_point(A, r, ip, nf, nb);
+  }
 }
 
 // March-order key of each ray from its straight line observer -> view
