@@ -33,11 +33,14 @@ struct EC {
   float e, c;
 };
 
-// GSL_MAX(GSL_MIN(eps, 1), 0) on the pair; NaN becomes 1 like the reference.
+// GSL_MAX(GSL_MIN(eps, 1), 0) on the pair; NaN becomes 1 like the reference:
+// eps -> NaN ? 1 : eps <= 0 ? 0 : 1 - eps <= 0 ? 1 : min(eps, 1), and the
+// complement to match.  Written with the NaN test last (a NaN eps fails
+// eps <= 0, and fminf(NaN, 1) is 1), which needs no NaN-dependent constants.
 __device__ inline EC clamp_ec(float e, float c) {
-  const bool nan = !(e == e), lo = e <= 0.f, hi = c <= 0.f;
-  const float re = nan ? 1.f : (lo ? 0.f : (hi ? 1.f : (e < 1.f ? e : 1.f)));
-  const float rc = nan ? 0.f : (lo ? 1.f : (hi ? 0.f : c));
+  const bool lo = e <= 0.f, hi = c <= 0.f, nan = !(e == e);
+  const float re = lo ? 0.f : (hi ? 1.f : fminf(e, 1.f));
+  const float rc = lo ? 1.f : ((hi || nan) ? 0.f : c);
   return {re, rc};
 }
 
