@@ -699,7 +699,10 @@ __device__ inline int locate_colx(const C& c, int n, X x) {
 // together, one load round per step for the four of them (a wave takes a
 // step when any of its lanes needs it), three steps at most; what is still
 // off then, and NaN, is found by bisection.  Returns the reference's
-// bracket (the unique one in a strictly increasing column).
+// bracket (the unique one in a strictly increasing column).  Lanes that are
+// not live step too (within [0, n - 2], their results unused) but neither
+// ask for a round nor go to the bisection: no per-lane branch around the
+// step tests.
 template <bool KEY_EPS, class Col, class X>
 __device__ inline void lockstep(const Col (&col)[4], const int (&n)[4], const X (&x)[4], bool live, int (&h)[4],
                                 float4 (&q)[4]) {
@@ -707,8 +710,8 @@ __device__ inline void lockstep(const Col (&col)[4], const int (&n)[4], const X 
   bool any = false;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    d[k] = live ? step_dir<KEY_EPS>(q[k], h[k], n[k], x[k]) : 0;
-    any = any || d[k] != 0;
+    d[k] = step_dir<KEY_EPS>(q[k], h[k], n[k], x[k]);
+    any = any || (live && d[k] != 0);
   }
 #pragma unroll
   for (int s = 0; s < 3; s++) {
@@ -723,8 +726,8 @@ __device__ inline void lockstep(const Col (&col)[4], const int (&n)[4], const X 
     any = false;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      d[k] = live ? step_dir<KEY_EPS>(q[k], h[k], n[k], x[k]) : 0;
-      any = any || d[k] != 0;
+      d[k] = step_dir<KEY_EPS>(q[k], h[k], n[k], x[k]);
+      any = any || (live && d[k] != 0);
     }
   }
   bool slow = any;
