@@ -939,7 +939,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
           const bool sat = tp[c][g] < 1e-9f;
           const bool live = gok && !sat && nn[c][0] >= 2 && nn[c][1] >= 2 && nn[c][2] >= 2 && nn[c][3] >= 2;
           const float e = ep[c][g];
-          const int h0[4] = {hh[c][0], hh[c][1], hh[c][2], hh[c][3]};
           const float ex[4] = {e, e, e, e};
           const ColP cols[4] = {{{ur, bb[c][0]}}, {{ur, bb[c][1]}}, {{ur, bb[c][2]}}, {{ur, bb[c][3]}}};
           lockstep<true>(cols, nn[c], ex, live, hh[c], qa[c]);
@@ -951,9 +950,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
           }
           // the forward search starts in the inverse bracket, in place
           lockstep<false>(cols, nn[c], ux, live, hh[c], qa[c]);
-          if (hh[c][0] != h0[0] || hh[c][1] != h0[1] || hh[c][2] != h0[2] || hh[c][3] != h0[3])
-            hints[(c * NGT + g) * 64] = make_uint2((unsigned)hh[c][0] | ((unsigned)hh[c][1] << 16),
-                                                   (unsigned)hh[c][2] | ((unsigned)hh[c][3] << 16));
+          // stored every time (cheaper than testing whether a bracket moved)
+          hints[(c * NGT + g) * 64] = make_uint2((unsigned)hh[c][0] | ((unsigned)hh[c][1] << 16),
+                                                 (unsigned)hh[c][2] | ((unsigned)hh[c][3] << 16));
           EC ec[4];
 #pragma unroll
           for (int k = 0; k < 4; k++) {
