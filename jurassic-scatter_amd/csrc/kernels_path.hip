@@ -884,33 +884,33 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
         const int np_ = gnp[g];
         if (np_ < 2) continue;
         const float du = dus[g];
-        int ipr, it0, it1;
+        int gb0, gb1;  // metadata rows of the (p, T) corners (ipr, it0) and (ipr + 1, it1)
         bool gok;
         float wt0, wt1, wp;
         if (PACK) {
-          ipr = bc.x & 0xff;
-          it0 = (bc.x >> 8) & 0xff;
-          it1 = (bc.x >> 16) & 0xff;
-          gok = (bc.x >> 24) != 0;
+          gb0 = bc.x & 0x7ff;
+          gb1 = (bc.x >> 11) & 0x7ff;
+          gok = (bc.x >> 22) != 0;
           wt0 = __int_as_float(bc.y);
           wt1 = __int_as_float(bc.z);
           wp = __int_as_float(bc.w);
         } else {
           // (p, T) brackets and weights from the staged grids
           const float* pg = sgp + g * NPS;
-          ipr = grid_bracket(pg, np_, p, hpt[g] & 0xff);
+          const int ipr = grid_bracket(pg, np_, p, hpt[g] & 0xff);
           const int nt0 = sgnt[g * NPS + ipr], nt1 = sgnt[g * NPS + ipr + 1];
           gok = nt0 >= 2 && nt1 >= 2;
           const float* t0g = sgt + (g * NPS + ipr) * NTS;
           const float* t1g = t0g + NTS;
-          it0 = grid_bracket(t0g, gok ? nt0 : 2, t, (hpt[g] >> 8) & 0xff);
-          it1 = grid_bracket(t1g, gok ? nt1 : 2, t, (hpt[g] >> 16) & 0xff);
+          const int it0 = grid_bracket(t0g, gok ? nt0 : 2, t, (hpt[g] >> 8) & 0xff);
+          const int it1 = grid_bracket(t1g, gok ? nt1 : 2, t, (hpt[g] >> 16) & 0xff);
           if (gok) hpt[g] = ipr | (it0 << 8) | (it1 << 16);
           wt0 = fdiv(t - t0g[it0], t0g[it0 + 1] - t0g[it0]);
           wt1 = fdiv(t - t1g[it1], t1g[it1 + 1] - t1g[it1]);
           wp = fdiv(p - pg[ipr], pg[ipr + 1] - pg[ipr]);
+          gb0 = ipr * TBLNTMAX + it0;
+          gb1 = (ipr + 1) * TBLNTMAX + it1;
         }
-        const int gb0 = ipr * TBLNTMAX + it0, gb1 = (ipr + 1) * TBLNTMAX + it1;
         // all channels: metadata, hints and the hinted bracket nodes
         int nn[CPW][4], bb[CPW][4], hh[CPW][4];
         float4 qa[CPW][4];
@@ -1013,9 +1013,10 @@ size_t stage_lds_bytes(const PathArgs& A, int ngt, int cpw, int wpb, bool pack =
 // Packed stream of a ray group for k_path_stage<..., PACK>: per group of 64
 // consecutive rays of the sorted list and LOS point slot, the rays' point
 // records lane-contiguous ([R][64] floats) followed by the (p, T) brackets
-// and interpolation weights of every emitter ([ng][64] int4: ipr | it0 << 8
-// | it1 << 16 | gok << 24, wt0, wt1, wp; gok = both pressure rows hold >= 2
-// temperatures) on the shared table grids (forwardmodel.c:442-446 and the
+// and interpolation weights of every emitter ([ng][64] int4: gb0 | gb1 << 11
+// | gok << 22, wt0, wt1, wp; gb0 = ipr TBLNTMAX + it0 and gb1 = (ipr + 1)
+// TBLNTMAX + it1 are the metadata rows of the (p, T) corners, gok = both
+// pressure rows hold >= 2 temperatures) on the shared table grids (forwardmodel.c:442-446 and the
 // weights of the LIN calls at :477-491), computed once per (point, emitter)
 // for all channels.  The searches are the staged kernel's own (grid_bracket:
 // on a strictly increasing grid the unique bracket, whatever the hint) and
@@ -1023,7 +1024,7 @@ size_t stage_lds_bytes(const PathArgs& A, int ngt, int cpw, int wpb, bool pack =
 // y): its 4 waves write point slots 32 y .. 32 y + 31 of the group, a wave
 // one slot's 64 lanes per step; blocks past the group's longest ray leave
 // at once.
-static_assert(TBLNPMAX <= 256 && TBLNTMAX <= 256, "bracket indices are packed in 8 bits");
+static_assert(TBLNPMAX * TBLNTMAX <= 2048, "metadata rows are packed in 11 bits");
 constexpr int PACK_SLOTS = 32;  // point slots per block of k_pack
 __device__ inline void pack_point(const PathArgs& A, int group, int lane, int ray, int ip);
 __global__ void __launch_bounds__(256) k_pack(PathArgs A) {
@@ -1061,7 +1062,8 @@ __device__ inline void pack_point(const PathArgs& A, int group, int lane, int ra
     const float wt0 = fdiv(tt - t0g[it0], t0g[it0 + 1] - t0g[it0]);
     const float wt1 = fdiv(tt - t1g[it1], t1g[it1 + 1] - t1g[it1]);
     const float wp = fdiv(p - pg[ipr], pg[ipr + 1] - pg[ipr]);
-    ob[g * 64] = make_int4(ipr | (it0 << 8) | (it1 << 16) | ((gok ? 1 : 0) << 24), __float_as_int(wt0),
+    const int gb0 = ipr * TBLNTMAX + it0, gb1 = (ipr + 1) * TBLNTMAX + it1;
+    ob[g * 64] = make_int4(gb0 | (gb1 << 11) | ((gok ? 1 : 0) << 22), __float_as_int(wt0),
                            __float_as_int(wt1), __float_as_int(wp));
   }
 }
