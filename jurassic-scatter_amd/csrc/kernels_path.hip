@@ -821,9 +821,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
   const size_t PF = PACK ? (size_t)(A.R + 4 * A.ng) * 64 : (size_t)A.R;  // floats per point slot
   const float* rp = PACK ? A.pack + (size_t)group * A.cap * PF + lane : A.rec + (size_t)ray * A.cap * A.R;
 
-  // the next point's record, loaded one iteration ahead
-  float np_p, np_t, np_ds, np_af, np_qh, np_ai, np_du[NGT];
-  {
+  // without the packed stream: the next point's record, loaded one
+  // iteration ahead (the grid searches need p and T first)
+  float np_p = 0.f, np_t = 0.f, np_ds = 0.f, np_af = 0.f, np_qh = 0.f, np_ai = 0.f, np_du[NGT];
+  if constexpr (!PACK) {
     const float* pr = rp;
     np_p = pr[REC_P * RS];
     np_t = pr[REC_T * RS];
@@ -838,19 +839,38 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
   for (int ip = 0;; ip++) {
     const bool act = ip < npts;
     if (!__any(act)) break;
-    const float p = np_p, t = np_t, ds = np_ds, af = np_af, qh = np_qh, aif = np_ai;
+    float p, t, ds, af, qh, aif;
     float dus[NGT];
     int4 bks[NGT];
-    // precomputed brackets of this point (ip < cap: some lane of the wave
-    // still has points), loaded before the next point's record so that
-    // waiting for them does not wait for the prefetch
-    const int4* bq = PACK ? (const int4*)(rp - lane + (size_t)ip * PF + (size_t)A.R * 64) + lane : nullptr;
+    if constexpr (PACK) {
+      // this point's brackets and record (ip < cap: some lane of the wave
+      // still has points), loaded together: the first use of either, the
+      // metadata loads, needs the brackets, and the record's first use
+      // comes after the node loads, so no prefetch
+      const float* pc = rp + (size_t)ip * PF;
+      const int4* bq = (const int4*)(pc - lane + (size_t)A.R * 64) + lane;
 #pragma unroll
-    for (int g = 0; g < NGT; g++) {
-      dus[g] = np_du[g];
-      bks[g] = PACK && g < A.ng ? bq[g * 64] : make_int4(0, 0, 0, 0);
-    }
-    {
+      for (int g = 0; g < NGT; g++) bks[g] = g < A.ng ? bq[g * 64] : make_int4(0, 0, 0, 0);
+      p = pc[REC_P * RS];
+      t = pc[REC_T * RS];
+      ds = pc[REC_DS * RS];
+      af = pc[REC_AF * RS];
+      qh = pc[REC_QH2O * RS];
+      aif = pc[REC_AI * RS];
+#pragma unroll
+      for (int g = 0; g < NGT; g++) dus[g] = g < A.ng ? pc[(uo + g) * RS] : 0.f;
+    } else {
+      p = np_p;
+      t = np_t;
+      ds = np_ds;
+      af = np_af;
+      qh = np_qh;
+      aif = np_ai;
+#pragma unroll
+      for (int g = 0; g < NGT; g++) {
+        dus[g] = np_du[g];
+        bks[g] = make_int4(0, 0, 0, 0);
+      }
       // prefetch (the last point of the ray reloads itself)
       const int in = ip + 1 < npts ? ip + 1 : (npts > 0 ? npts - 1 : 0);
       const float* pn = rp + (size_t)in * PF;
