@@ -76,7 +76,7 @@ MEASURED_TRAFFIC = {
     ("clear_batch_65536", 65536, 9, 2): (1.5405e10, "profiles/k_path_stage_clear_batch_65536.md"),
     ("clear_batch_65536", 65536, 9, 4): (2.4268e10, "profiles/k_path_stage_clear_batch_65536.md"),
     # variant 10: k_pack and k_path_stage together (the launch the HIP events time)
-    ("clear_batch_65536", 65536, 10, 1): (4.1996e10, "profiles/k_path_stage_clear_batch_65536.md"),
+    ("clear_batch_65536", 65536, 10, 1): (3.8417e10, "profiles/k_path_stage_clear_batch_65536.md"),
 }
 # What binds the path kernel, from rocprofv3 counters of the same
 # configuration (the profile named beside each entry holds the passes):
@@ -84,7 +84,7 @@ MEASURED_TRAFFIC = {
 # vector-memory data-return path (TD_TD_BUSY_sum / (CUs x cycles)), the
 # address path (TA) and the L2 hit rate.
 MEASURED_BINDING = {
-    ("clear_batch_65536", 65536, 10, 1): ({"valu_issue": 1.03, "td_busy": 0.94, "ta_busy": 0.71, "l2_hit": 0.76},
+    ("clear_batch_65536", 65536, 10, 1): ({"valu_issue": 1.00, "td_busy": 0.96, "ta_busy": 0.77, "l2_hit": 0.78},
                                           "profiles/k_path_stage_clear_batch_65536.md"),
     ("clear_batch_65536", 65536, 9, 1): ({"valu_issue": 0.88, "td_busy": 0.92, "ta_busy": 0.66, "l2_hit": 0.96},
                                          "profiles/k_path_stage_clear_batch_65536.md"),
